@@ -1,0 +1,217 @@
+"""Benchmark of the GraSS sampling path on one MI355X.
+
+One step = one pass of the sampling hot path over one batch of synthetic requests on a
+compressed R-MAT graph (a/b/c/d = .57/.19/.19/.05, edge factor 16, seed 42, graph_sort, LE or GT
+compressor), inputs resident in HBM before the timed region:
+  * random-walk requests: walkers U[1, 4096], length U[10, 100], Philox starts;
+  * 2-hop requests (fanout [25, 10], without replacement, per-hop dedup), seed counts
+    lognormal(6, 1.2) clipped to [1, 8192]   [when --khop is on; see DESIGN.md for status].
+`value` is sampled edges/s of the batched path (all requests in persistent launches);
+`plain_*` is the same batch issued one launch per request.  `roofline` is the dominant
+kernel's algorithmic bytes per launch over its HIP-event-timed duration, `cpu_baseline` the
+CPU oracle on a bounded sample of the same requests.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 24] [--fmt gt]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "sampled edges/s on compressed R-MAT-24 (RW + 2-hop), batched vs plain; p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rw_requests(rng, count, n_max_walkers=4096):
+    reqs = []
+    for i in range(count):
+        reqs.append({"walkers": int(rng.integers(1, n_max_walkers + 1)),
+                     "length": int(rng.integers(10, 101)), "instance": i})
+    return reqs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--fmt", default="gt", choices=["le", "gt"])
+    ap.add_argument("--rw-requests", type=int, default=256)
+    ap.add_argument("--threads", type=int, default=16, help="host threads for ingest")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--decode", action="store_true", help="also time the full decode to CSR")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "design is described in DESIGN.md and not measured here.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import grass_amd
+    from grass_amd import ingest
+
+    assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(0)
+    seed = 42
+
+    t0 = time.time()
+    gb, ob, csr = ingest.rmat_graph(args.scale, 16, fmt=args.fmt, seed=seed, threads=args.threads)
+    t_ingest = time.time() - t0
+    log(f"ingest s{args.scale} {args.fmt}: n={csr.n} m={csr.m} graph={len(gb)/1e6:.1f} MB "
+        f"offset={len(ob)/1e6:.1f} MB in {t_ingest:.1f} s")
+    t0 = time.time()
+    g = grass_amd.Graph.from_bytes(gb, ob, args.fmt)
+    t_load = time.time() - t0
+    log(f"load to HBM: {t_load:.2f} s ({(len(gb) + len(ob)) / t_load / 1e9:.2f} GB/s incl. validation)")
+
+    rng = np.random.default_rng(seed)
+    reqs = rw_requests(rng, args.rw_requests)
+    outs = [torch.empty((r["walkers"], r["length"] + 1), dtype=torch.int32, device=dev) for r in reqs]
+    arr, keep = g.rw_request_array(reqs, outs)
+    rw_edges = sum(r["walkers"] * r["length"] for r in reqs)
+    stream = torch.cuda.current_stream()
+    sptr = stream.cuda_stream
+    L = grass_amd.lib()
+
+    def batched_step():
+        grass_amd._check(L.grass_random_walk_batch(g.handle, arr, len(reqs), seed,
+                                                   grass_amd.C.c_void_p(sptr)), "batch")
+
+    def plain_step():
+        for r, o in zip(reqs, outs):
+            grass_amd._check(L.grass_random_walk(g.handle, None, r["walkers"], r["length"], seed,
+                                                 r["instance"], grass_amd.C.c_void_p(o.data_ptr()),
+                                                 grass_amd.C.c_void_p(sptr)), "plain")
+
+    def timed(fn, steps, warmup, events=False):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        evs = []
+        t = time.perf_counter()
+        for _ in range(steps):
+            if events:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                fn()
+                e1.record(stream)
+                evs.append((e0, e1))
+            else:
+                fn()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
+        ev_ms = [a.elapsed_time(b) for a, b in evs]
+        return wall, ev_ms
+
+    # correctness spot check of the batched path against the plain path (same outputs)
+    batched_step()
+    torch.cuda.synchronize()
+    ref0 = outs[0].clone()
+    plain_step()
+    torch.cuda.synchronize()
+    assert torch.equal(ref0, outs[0]), "batched and plain outputs differ"
+    g.sync()
+
+    wall_b, ev_b = timed(batched_step, args.steps, args.warmup, events=True)
+    ms_step = wall_b / args.steps * 1e3
+    value = rw_edges * args.steps / wall_b
+    log(f"batched: {ms_step:.3f} ms/step, {value/1e9:.3f} G edges/s")
+    plain = None
+    if not args.no_plain:
+        wall_p, _ = timed(plain_step, args.steps, args.warmup)
+        plain = rw_edges * args.steps / wall_p
+        log(f"plain: {wall_p/args.steps*1e3:.3f} ms/step, {plain/1e9:.3f} G edges/s")
+
+    # roofline of the dominant kernel (the persistent random-walk kernel)
+    bytes_launch = g.random_walk_batch_bytes(reqs, seed)
+    kern_ms = float(np.mean(ev_b))
+    achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
+    roof = {"kernel": "rw_batch_kernel", "bound": "hbm", "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "bytes_per_launch": bytes_launch,
+            "bytes_per_edge": round(bytes_launch / rw_edges, 3), "launch_ms": round(kern_ms, 4)}
+
+    extra = {}
+    if args.decode:
+        rowptr = torch.empty(g.n + 1, dtype=torch.int64, device=dev)
+        col = torch.empty(g.m, dtype=torch.int32, device=dev)
+
+        def dec():
+            grass_amd._check(L.grass_decode_csr(g.handle, grass_amd.C.c_void_p(rowptr.data_ptr()),
+                                                grass_amd.C.c_void_p(col.data_ptr()),
+                                                grass_amd.C.c_void_p(sptr)), "decode")
+        _, ev_d = timed(dec, 3, 1, events=True)
+        dms = float(np.mean(ev_d))
+        dbytes = len(gb) + len(ob) + 8 * (g.n + 1) + 4 * g.m
+        extra["decode"] = {"ms": round(dms, 3), "decoded_bytes_per_s": (len(gb) + len(ob)) / (dms * 1e-3),
+                           "algorithmic_GBps": round(dbytes / (dms * 1e-3) / 1e9, 2),
+                           "frac": round(dbytes / (dms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"decode CSR: {dms:.3f} ms  {extra['decode']}")
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        og = oracle.Graph(gb, ob, args.fmt)
+        cores = min(16, os.cpu_count() or 1)
+        # bounded sample: a prefix of the same requests, sized to ~cpu-seconds of work
+        probe = reqs[:4]
+        t = time.perf_counter()
+        pe = 0
+        for r in probe:
+            og.random_walk(r["walkers"], r["length"], seed, instance=r["instance"], nthreads=cores)
+            pe += r["walkers"] * r["length"]
+        rate = pe / (time.perf_counter() - t)
+        budget = rate * args.cpu_seconds
+        sample, se = [], 0
+        for r in reqs:
+            if se >= budget:
+                break
+            sample.append(r)
+            se += r["walkers"] * r["length"]
+        t = time.perf_counter()
+        for r in sample:
+            og.random_walk(r["walkers"], r["length"], seed, instance=r["instance"], nthreads=cores)
+        dt = time.perf_counter() - t
+        cpu = {"value": se / dt, "unit": "edges/s", "cores": cores, "kind": "port",
+               "sample": f"first {len(sample)} of the {len(reqs)} random-walk requests "
+                         f"({se} sampled edges), oracle C restatement, OpenMP over walkers"}
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32",
+        "data": f"synthetic R-MAT s{args.scale} ef16 (.57,.19,.19,.05) seed 42, graph_sort, "
+                f"{args.fmt.upper()} compressor; Philox-drawn walk starts",
+        "config": {"workload": f"batched random-walk requests on R-MAT-{args.scale} "
+                               f"({args.fmt.upper()})", "scale": args.scale, "edge_factor": 16,
+                   "format": args.fmt, "n": g.n, "m": g.m, "rw_requests": len(reqs),
+                   "rw_edges_per_step": rw_edges, "khop_requests": 0},
+        "plain_value": plain, "batched_over_plain": (value / plain) if plain else None,
+        "p99_ms": None,
+        "roofline": roof, "cpu_baseline": cpu,
+        "ingest_s": round(t_ingest, 2), "load_s": round(t_load, 3),
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+    g.close()
+
+
+if __name__ == "__main__":
+    main()

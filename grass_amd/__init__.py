@@ -1,0 +1,272 @@
+"""grass_amd: MI355X-native sampler for GraSS compressed graphs.
+
+Python mirror of the C ABI in include/grass.h (libgrass_amd.so, loaded with ctypes).  Device
+buffers are torch tensors (plumbing only: allocation, streams and copies); every kernel is the
+library's own HIP code.  There is no CPU fallback: without the built library, or without a
+GPU, the calls raise.
+
+The reference exposes its path through the rw / ks / mrw apps (`-g <cgc_path> [-n runs]`) and
+App templates; the names below follow that surface:
+  Graph.open(prefix, fmt)           loadGraph + transferToGPUs   (LE/src/grass.cu:1015-1113)
+  Graph.decode_csr()                BCGVertex::get_vertex, all k (LE/src/bcg.cuh:196-261)
+  Graph.random_walk(...)            DeepWalkApp (LE/src/apps/randomwalk/randomWalks.cu:62-79)
+  Graph.random_walk_batch(...)      many random-walk requests, one persistent launch
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgrass_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "grass.h")
+
+LE, GT = 0, 1
+_FMT = {"le": LE, "gt": GT, LE: LE, GT: GT}
+
+OK, EINVAL, EIO, EFORMAT, ENOMEM, EDEVICE, ECAPACITY, ECANCELLED, ETIMEOUT, ENOTREADY = range(10)
+
+
+class GrassError(RuntimeError):
+    def __init__(self, status, what):
+        self.status = status
+        super().__init__(f"{what}: {strerror(status)} (status {status})")
+
+
+class _Info(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("m", C.c_uint64), ("max_degree", C.c_uint64),
+                ("graph_bytes", C.c_uint64), ("offset_bytes", C.c_uint64),
+                ("total_bits", C.c_uint64), ("ubl", C.c_uint32), ("format", C.c_int32),
+                ("device", C.c_int32), ("device_bytes", C.c_uint64)]
+
+
+class RwRequest(C.Structure):
+    _fields_ = [("walkers", C.c_uint64), ("length", C.c_uint32), ("instance", C.c_uint32),
+                ("starts_dev", C.c_void_p), ("out_dev", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libgrass_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C grass_amd` "
+                               "(or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        P, U64, U32, I32 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int
+        sig = {
+            "grass_strerror": (C.c_char_p, [I32]),
+            "grass_open": (I32, [C.c_char_p, I32, I32, P]),
+            "grass_open_memory": (I32, [P, U64, P, U64, I32, I32, P]),
+            "grass_close": (I32, [P]),
+            "grass_info": (I32, [P, P]),
+            "grass_device_arrays": (I32, [P, P, P, P]),
+            "grass_sync": (I32, [P, P]),
+            "grass_decode_csr": (I32, [P, P, P, P]),
+            "grass_random_walk": (I32, [P, P, U64, U32, U64, U32, P, P]),
+            "grass_random_walk_batch": (I32, [P, P, U32, U64, P]),
+            "grass_random_walk_bytes": (I32, [P, P, U64, U32, U64, U32, P, P]),
+            "grass_random_walk_batch_bytes": (I32, [P, P, U32, U64, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def strerror(status):
+    return lib().grass_strerror(status).decode()
+
+
+def _check(rc, what):
+    if rc != OK:
+        raise GrassError(rc, what)
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("grass_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return _torch().cuda.current_stream().cuda_stream
+    return stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+
+
+def _dptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def to_numpy_u32(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+class Graph:
+    """A compressed graph resident in HBM."""
+
+    def __init__(self, handle, device):
+        self._h = handle
+        self.device = device
+        info = _Info()
+        _check(lib().grass_info(self._h, C.byref(info)), "grass_info")
+        self.n = info.n
+        self.m = info.m
+        self.max_degree = info.max_degree
+        self.graph_bytes = info.graph_bytes
+        self.offset_bytes = info.offset_bytes
+        self.total_bits = info.total_bits
+        self.ubl = info.ubl
+        self.fmt = info.format
+        self.device_bytes = info.device_bytes
+
+    @classmethod
+    def open(cls, prefix, fmt="le", device=0):
+        h = C.c_void_p()
+        _check(lib().grass_open(os.fsencode(prefix), _FMT[fmt], device, C.byref(h)), "grass_open")
+        return cls(h, device)
+
+    @classmethod
+    def from_bytes(cls, graph_bytes, offset_bytes, fmt="le", device=0):
+        gb = np.ascontiguousarray(np.frombuffer(graph_bytes, dtype=np.uint8))
+        ob = np.ascontiguousarray(np.frombuffer(offset_bytes, dtype=np.uint8))
+        h = C.c_void_p()
+        _check(lib().grass_open_memory(gb.ctypes.data, gb.nbytes, ob.ctypes.data, ob.nbytes,
+                                       _FMT[fmt], device, C.byref(h)), "grass_open_memory")
+        return cls(h, device)
+
+    def close(self):
+        if self._h:
+            lib().grass_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def sync(self, stream=None):
+        _check(lib().grass_sync(self._h, C.c_void_p(_stream_ptr(stream))), "grass_sync")
+
+    def device_arrays(self):
+        w, s, d = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        _check(lib().grass_device_arrays(self._h, C.byref(w), C.byref(s), C.byref(d)), "arrays")
+        return w.value, s.value, d.value
+
+    def _cuda(self):
+        return f"cuda:{self.device}"
+
+    def decode_csr(self, with_col=True, stream=None):
+        """Full decode; returns host (rowptr uint64[n+1], col uint32[m])."""
+        torch = _torch()
+        rowptr = torch.empty(self.n + 1, dtype=torch.int64, device=self._cuda())
+        col = torch.empty(max(self.m, 1), dtype=torch.int32, device=self._cuda()) if with_col else None
+        _check(lib().grass_decode_csr(self._h, _dptr(rowptr), _dptr(col),
+                                      C.c_void_p(_stream_ptr(stream))), "grass_decode_csr")
+        self.sync(stream)
+        rp = rowptr.cpu().numpy().view(np.uint64)
+        c = to_numpy_u32(col)[: self.m] if with_col else None
+        return rp, c
+
+    def random_walk(self, walkers, length, seed, instance=0, starts=None, out=None, stream=None,
+                    sync=True):
+        """Walk matrix [walkers][length+1] as a device int32 tensor (uint32 ids)."""
+        torch = _torch()
+        if out is None:
+            out = torch.empty((walkers, length + 1), dtype=torch.int32, device=self._cuda())
+        st = None
+        if starts is not None:
+            st = starts if hasattr(starts, "data_ptr") else torch.from_numpy(
+                np.ascontiguousarray(starts, dtype=np.uint32).view(np.int32)).to(self._cuda())
+        _check(lib().grass_random_walk(self._h, _dptr(st), walkers, length, seed, instance,
+                                       _dptr(out), C.c_void_p(_stream_ptr(stream))),
+               "grass_random_walk")
+        if sync:
+            self.sync(stream)
+        return out
+
+    def random_walk_bytes(self, walkers, length, seed, instance=0, starts=None, stream=None):
+        """Algorithmic HBM bytes of the walks random_walk would produce."""
+        torch = _torch()
+        st = None
+        if starts is not None:
+            st = starts if hasattr(starts, "data_ptr") else torch.from_numpy(
+                np.ascontiguousarray(starts, dtype=np.uint32).view(np.int32)).to(self._cuda())
+        nbytes = C.c_uint64()
+        _check(lib().grass_random_walk_bytes(self._h, _dptr(st), walkers, length, seed, instance,
+                                             C.c_void_p(_stream_ptr(stream)), C.byref(nbytes)),
+               "grass_random_walk_bytes")
+        return nbytes.value
+
+    def rw_request_array(self, requests, outs=None):
+        """ctypes array of grass_rw_request (keeps device buffers alive in the returned list)."""
+        torch = _torch()
+        keep, arr = [], (RwRequest * max(len(requests), 1))()
+        for i, r in enumerate(requests):
+            st = r.get("starts")
+            if st is not None and not hasattr(st, "data_ptr"):
+                st = torch.from_numpy(np.ascontiguousarray(st, dtype=np.uint32).view(np.int32)).to(
+                    self._cuda())
+            keep.append(st)
+            o = None if outs is None else outs[i]
+            arr[i] = RwRequest(r["walkers"], r["length"], r.get("instance", i),
+                               None if st is None else st.data_ptr(),
+                               None if o is None else o.data_ptr())
+        return arr, keep
+
+    def random_walk_batch_bytes(self, requests, seed, stream=None):
+        arr, keep = self.rw_request_array(requests)
+        nbytes = C.c_uint64()
+        _check(lib().grass_random_walk_batch_bytes(self._h, arr, len(requests), seed,
+                                                   C.c_void_p(_stream_ptr(stream)),
+                                                   C.byref(nbytes)),
+               "grass_random_walk_batch_bytes")
+        return nbytes.value
+
+    def random_walk_batch(self, requests, seed, stream=None, sync=True):
+        """requests: list of dicts {walkers, length, instance, starts (optional)}.
+        Returns the list of device walk matrices."""
+        torch = _torch()
+        outs, keep, arr = [], [], (RwRequest * max(len(requests), 1))()
+        for i, r in enumerate(requests):
+            o = torch.empty((r["walkers"], r["length"] + 1), dtype=torch.int32, device=self._cuda())
+            st = r.get("starts")
+            if st is not None and not hasattr(st, "data_ptr"):
+                st = torch.from_numpy(np.ascontiguousarray(st, dtype=np.uint32).view(np.int32)).to(
+                    self._cuda())
+            keep.append(st)
+            outs.append(o)
+            arr[i] = RwRequest(r["walkers"], r["length"], r.get("instance", i),
+                               None if st is None else st.data_ptr(), o.data_ptr())
+        _check(lib().grass_random_walk_batch(self._h, arr, len(requests), seed,
+                                             C.c_void_p(_stream_ptr(stream))),
+               "grass_random_walk_batch")
+        if sync:
+            self.sync(stream)
+        return outs
+
+
+def header_functions(path=HEADER):
+    """Names of the functions include/grass.h declares."""
+    import re
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(grass_[a-z_0-9]+)\s*\(", text)))

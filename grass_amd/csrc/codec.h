@@ -1,0 +1,259 @@
+// Device codec for the GraSS compressed graph (LE and GT variants).
+//
+// The .graph file is one MSB-first bit stream: the records of vertices 0..n-1 back to back.
+// In HBM it is held as big-endian 32-bit words (word k = bytes 4k..4k+3, first byte in the
+// top bits), the view the reference builds on the host (LE/src/grass.cu:1036-1050), padded
+// with kPadWords zero words so that every window read below stays inside the allocation.
+//
+// Record layout (LE/chunk_compress/compressor.hpp:288-390, GT/chunk_compress/compressor.hpp:
+// 245-349), header fields are Elias-gamma codes of value+1:
+//   LE: g(d) [g(rock_bit) [g(sea_bit) g(first_pos)]] rocks... seas...
+//       d = 1 stores the neighbour itself in the rock_bit slot.
+//   GT: g(d) g(head_bit) g(body_bit) g(first_pos) then per block: head, bodies (interleaved).
+// The header parse follows LE/src/bcg.cuh:131-152 and GT/src/bcg.cuh:110-136; random access
+// follows LE/src/bcg.cuh:196-261 and GT/src/bcg.cuh:139-168.  Chunk geometry uses an exact
+// integer square root (the reference decoder's sqrt((float)x) disagrees with its encoder's
+// sqrt(double) for degrees above ~8.4M).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace grass {
+
+enum Format : int { kLE = 0, kGT = 1 };
+
+constexpr int kPadWords = 16;  // zero words after the last .graph word (a header parse may run
+                                // <= 252 bits past a corrupt record start before the validator stops)
+
+struct DevGraph {
+    const uint32_t* __restrict__ w;      // big-endian word view of .graph, padded
+    const uint64_t* __restrict__ start;  // bit start of record v, n + 1 entries
+    const uint32_t* __restrict__ deg;    // degree of v, decoded and validated at load
+    uint32_t n;
+    int fmt;
+};
+
+// Decoded record header.  f1/f2 = rock_bit/sea_bit (LE) or head_bit/body_bit (GT).
+struct Hdr {
+    uint64_t pay;  // bit where the payload starts
+    uint32_t d;
+    uint32_t f1, f2, fp;
+    uint32_t C, q, R;  // LE: chunks, q = floor((d-1)/C); GT: blocks, q = floor(d/C); R remainder
+    uint32_t hbits;    // header length in bits
+};
+
+__host__ __device__ __forceinline__ uint32_t isqrt_u32(uint32_t x) {
+    uint32_t s = (uint32_t)sqrtf((float)x);
+    while ((uint64_t)s * s > x) --s;
+    while ((uint64_t)(s + 1) * (s + 1) <= x) ++s;
+    return s;
+}
+
+// Chunk geometry of an LE record with d >= 2 (LE/chunk_compress/compressor.hpp:61-67).
+__host__ __device__ __forceinline__ void le_geometry(uint32_t d, uint32_t& C, uint32_t& q,
+                                                     uint32_t& R) {
+    uint32_t D = d - 1, s = isqrt_u32(D), r = D - s * s;
+    C = s + (r > s);
+    q = D / C;
+    R = D - C * q;
+}
+
+// Block geometry of a GT record (GT/chunk_compress/compressor.hpp:49-62).
+__host__ __device__ __forceinline__ void gt_geometry(uint32_t d, uint32_t& C, uint32_t& q,
+                                                     uint32_t& R) {
+    uint32_t s = isqrt_u32(d);
+    if (s * s == d) {
+        C = q = s;
+        R = 0;
+    } else {
+        C = s + 1;
+        q = d / C;
+        R = d - C * q;
+    }
+}
+
+// 32-bit window at bit b (the funnel shift of LE/src/bcg.cuh:66-72).
+__device__ __forceinline__ uint32_t peek32(const uint32_t* __restrict__ w, uint64_t b) {
+    const uint32_t* p = w + (b >> 5);
+    uint64_t x = ((uint64_t)p[0] << 32) | p[1];
+    return (uint32_t)(x >> (32 - (uint32_t)(b & 31)));
+}
+
+// nbits in [1, 32].
+__device__ __forceinline__ uint32_t read_bits(const uint32_t* __restrict__ w, uint64_t b,
+                                              uint32_t nbits) {
+    return peek32(w, b) >> (32 - nbits);
+}
+
+// nbits in [0, 32].
+__device__ __forceinline__ uint32_t read_bits0(const uint32_t* __restrict__ w, uint64_t b,
+                                               uint32_t nbits) {
+    return nbits ? read_bits(w, b, nbits) : 0u;
+}
+
+// Elias-gamma value at b, b advanced past it (LE/src/bcg.cuh:74-98).  Valid input only: the
+// load-time validator has checked every header of the graph.
+__device__ __forceinline__ uint32_t gamma_at(const uint32_t* __restrict__ w, uint64_t& b) {
+    uint32_t L = __clz(peek32(w, b));
+    uint32_t y = peek32(w, b + L);
+    b += 2 * L + 1;
+    return (y >> (31 - L)) - 1u;
+}
+
+// Checked gamma for the validator: false on an all-zero 32-bit window (a code the
+// reference's 32-bit clz cannot represent).
+__device__ __forceinline__ bool gamma_checked(const uint32_t* __restrict__ w, uint64_t& b,
+                                              uint32_t& val) {
+    uint32_t x = peek32(w, b);
+    if (x == 0) return false;
+    uint32_t L = __clz(x);
+    uint32_t y = peek32(w, b + L);
+    b += 2 * L + 1;
+    val = (y >> (31 - L)) - 1u;
+    return true;
+}
+
+template <int FMT>
+__device__ __forceinline__ void parse_header(const uint32_t* __restrict__ w, uint64_t b, Hdr& h) {
+    uint64_t b0 = b;
+    h.d = gamma_at(w, b);
+    h.f1 = h.f2 = h.fp = h.C = h.q = h.R = 0;
+    if (FMT == kLE) {
+        if (h.d >= 1) h.f1 = gamma_at(w, b);
+        if (h.d >= 2) {
+            h.f2 = gamma_at(w, b);
+            h.fp = gamma_at(w, b);
+            le_geometry(h.d, h.C, h.q, h.R);
+        }
+    } else {
+        h.f1 = gamma_at(w, b);
+        h.f2 = gamma_at(w, b);
+        h.fp = gamma_at(w, b);
+        if (h.d) gt_geometry(h.d, h.C, h.q, h.R);
+    }
+    h.pay = b;
+    h.hbits = (uint32_t)(b - b0);
+}
+
+// Payload length implied by a header; the validator requires header + payload == record.
+__host__ __device__ __forceinline__ uint64_t payload_bits(int fmt, const Hdr& h) {
+    if (fmt == kLE) {
+        if (h.d <= 1) return 0;
+        return (uint64_t)(h.C + 1) * h.f1 + (uint64_t)(h.d - 1 - h.C) * (h.f2 + 1);
+    }
+    if (h.d == 0) return 0;
+    return (uint64_t)h.C * h.f1 + (uint64_t)(h.d - h.C) * h.f2;
+}
+
+// Position k of an LE record -> (chunk c, index i in chunk, chunk length len).
+__device__ __forceinline__ void le_locate(const Hdr& h, uint32_t k, uint32_t& c, uint32_t& i,
+                                          uint32_t& len) {
+    uint32_t big = h.R * (h.q + 1);
+    if (k < big) {
+        len = h.q + 1;
+        c = k / len;
+        i = k - c * len;
+    } else {
+        len = h.q;
+        uint32_t kk = k - big;
+        c = kk / len;
+        i = kk - c * len;
+        c += h.R;
+    }
+}
+
+// Neighbour k of vertex v, LE (LE/src/bcg.cuh:196-261); 0 <= k < d.
+__device__ __forceinline__ uint32_t le_get(const uint32_t* __restrict__ w, const Hdr& h,
+                                           uint32_t v, uint32_t k, uint32_t* fbits = nullptr) {
+    if (h.d == 1) {
+        if (fbits) *fbits = 0;
+        return h.f1;
+    }
+    uint32_t c, i, len;
+    le_locate(h, k, c, i, len);
+    const uint32_t rb = h.f1;
+    uint64_t rbit = h.pay + (uint64_t)c * rb;
+    uint32_t r1 = read_bits(w, rbit, rb);
+    uint32_t a = c < h.fp ? v - r1 : v + r1;
+    if (i == 0) {
+        if (fbits) *fbits = rb;
+        return a;
+    }
+    uint32_t r2 = read_bits(w, rbit + rb, rb);
+    uint32_t z = c + 1 < h.fp ? v - r2 : v + r2;
+    uint32_t j = (h.q - 1) * c + (c < h.R ? c : h.R) + i - 1;
+    uint64_t so = h.pay + (uint64_t)(h.C + 1) * rb + (uint64_t)j * (h.f2 + 1);
+    uint32_t x = read_bits(w, so, h.f2 + 1);
+    uint32_t mag = x >> 1;
+    if (fbits) *fbits = 2 * rb + h.f2 + 1;
+    return (z - a) / len * i + a + ((x & 1) ? 0u - mag : mag);
+}
+
+// Position k of a GT record -> (block, index i in block).
+__device__ __forceinline__ void gt_locate(const Hdr& h, uint32_t k, uint32_t& blk, uint32_t& i) {
+    uint32_t big = (h.q + 1) * h.R;
+    if (k < big) {
+        blk = k / (h.q + 1);
+        i = k - blk * (h.q + 1);
+    } else {
+        uint32_t kk = k - big;
+        uint32_t b2 = kk / h.q;
+        i = kk - b2 * h.q;
+        blk = b2 + h.R;
+    }
+}
+
+// Bit where block blk of a GT record starts (GT/src/bcg.cuh:161).
+__device__ __forceinline__ uint64_t gt_block_bit(const Hdr& h, uint32_t blk) {
+    return h.pay + (uint64_t)blk * (h.f2 * (h.q - 1) + h.f1) +
+           (uint64_t)(blk < h.R ? blk : h.R) * h.f2;
+}
+
+// Absolute value of the head of block blk.
+__device__ __forceinline__ uint32_t gt_head(const uint32_t* __restrict__ w, const Hdr& h,
+                                            uint32_t v, uint32_t blk, uint64_t bo) {
+    uint32_t head = read_bits(w, bo, h.f1);
+    return blk >= h.fp ? v + head + 1 : v - head - 1;
+}
+
+// Sum of `count` consecutive width-bit fields starting at bit b (width in [1, 32]); one load
+// per 32 bits consumed.
+__device__ __forceinline__ uint32_t sum_fields(const uint32_t* __restrict__ w, uint64_t b,
+                                               uint32_t count, uint32_t width) {
+    const uint32_t* p = w + (b >> 5);
+    uint32_t s = (uint32_t)(b & 31);
+    uint64_t buf = (((uint64_t)p[0] << 32) | p[1]) << s;
+    int avail = 64 - (int)s;
+    p += 2;
+    uint32_t sum = 0;
+    for (uint32_t t = 0; t < count; ++t) {
+        if (avail < (int)width) {
+            buf |= (uint64_t)(*p++) << (32 - avail);
+            avail += 32;
+        }
+        sum += (uint32_t)(buf >> (64 - width));
+        buf <<= width;
+        avail -= (int)width;
+    }
+    return sum;
+}
+
+// Neighbour k of vertex v, GT (GT/src/bcg.cuh:154-168): head plus i gaps of (body + 1).
+__device__ __forceinline__ uint32_t gt_get(const uint32_t* __restrict__ w, const Hdr& h,
+                                           uint32_t v, uint32_t k, uint32_t* fbits = nullptr) {
+    uint32_t blk, i;
+    gt_locate(h, k, blk, i);
+    uint64_t bo = gt_block_bit(h, blk);
+    uint32_t val = gt_head(w, h, v, blk, bo);
+    if (i) val += sum_fields(w, bo + h.f1, i, h.f2) + i;
+    if (fbits) *fbits = h.f1 + i * h.f2;
+    return val;
+}
+
+template <int FMT>
+__device__ __forceinline__ uint32_t get_neighbor(const uint32_t* __restrict__ w, const Hdr& h,
+                                                 uint32_t v, uint32_t k) {
+    return FMT == kLE ? le_get(w, h, v, k) : gt_get(w, h, v, k);
+}
+
+}  // namespace grass

@@ -1,0 +1,125 @@
+// Full decode of the resident compressed graph to CSR (rowptr u64[n+1], col u32[m]).
+//
+// Edge-parallel: each wave64 owns a contiguous range of kEdgesPerWave output slots, so writes
+// are coalesced and work is balanced whatever the degree skew.  A lane finds the vertex of its
+// slot by binary search of rowptr inside the wave's vertex range, parses that vertex's header
+// (lanes of one vertex read the same words) and decodes its element:
+//   LE: O(1) random access (LE/src/bcg.cuh:196-261);
+//   GT: the head of each block is absolute and every body is a gap, so the wave turns
+//       (head | gap+1) values into ids with a segmented prefix sum, carrying the running id
+//       from lane 63 into the next 64 slots (the cooperative decode of GT/src/bcg.cuh:171-345
+//       and GT/src/grass.cu:525-553, without its per-warp cache limits).  A wave whose range
+//       starts inside a block first backs up to that block's head.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "devutil.h"
+#include "runtime.h"
+#include "scan.h"
+
+namespace grass {
+
+constexpr int kDecBlock = 256;
+constexpr uint64_t kEdgesPerWave = 2048;
+
+struct DegGen {
+    const uint32_t* deg;
+    __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return deg[i]; }
+};
+
+// Largest v in [lo, hi] with rowptr[v] <= e.
+__device__ __forceinline__ uint32_t owner_of(const uint64_t* __restrict__ rowptr, uint32_t lo,
+                                             uint32_t hi, uint64_t e) {
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo + 1) / 2;
+        if (rowptr[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int FMT>
+__global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
+                                                               const uint64_t* __restrict__ rowptr,
+                                                               uint32_t* __restrict__ col,
+                                                               uint64_t m) {
+    const uint64_t wave = ((uint64_t)blockIdx.x * kDecBlock + threadIdx.x) / kWave;
+    const int lane = lane_id();
+    const uint64_t e0 = wave * kEdgesPerWave;
+    if (e0 >= m) return;  // uniform per wave
+    const uint64_t e1 = e0 + kEdgesPerWave < m ? e0 + kEdgesPerWave : m;
+    const uint32_t vlo = owner_of(rowptr, 0, g.n - 1, e0);
+    const uint32_t vhi = owner_of(rowptr, vlo, g.n - 1, e1 - 1);
+    uint64_t eb = e0;
+    if (FMT == kGT) {
+        Hdr h;
+        parse_header<kGT>(g.w, g.start[vlo], h);
+        uint32_t blk, i;
+        gt_locate(h, (uint32_t)(e0 - rowptr[vlo]), blk, i);
+        eb = e0 - i;
+    }
+    uint32_t carry = 0;
+    for (uint64_t base = eb; base < e1; base += kWave) {
+        const uint64_t e = base + lane;
+        const bool act = e < e1;
+        const uint64_t ee = act ? e : e1 - 1;
+        const uint32_t v = owner_of(rowptr, vlo, vhi, ee);
+        const uint32_t k = (uint32_t)(ee - rowptr[v]);
+        Hdr h;
+        parse_header<FMT>(g.w, g.start[v], h);
+        if (FMT == kLE) {
+            uint32_t x = le_get(g.w, h, v, k);
+            if (act && e >= e0) col[e] = x;
+        } else {
+            uint32_t blk, i;
+            gt_locate(h, k, blk, i);
+            const uint64_t bo = gt_block_bit(h, blk);
+            uint32_t x;
+            int head;
+            if (i == 0) {
+                x = gt_head(g.w, h, v, blk, bo);
+                head = 1;
+            } else {
+                x = read_bits(g.w, bo + h.f1 + (uint64_t)(i - 1) * h.f2, h.f2) + 1;
+                head = 0;
+            }
+            if (lane == 0 && !head) {
+                x += carry;
+                head = 1;
+            }
+            x = wave_segmented_sum(x, head);
+            carry = __shfl(x, kWave - 1, kWave);
+            if (act && e >= e0) col[e] = x;
+        }
+    }
+}
+
+int decode_csr(grass_graph* g, uint64_t* rowptr, uint32_t* col, hipStream_t s) {
+    const uint64_t n = g->n, m = g->m;
+    const uint64_t nb = scan_blocks(n);
+    {
+        std::lock_guard<std::mutex> lock(g->mu);
+        hipError_t e = g->scratch.reserve((nb + 4) * 8);
+        if (e != hipSuccess) return GRASS_ENOMEM;
+        uint64_t* part = (uint64_t*)g->scratch.ptr;
+        e = exclusive_scan(DegGen{g->d_deg}, Count{n, nullptr}, n, rowptr, part, part + nb, s);
+        if (e != hipSuccess) return status_of(e);
+        // the scratch is reused by the next call on any stream: finish the scan first
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return status_of(e);
+    }
+    if (m == 0 || !col) return GRASS_OK;
+    const uint64_t waves = (m + kEdgesPerWave - 1) / kEdgesPerWave;
+    const uint64_t blocks = (waves + kDecBlock / kWave - 1) / (kDecBlock / kWave);
+    if (blocks > 0x7fffffffull) return GRASS_EINVAL;
+    if (g->fmt == kLE)
+        hipLaunchKernelGGL(decode_csr_kernel<kLE>, dim3((unsigned)blocks), dim3(kDecBlock), 0, s,
+                           g->dev(), (const uint64_t*)rowptr, col, m);
+    else
+        hipLaunchKernelGGL(decode_csr_kernel<kGT>, dim3((unsigned)blocks), dim3(kDecBlock), 0, s,
+                           g->dev(), (const uint64_t*)rowptr, col, m);
+    return status_of(hipGetLastError());
+}
+
+}  // namespace grass

@@ -1,0 +1,102 @@
+// Wave64 / workgroup primitives used by the kernels (CDNA4: 64-lane wavefronts).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace grass {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_sum(T x) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        T y = __shfl_up(x, off, kWave);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, kWave);
+    return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T x) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        T y = __shfl_xor(x, off, kWave);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
+// Segmented inclusive sum over the wave: a lane with head != 0 starts a new segment.
+__device__ __forceinline__ uint32_t wave_segmented_sum(uint32_t x, int head) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        uint32_t px = __shfl_up(x, off, kWave);
+        int ph = __shfl_up(head, off, kWave);
+        if (lane >= off && !head) {
+            x += px;
+            head = ph;
+        }
+    }
+    return x;
+}
+
+// Exclusive sum over a workgroup of BLOCK threads; `total` receives the workgroup sum.
+// `lds` holds BLOCK / 64 elements.  Every thread of the workgroup must call it.
+template <typename T, int BLOCK>
+__device__ __forceinline__ T block_exclusive_sum(T x, T& total, T* lds) {
+    static_assert(BLOCK % kWave == 0, "block must be whole waves");
+    T inc = wave_inclusive_sum(x);
+    const int wid = threadIdx.x / kWave;
+    if (lane_id() == kWave - 1) lds[wid] = inc;
+    __syncthreads();
+    T pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < BLOCK / kWave; ++i) {
+        T t = lds[i];
+        pre += (i < wid) ? t : T(0);
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return pre + inc - x;
+}
+
+template <typename T, int BLOCK>
+__device__ __forceinline__ T block_sum(T x, T* lds) {
+    T s = wave_sum(x);
+    const int wid = threadIdx.x / kWave;
+    if (lane_id() == 0) lds[wid] = s;
+    __syncthreads();
+    T tot = 0;
+#pragma unroll
+    for (int i = 0; i < BLOCK / kWave; ++i) tot += lds[i];
+    __syncthreads();
+    return tot;
+}
+
+template <typename T, int BLOCK>
+__device__ __forceinline__ T block_max(T x, T* lds) {
+    T s = wave_max(x);
+    const int wid = threadIdx.x / kWave;
+    if (lane_id() == 0) lds[wid] = s;
+    __syncthreads();
+    T m = lds[0];
+#pragma unroll
+    for (int i = 1; i < BLOCK / kWave; ++i) m = lds[i] > m ? lds[i] : m;
+    __syncthreads();
+    return m;
+}
+
+}  // namespace grass

@@ -1,0 +1,255 @@
+// Compressed-graph loader: files -> pinned host staging -> HBM, then on the device:
+//   1. byte swap of the .graph and .offset images into big-endian word views,
+//   2. record-length extraction from .offset fused into a reduce-then-scan (bit starts),
+//   3. a validator that parses every record header, checks header + payload == record
+//      length, and writes the degree array.
+// Replaces loadGraph (LE/src/grass.cu:1015-1095), copyPartitionToGPU (LE/src/utils.hpp:328-353)
+// with its decode_offset kernel (LE/src/bcg.cuh:284-329) and CUB scan.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "devutil.h"
+#include "runtime.h"
+#include "scan.h"
+
+namespace grass {
+
+__global__ void bswap_words_kernel(uint32_t* w, uint64_t nwords) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < nwords; i += stride) w[i] = __builtin_bswap32(w[i]);
+}
+
+// Length field u of .offset: bits [8 + u*ubl, 8 + (u+1)*ubl) of the file, ubl in [1, 64]
+// (LE/src/bcg.cuh:293-318 reads the same field across up to three words).
+struct OffsetFieldGen {
+    const uint32_t* ow;  // big-endian word view of the whole .offset file, padded
+    uint32_t ubl;
+    __device__ __forceinline__ uint64_t operator()(uint64_t u) const {
+        uint64_t b = 8 + u * ubl;
+        const uint32_t* p = ow + (b >> 5);
+        uint32_t s = (uint32_t)(b & 31);
+        uint64_t hi = ((uint64_t)p[0] << 32) | p[1];
+        uint64_t win = s ? (hi << s) | (p[2] >> (32 - s)) : hi;
+        return win >> (64 - ubl);
+    }
+};
+
+constexpr int kValBlock = 256;
+
+// stats: [0] sum of degrees, [1] max degree, [2] malformed records, [3] first malformed vertex
+template <int FMT>
+__global__ __launch_bounds__(kValBlock) void validate_kernel(const uint32_t* __restrict__ w,
+                                                             const uint64_t* __restrict__ start,
+                                                             uint32_t n, uint64_t gbits,
+                                                             uint32_t* __restrict__ deg,
+                                                             unsigned long long* stats) {
+    __shared__ unsigned long long lds[kValBlock / kWave];
+    unsigned long long sum = 0, mx = 0, bad = 0;
+    for (uint32_t v = blockIdx.x * kValBlock + threadIdx.x; v < n; v += gridDim.x * kValBlock) {
+        const uint64_t b0 = start[v], end = start[v + 1];
+        uint64_t b = b0;
+        // Every read below stays within [b0, end + 95] bits, inside the padded word view,
+        // whatever the .offset held (a corrupt one may even wrap the 64-bit scan).
+        bool ok = end > b0 && end <= gbits;
+        Hdr h;
+        h.d = h.f1 = h.f2 = h.fp = h.C = h.q = h.R = 0;
+        ok = ok && gamma_checked(w, b, h.d) && b <= end;
+        if (FMT == kLE) {
+            if (ok && h.d >= 1) ok = gamma_checked(w, b, h.f1) && b <= end;
+            if (ok && h.d == 1) ok = h.f1 < n;
+            if (ok && h.d >= 2) {
+                ok = gamma_checked(w, b, h.f2) && b <= end && gamma_checked(w, b, h.fp) && b <= end;
+                if (ok) {
+                    le_geometry(h.d, h.C, h.q, h.R);
+                    ok = h.f1 >= 1 && h.f1 <= 31 && h.f2 <= 31 && h.fp <= h.C + 1;
+                }
+            }
+        } else {
+            ok = ok && gamma_checked(w, b, h.f1) && b <= end && gamma_checked(w, b, h.f2) &&
+                 b <= end && gamma_checked(w, b, h.fp) && b <= end;
+            if (ok && h.d) {
+                gt_geometry(h.d, h.C, h.q, h.R);
+                ok = h.f1 >= 1 && h.f1 <= 31 && h.f2 <= 31 && h.fp <= h.C &&
+                     (h.f2 >= 1 || h.d == h.C);
+            }
+        }
+        if (ok) {
+            h.pay = b;
+            ok = (b + payload_bits(FMT, h) == end) && h.d < 0x80000000u;
+        }
+        if (!ok) {
+            ++bad;
+            atomicMin(&stats[3], (unsigned long long)v);
+            h.d = 0;
+        }
+        deg[v] = h.d;
+        sum += h.d;
+        mx = h.d > mx ? h.d : mx;
+    }
+    sum = block_sum<unsigned long long, kValBlock>(sum, lds);
+    mx = block_max<unsigned long long, kValBlock>(mx, lds);
+    bad = block_sum<unsigned long long, kValBlock>(bad, lds);
+    if (threadIdx.x == 0) {
+        atomicAdd(&stats[0], sum);
+        atomicMax(&stats[1], mx);
+        if (bad) atomicAdd(&stats[2], bad);
+    }
+}
+
+namespace {
+
+constexpr uint64_t kChunk = 64ull << 20;  // pinned staging chunk
+
+// Streams `src` into device memory through two pinned chunks so that the read of chunk i+1
+// overlaps the DMA of chunk i.
+int stage_to_device(ByteSource& src, char* dst, hipStream_t stream, uint64_t* tail,
+                    unsigned tail_len, uint8_t* tail_out) {
+    const uint64_t total = src.size();
+    if (total == 0) return GRASS_OK;
+    const uint64_t chunk = std::min<uint64_t>(kChunk, total);
+    void* pin[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int rc = GRASS_OK;
+    for (int i = 0; i < 2 && rc == GRASS_OK; ++i) {
+        if (hipHostMalloc(&pin[i], chunk, hipHostMallocDefault) != hipSuccess) rc = GRASS_ENOMEM;
+        else if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess)
+            rc = GRASS_EDEVICE;
+    }
+    uint64_t off = 0;
+    int slot = 0;
+    bool used[2] = {false, false};
+    while (rc == GRASS_OK && off < total) {
+        uint64_t len = std::min(chunk, total - off);
+        if (used[slot] && hipEventSynchronize(ev[slot]) != hipSuccess) { rc = GRASS_EDEVICE; break; }
+        if (!src.read(pin[slot], len)) { rc = GRASS_EIO; break; }
+        // keep the last tail_len bytes of the stream on the host
+        if (tail_out && off + len > total - std::min<uint64_t>(tail_len, total)) {
+            uint64_t t0 = total - std::min<uint64_t>(tail_len, total);
+            for (uint64_t i = std::max(off, t0); i < off + len; ++i)
+                tail_out[i - t0] = ((uint8_t*)pin[slot])[i - off];
+        }
+        if (hipMemcpyAsync(dst + off, pin[slot], len, hipMemcpyHostToDevice, stream) != hipSuccess ||
+            hipEventRecord(ev[slot], stream) != hipSuccess) {
+            rc = GRASS_EDEVICE;
+            break;
+        }
+        used[slot] = true;
+        off += len;
+        slot ^= 1;
+    }
+    if (tail) *tail = std::min<uint64_t>(tail_len, total);
+    if (hipStreamSynchronize(stream) != hipSuccess && rc == GRASS_OK) rc = GRASS_EDEVICE;
+    for (int i = 0; i < 2; ++i) {
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+        if (pin[i]) (void)hipHostFree(pin[i]);
+    }
+    return rc;
+}
+
+// Field u of .offset read on the host from the file's last `tlen` bytes.
+uint64_t host_field(const uint8_t* tailb, uint64_t tlen, uint64_t file_bytes, uint64_t u,
+                    uint32_t ubl) {
+    uint64_t b = 8 + u * ubl;  // bit position in the file
+    uint64_t first = file_bytes - tlen;
+    uint64_t v = 0;
+    for (uint32_t t = 0; t < ubl; ++t, ++b) {
+        uint64_t byte = b >> 3;
+        uint32_t bit = byte >= first && byte < file_bytes ? (tailb[byte - first] >> (7 - (b & 7))) & 1 : 0;
+        v = (v << 1) | bit;
+    }
+    return v;
+}
+
+}  // namespace
+
+int load_graph(grass_graph* g, ByteSource& gsrc, ByteSource& osrc) {
+    const uint64_t gbytes = gsrc.size(), obytes = osrc.size();
+    if (obytes < 2 || gbytes < 1) return GRASS_EFORMAT;
+    hipStream_t s = g->stream;
+    const uint64_t gwords = (gbytes + 3) / 4 + kPadWords;
+    const uint64_t owords = (obytes + 3) / 4 + kPadWords;
+    uint32_t* d_off = nullptr;
+    if (hipMalloc(&g->d_words, gwords * 4) != hipSuccess) return GRASS_ENOMEM;
+    if (hipMalloc(&d_off, owords * 4) != hipSuccess) return GRASS_ENOMEM;
+    int rc = GRASS_OK;
+    auto fail = [&](int code) {
+        (void)hipFree(d_off);
+        return code;
+    };
+    if (hipMemsetAsync(g->d_words, 0, gwords * 4, s) != hipSuccess ||
+        hipMemsetAsync(d_off, 0, owords * 4, s) != hipSuccess)
+        return fail(GRASS_EDEVICE);
+    rc = stage_to_device(gsrc, (char*)g->d_words, s, nullptr, 0, nullptr);
+    if (rc) return fail(rc);
+    uint8_t tailb[64];
+    uint64_t tlen = 0;
+    rc = stage_to_device(osrc, (char*)d_off, s, &tlen, sizeof(tailb), tailb);
+    if (rc) return fail(rc);
+
+    // ubl = byte 0 of .offset; n = floor(8 (bytes - 1) / ubl) (LE/src/grass.cu:1071,1088),
+    // minus trailing zero-length fields, which can only be padding (a record is >= 1 bit).
+    uint8_t ubl = 0;
+    if (hipMemcpy(&ubl, d_off, 1, hipMemcpyDeviceToHost) != hipSuccess) return fail(GRASS_EDEVICE);
+    if (ubl == 0 || ubl > 64) return fail(GRASS_EFORMAT);
+    uint64_t n = ((obytes - 1) * 8) / ubl;
+    uint64_t tail_fields_max = std::min<uint64_t>(n, 7);
+    for (uint64_t t = 0; t < tail_fields_max && n > 0; ++t) {
+        if (8 + (n - 1) * (uint64_t)ubl < 8 * (obytes - tlen)) break;  // field not in the tail
+        if (host_field(tailb, tlen, obytes, n - 1, ubl) != 0) break;
+        --n;
+    }
+    if (n == 0 || n >= 0x80000000ull) return fail(GRASS_EFORMAT);
+    g->n = n;
+    g->ubl = ubl;
+    g->graph_bytes = gbytes;
+    g->offset_bytes = obytes;
+
+    const unsigned bsg = 2048;
+    hipLaunchKernelGGL(bswap_words_kernel, dim3(bsg), dim3(256), 0, s, g->d_words, gwords);
+    hipLaunchKernelGGL(bswap_words_kernel, dim3(bsg), dim3(256), 0, s, d_off, owords);
+
+    if (hipMalloc(&g->d_start, (n + 1) * 8) != hipSuccess) return fail(GRASS_ENOMEM);
+    if (hipMalloc(&g->d_deg, n * 4) != hipSuccess) return fail(GRASS_ENOMEM);
+    const uint64_t nb = scan_blocks(n);
+    uint64_t* d_tmp = nullptr;  // partials[nb], total, stats[4]
+    if (hipMalloc(&d_tmp, (nb + 8) * 8) != hipSuccess) return fail(GRASS_ENOMEM);
+    uint64_t* d_total = d_tmp + nb;
+    unsigned long long* d_stats = (unsigned long long*)(d_tmp + nb + 2);
+    unsigned long long init[4] = {0, 0, 0, ~0ull};
+    hipError_t e = hipMemcpyAsync(d_stats, init, sizeof(init), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = exclusive_scan(OffsetFieldGen{d_off, ubl}, Count{n, nullptr}, n, g->d_start, d_tmp,
+                           d_total, s);
+    if (e == hipSuccess) {
+        unsigned grid = (unsigned)std::min<uint64_t>((n + kValBlock - 1) / kValBlock, 16384);
+        if (g->fmt == kLE)
+            hipLaunchKernelGGL(validate_kernel<kLE>, dim3(grid), dim3(kValBlock), 0, s, g->d_words,
+                               g->d_start, (uint32_t)n, 8 * gbytes, g->d_deg, d_stats);
+        else
+            hipLaunchKernelGGL(validate_kernel<kGT>, dim3(grid), dim3(kValBlock), 0, s, g->d_words,
+                               g->d_start, (uint32_t)n, 8 * gbytes, g->d_deg, d_stats);
+        e = hipGetLastError();
+    }
+    uint64_t total = 0;
+    unsigned long long stats[4] = {0, 0, 0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, d_total, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(stats, d_stats, sizeof(stats), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d_tmp);
+    (void)hipFree(d_off);
+    if (e != hipSuccess) return status_of(e);
+    // records must cover the .graph bits up to < 8 bits of final padding
+    if (total > 8 * gbytes || 8 * gbytes - total >= 8) return GRASS_EFORMAT;
+    if (stats[2] != 0) return GRASS_EFORMAT;
+    g->total_bits = total;
+    g->m = stats[0];
+    g->max_degree = stats[1];
+    g->device_bytes = gwords * 4 + (n + 1) * 8 + n * 4;
+    return GRASS_OK;
+}
+
+}  // namespace grass

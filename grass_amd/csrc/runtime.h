@@ -1,0 +1,113 @@
+// Internal host-side types shared by the HIP translation units of libgrass_amd.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+
+#include "../../include/grass.h"
+#include "codec.h"
+
+namespace grass {
+
+// Device scratch that grows on demand and is reused across calls (no per-call hipMalloc in
+// steady state).  Guarded by the owning graph's mutex.
+struct Scratch {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (ptr) {
+            hipError_t e = hipFree(ptr);
+            if (e != hipSuccess) return e;
+            ptr = nullptr;
+            bytes = 0;
+        }
+        size_t cap = need + need / 4 + (1u << 20);
+        hipError_t e = hipMalloc(&ptr, cap);
+        if (e == hipSuccess) bytes = cap;
+        return e;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+// Bump allocator over a Scratch region (256-byte aligned slices).
+struct Carve {
+    char* base;
+    size_t off = 0;
+    explicit Carve(void* p) : base((char*)p) {}
+    template <typename T>
+    T* take(size_t count) {
+        off = (off + 255) & ~(size_t)255;
+        T* p = (T*)(base + off);
+        off += count * sizeof(T);
+        return p;
+    }
+    static size_t need(size_t bytes) { return ((bytes + 255) & ~(size_t)255) + 256; }
+};
+
+// Error word layout in device memory (kernels record the first problem they meet).
+enum : uint32_t {
+    kErrNone = 0,
+    kErrBadNeighbor = 1,  // a decoded id >= n: the graph decodes to garbage
+    kErrBadStart = 2,     // a caller-supplied vertex id >= n
+};
+
+}  // namespace grass
+
+struct grass_graph {
+    int device = 0;
+    int fmt = 0;
+    uint64_t n = 0, m = 0, max_degree = 0;
+    uint64_t graph_bytes = 0, offset_bytes = 0, total_bits = 0;
+    uint32_t ubl = 0;
+    uint64_t device_bytes = 0;
+    uint32_t* d_words = nullptr;
+    uint64_t* d_start = nullptr;
+    uint32_t* d_deg = nullptr;
+    uint32_t* d_err = nullptr;  // [0] first error kind, [1] detail
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    grass::Scratch scratch;  // per-call working memory (decode, samplers)
+
+    grass::DevGraph dev() const {
+        grass::DevGraph d;
+        d.w = d_words;
+        d.start = d_start;
+        d.deg = d_deg;
+        d.n = (uint32_t)n;
+        d.fmt = fmt;
+        return d;
+    }
+};
+
+namespace grass {
+
+int status_of(hipError_t e);
+
+// loader.hip
+struct ByteSource {
+    virtual ~ByteSource() {}
+    virtual uint64_t size() const = 0;
+    // Reads len bytes at the current position; false on a short read.
+    virtual bool read(void* dst, uint64_t len) = 0;
+};
+int load_graph(grass_graph* g, ByteSource& graph_src, ByteSource& offset_src);
+
+// decode.hip
+int decode_csr(grass_graph* g, uint64_t* rowptr, uint32_t* col, hipStream_t stream);
+
+// rw.hip
+// bytes != nullptr: write nothing, return the algorithmic byte count of the same walks
+int random_walk(grass_graph* g, const uint32_t* starts, uint64_t walkers, uint32_t length,
+                uint64_t seed, uint32_t instance, uint32_t* out, hipStream_t stream,
+                uint64_t* bytes);
+int random_walk_batch(grass_graph* g, const grass_rw_request* reqs, uint32_t count, uint64_t seed,
+                      hipStream_t stream, uint64_t* bytes);
+
+}  // namespace grass

@@ -1,0 +1,104 @@
+// Reduce-then-scan exclusive prefix sum over a generated sequence of uint64 values.
+//
+// Three launches with no inter-workgroup waiting: per-tile sums, one workgroup scanning the
+// tile sums, then each tile re-generates its values and writes prefixes.  The length can be
+// a host value or a device-resident count (so a sequence whose size a previous kernel
+// produced needs no host round trip); the grid is sized for the host-side upper bound.
+// Used for the record-length scan at load (replacing cub::DeviceScan::ExclusiveSum,
+// LE/src/utils.hpp:337-342), CSR row pointers and sampler output offsets.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "devutil.h"
+
+namespace grass {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+constexpr uint64_t kScanTile = (uint64_t)kScanBlock * kScanItems;
+
+struct Count {
+    uint64_t imm;
+    const uint64_t* dev;  // when non-null the count is *dev
+    __device__ __forceinline__ uint64_t get() const { return dev ? *dev : imm; }
+};
+
+template <class Gen>
+__global__ __launch_bounds__(kScanBlock) void scan_reduce_kernel(Gen gen, Count cnt,
+                                                                 uint64_t* partials) {
+    __shared__ uint64_t lds[kScanBlock / kWave];
+    const uint64_t n = cnt.get();
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    uint64_t s = 0;
+    if (base < n) {
+#pragma unroll 4
+        for (int j = 0; j < kScanItems; ++j) {
+            uint64_t i = base + (uint64_t)j * kScanBlock + threadIdx.x;
+            if (i < n) s += gen(i);
+        }
+    }
+    s = block_sum<uint64_t, kScanBlock>(s, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// One workgroup: partials[0..nb) -> exclusive prefixes in place; *total = sum.
+// (A template so that every translation unit including this header may instantiate it.)
+template <typename T>
+__global__ __launch_bounds__(1024) void scan_partials_kernel(T* partials, uint64_t nb, T* total) {
+    __shared__ T lds[1024 / kWave];
+    const uint64_t per = (nb + 1023) / 1024;
+    const uint64_t lo = threadIdx.x * per;
+    T s = 0;
+    for (uint64_t i = lo; i < lo + per && i < nb; ++i) s += partials[i];
+    T tot;
+    T pre = block_exclusive_sum<T, 1024>(s, tot, lds);
+    for (uint64_t i = lo; i < lo + per && i < nb; ++i) {
+        T t = partials[i];
+        partials[i] = pre;
+        pre += t;
+    }
+    if (threadIdx.x == 0 && total) *total = tot;
+}
+
+// out[i] = sum of gen(0..i-1) for i < n, and out[n] = total.
+template <class Gen>
+__global__ __launch_bounds__(kScanBlock) void scan_write_kernel(Gen gen, Count cnt,
+                                                                const uint64_t* partials,
+                                                                const uint64_t* total,
+                                                                uint64_t* out) {
+    __shared__ uint64_t lds[kScanBlock / kWave];
+    const uint64_t n = cnt.get();
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    if (base >= n) {
+        if (base == 0 && threadIdx.x == 0) out[0] = 0;  // empty sequence
+        return;
+    }
+    uint64_t carry = partials[blockIdx.x];
+    for (int j = 0; j < kScanItems; ++j) {
+        uint64_t i = base + (uint64_t)j * kScanBlock + threadIdx.x;
+        uint64_t x = i < n ? gen(i) : 0;
+        uint64_t tot;
+        uint64_t ex = block_exclusive_sum<uint64_t, kScanBlock>(x, tot, lds);
+        if (i < n) out[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && base + kScanTile >= n) out[n] = *total;
+}
+
+inline uint64_t scan_blocks(uint64_t n_max) { return n_max ? (n_max + kScanTile - 1) / kScanTile : 1; }
+
+// Scratch: partials[scan_blocks(n_max)] and total[1].
+template <class Gen>
+hipError_t exclusive_scan(Gen gen, Count cnt, uint64_t n_max, uint64_t* out, uint64_t* partials,
+                          uint64_t* total, hipStream_t stream) {
+    const uint64_t nb = scan_blocks(n_max);
+    hipLaunchKernelGGL(scan_reduce_kernel<Gen>, dim3((unsigned)nb), dim3(kScanBlock), 0, stream,
+                       gen, cnt, partials);
+    hipLaunchKernelGGL(scan_partials_kernel<uint64_t>, dim3(1), dim3(1024), 0, stream, partials, nb, total);
+    hipLaunchKernelGGL(scan_write_kernel<Gen>, dim3((unsigned)nb), dim3(kScanBlock), 0, stream,
+                       gen, cnt, (const uint64_t*)partials, (const uint64_t*)total, out);
+    return hipGetLastError();
+}
+
+}  // namespace grass

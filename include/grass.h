@@ -1,0 +1,125 @@
+/*
+ * grass.h -- C ABI of the MI355X GraSS sampler (libgrass_amd.so).
+ *
+ * The reference has no C ABI: its boundary is the .graph/.offset files, the rw/ks/mrw apps
+ * (`-g <cgc_path> [-n <run_times>]`) and compile-time App templates.  Each entry point below
+ * names the reference interface it replaces.  Conventions:
+ *   - every function returns a grass_status (0 = GRASS_OK) and never throws;
+ *   - pointers named *_dev are device (HBM) pointers owned by the caller unless stated;
+ *   - `stream` is a hipStream_t (NULL = the graph's own stream); a call only enqueues work on
+ *     it unless it says it synchronises;
+ *   - a graph handle may be used from several host threads; calls on one stream are ordered.
+ */
+#ifndef GRASS_H
+#define GRASS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    GRASS_OK = 0,
+    GRASS_EINVAL = 1,     /* bad argument (null handle, size overflow, vertex id >= n, ...) */
+    GRASS_EIO = 2,        /* a .graph/.offset file is missing or unreadable */
+    GRASS_EFORMAT = 3,    /* malformed compressed graph */
+    GRASS_ENOMEM = 4,     /* host or device allocation failed */
+    GRASS_EDEVICE = 5,    /* HIP runtime error */
+    GRASS_ECAPACITY = 6,  /* an output buffer or the admission budget is too small */
+    GRASS_ECANCELLED = 7, /* request cancelled */
+    GRASS_ETIMEOUT = 8,   /* wait timed out; the request is still running */
+    GRASS_ENOTREADY = 9,  /* poll: the request has not finished */
+} grass_status;
+
+typedef enum { GRASS_FMT_LE = 0, GRASS_FMT_GT = 1 } grass_format;
+
+typedef struct grass_graph grass_graph;
+
+typedef struct {
+    uint64_t n;            /* vertices */
+    uint64_t m;            /* adjacency entries (sum of degrees) */
+    uint64_t max_degree;
+    uint64_t graph_bytes;  /* size of <p>.graph */
+    uint64_t offset_bytes; /* size of <p>.offset */
+    uint64_t total_bits;   /* sum of record lengths */
+    uint32_t ubl;          /* width of one record-length field in .offset */
+    int32_t format;        /* grass_format */
+    int32_t device;
+    uint64_t device_bytes; /* HBM held by the resident graph */
+} grass_graph_info;
+
+/* Text of a status code. */
+const char *grass_strerror(int status);
+
+/* Loads <prefix>.graph and <prefix>.offset onto `device` (HIP ordinal): pinned-host staged
+ * copies, device byte swap, record-length extraction + scan, full header validation.
+ * Replaces loadGraph (LE/src/grass.cu:1015-1095), transferToGPUs (LE/src/grass.cu:1097-1113)
+ * and copyPartitionToGPU with decode_offset + cub scan (LE/src/utils.hpp:328-353,
+ * LE/src/bcg.cuh:284-329).  The reference returns nullptr on a missing file (grass.cu:1023);
+ * this returns GRASS_EIO.  Synchronises. */
+int grass_open(const char *prefix, int format, int device, grass_graph **out);
+
+/* Same as grass_open from in-memory file images (host memory). */
+int grass_open_memory(const void *graph, uint64_t graph_bytes, const void *offset,
+                      uint64_t offset_bytes, int format, int device, grass_graph **out);
+
+/* Frees every device and host resource of the graph.  NULL is accepted. */
+int grass_close(grass_graph *g);
+
+int grass_info(const grass_graph *g, grass_graph_info *info);
+
+/* Device views of the resident graph: big-endian .graph words, record bit starts (n+1),
+ * degrees (n).  Valid until grass_close. */
+int grass_device_arrays(const grass_graph *g, const uint32_t **words_dev,
+                        const uint64_t **starts_dev, const uint32_t **degree_dev);
+
+/* Waits for `stream` and returns the first error a kernel of this graph recorded since the
+ * last call (a decoded neighbour id >= n, a start vertex >= n), then clears it. */
+int grass_sync(grass_graph *g, void *stream);
+
+/* Full decode to CSR: rowptr_dev[n+1] (uint64), col_dev[m] (uint32); col_dev == NULL writes
+ * the row pointers only.  Replaces running
+ * BCGVertex::get_vertex (LE/src/bcg.cuh:131-261, GT/src/bcg.cuh:110-168) over every k. */
+int grass_decode_csr(grass_graph *g, uint64_t *rowptr_dev, uint32_t *col_dev, void *stream);
+
+/* Uniform random walks, one launch: out_dev is row-major [walkers][length+1] uint32; a walk
+ * that reaches a vertex without neighbours is padded with n (INVALID_VERTEX, grass.cu:1182).
+ * starts_dev == NULL draws start w from Philox(seed, {instance, w, 0xFFFFFFFF, 0}); step s of
+ * walker w draws from Philox(seed, {instance, w, s, 0}).  Replaces DeepWalkApp::next
+ * (LE/src/apps/randomwalk/randomWalks.cu:62-79) driven by doTransitParallelSampling
+ * (LE/src/grass.cu:1299-1900). */
+int grass_random_walk(grass_graph *g, const uint32_t *starts_dev, uint64_t walkers,
+                      uint32_t length, uint64_t seed, uint32_t instance, uint32_t *out_dev,
+                      void *stream);
+
+/* One random-walk request of a batch. */
+typedef struct {
+    uint64_t walkers;
+    uint32_t length;
+    uint32_t instance;
+    const uint32_t *starts_dev; /* NULL: Philox-drawn starts */
+    uint32_t *out_dev;          /* [walkers][length+1] */
+} grass_rw_request;
+
+/* Many random-walk requests of different sizes in one persistent launch with a device work
+ * queue; each request's output is identical to grass_random_walk on it alone. */
+int grass_random_walk_batch(grass_graph *g, const grass_rw_request *reqs, uint32_t count,
+                            uint64_t seed, void *stream);
+
+/* Measurement companions (no reference counterpart; the reference only prints timers,
+ * LE/src/grass.cu:1865-1866,1984-1987): run the same walks without writing them and return
+ * their algorithmic HBM bytes -- per walker 4 (start written, +4 if read from starts_dev), per
+ * visited vertex 8 (record start) + ceil(header bits/8), per step ceil(payload bits read/8) +
+ * 4 (id written).  out_dev of the requests is ignored.  Synchronise. */
+int grass_random_walk_bytes(grass_graph *g, const uint32_t *starts_dev, uint64_t walkers,
+                            uint32_t length, uint64_t seed, uint32_t instance, void *stream,
+                            uint64_t *bytes);
+int grass_random_walk_batch_bytes(grass_graph *g, const grass_rw_request *reqs, uint32_t count,
+                                  uint64_t seed, void *stream, uint64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

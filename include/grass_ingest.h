@@ -1,0 +1,57 @@
+/*
+ * grass_ingest.h -- host-side ingest of libgrass_amd.so: synthetic R-MAT edge lists, the
+ * reference's graph_sort preprocessing, and the LE / GT compressors that write the
+ * .graph/.offset format.  All parallel (std::thread), streaming over flat arrays, and
+ * byte-identical to the reference tools on the same input.
+ */
+#ifndef GRASS_INGEST_H
+#define GRASS_INGEST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct grass_csr grass_csr;   /* host CSR owned by the library */
+typedef struct grass_blob grass_blob; /* host byte buffer owned by the library */
+
+/* R-MAT edge list: m = edge_factor << scale edges, vertex ids < 2^scale.  Edge i descends
+ * `scale` levels; level l picks quadrant (0,0)/(0,1)/(1,0)/(1,1) with probability a/b/c/rest
+ * using 32-bit word l%4 of Philox(seed, {i_lo, i_hi, l/4, 0x524D4154}).  The reference has no
+ * generator; this defines the synthetic inputs.  src/dst hold m entries. */
+int grass_rmat_edges(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                     uint64_t seed, uint32_t *src, uint32_t *dst, int nthreads);
+
+/* graph_sort (LE/chunk_compress/graph_sort.cpp:22-82): drops self loops, numbers vertices
+ * by first appearance in the stream (u before v), for undirected input makes each pair
+ * (min,max), sorts and removes duplicates, then adds the mirror of every edge.  The result
+ * is the sorted edge list the compressor reads, held as CSR (n = number of ids assigned). */
+int grass_graph_sort(const uint32_t *src, const uint32_t *dst, uint64_t m, int directed,
+                     int nthreads, grass_csr **out);
+
+/* Compressor::load_graph (LE/chunk_compress/compressor.hpp:93-128): a (u,v)-sorted edge list,
+ * adjacent duplicates skipped, n = max id + 1.  GRASS_EINVAL if the list is not sorted. */
+int grass_csr_from_sorted_edges(const uint32_t *src, const uint32_t *dst, uint64_t m,
+                                grass_csr **out);
+
+int grass_csr_info(const grass_csr *csr, uint64_t *n, uint64_t *m);
+int grass_csr_arrays(const grass_csr *csr, const uint64_t **rowptr, const uint32_t **col);
+void grass_csr_free(grass_csr *csr);
+
+/* The LE (format 0) or GT (format 1) compressor in "normal" mode
+ * (LE/chunk_compress/compressor.hpp:288-403,130-285; GT/chunk_compress/compressor.hpp:
+ * 245-349,120-243): writes the .graph and .offset images. */
+int grass_compress(const grass_csr *csr, int format, int nthreads, grass_blob **graph,
+                   grass_blob **offset);
+
+int grass_blob_data(const grass_blob *blob, const uint8_t **data, uint64_t *size);
+void grass_blob_free(grass_blob *blob);
+
+/* Writes <prefix>.graph and <prefix>.offset. */
+int grass_write_files(const char *prefix, const grass_blob *graph, const grass_blob *offset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,41 @@
+"""CPU checks of the C ABI library: it loads without a GPU and exports every function the
+headers under include/ declare; status strings; host-side argument errors."""
+import ctypes as C
+import glob
+import os
+
+import pytest
+
+from conftest import REPO
+
+
+def _declared():
+    import re
+    names = set()
+    for h in glob.glob(os.path.join(REPO, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(grass_[a-z_0-9]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    import grass_amd
+    L = grass_amd.lib()
+    names = _declared()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_strerror_and_null_handles():
+    import grass_amd
+    L = grass_amd.lib()
+    assert grass_amd.strerror(0) == "success"
+    assert "malformed" in grass_amd.strerror(grass_amd.EFORMAT)
+    assert L.grass_info(None, None) == grass_amd.EINVAL
+    assert L.grass_close(None) == grass_amd.OK
+    assert L.grass_decode_csr(None, None, None, None) == grass_amd.EINVAL
+    h = C.c_void_p()
+    assert L.grass_open(None, 0, 0, C.byref(h)) == grass_amd.EINVAL
+    assert L.grass_open(b"/nonexistent/x", 0, 0, C.byref(h)) == grass_amd.EIO
+    assert L.grass_open(b"/nonexistent/x", 7, 0, C.byref(h)) in (grass_amd.EINVAL, grass_amd.EIO)
