@@ -45,6 +45,19 @@ class RwRequest(C.Structure):
                 ("starts_dev", C.c_void_p), ("out_dev", C.c_void_p)]
 
 
+KHOP_MAX_HOPS = 8
+
+
+class KhopRequest(C.Structure):
+    _fields_ = [("seeds_dev", C.c_void_p), ("num_seeds", C.c_uint64), ("instance", C.c_uint32),
+                ("hops", C.c_uint32), ("fanout", C.c_uint32 * KHOP_MAX_HOPS)]
+
+
+class KhopOutput(C.Structure):
+    _fields_ = [("src_dev", C.c_void_p), ("dst_dev", C.c_void_p), ("capacity", C.c_uint64),
+                ("offsets_dev", C.c_void_p), ("total", C.c_uint64), ("max_hops", C.c_uint32)]
+
+
 _lib = None
 
 
@@ -70,6 +83,8 @@ def lib():
             "grass_random_walk_batch": (I32, [P, P, U32, U64, P]),
             "grass_random_walk_bytes": (I32, [P, P, U64, U32, U64, U32, P, P]),
             "grass_random_walk_batch_bytes": (I32, [P, P, U32, U64, P, P]),
+            "grass_khop_bound": (I32, [P, P, U32, P]),
+            "grass_khop_batch": (I32, [P, P, U32, U64, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -240,6 +255,61 @@ class Graph:
                                                    C.byref(nbytes)),
                "grass_random_walk_batch_bytes")
         return nbytes.value
+
+    def khop_request_array(self, seeds_list, fanouts_list, instances=None):
+        """ctypes array of grass_khop_request; the returned list keeps device seeds alive."""
+        torch = _torch()
+        R = len(seeds_list)
+        arr = (KhopRequest * max(R, 1))()
+        keep = []
+        for i, (s, f) in enumerate(zip(seeds_list, fanouts_list)):
+            if hasattr(s, "data_ptr"):
+                t = s
+            else:
+                t = torch.from_numpy(np.ascontiguousarray(s, dtype=np.uint32).view(np.int32)).to(
+                    self._cuda())
+            keep.append(t)
+            fan = (C.c_uint32 * KHOP_MAX_HOPS)(*([int(x) for x in f] + [0] * (KHOP_MAX_HOPS - len(f))))
+            inst = i if instances is None else int(instances[i])
+            arr[i] = KhopRequest(t.data_ptr() if t.numel() else None, t.numel(), inst, len(f), fan)
+        return arr, keep
+
+    def khop_bound(self, arr, count):
+        b = C.c_uint64()
+        _check(lib().grass_khop_bound(self._h, arr, count, C.byref(b)), "grass_khop_bound")
+        return b.value
+
+    def khop_batch(self, seeds_list, fanouts_list, seed, instances=None, stream=None):
+        """Runs the batch; returns host results {"offsets": [H][R+1], "src", "dst", "total"}."""
+        torch = _torch()
+        R = len(seeds_list)
+        arr, keep = self.khop_request_array(seeds_list, fanouts_list, instances)
+        cap = self.khop_bound(arr, R)
+        H = max([len(f) for f in fanouts_list] + [0])
+        src = torch.empty(max(cap, 1), dtype=torch.int32, device=self._cuda())
+        dst = torch.empty(max(cap, 1), dtype=torch.int32, device=self._cuda())
+        off = torch.zeros(max(H * (R + 1), 1), dtype=torch.int64, device=self._cuda())
+        out = KhopOutput(src.data_ptr(), dst.data_ptr(), cap, off.data_ptr(), 0, 0)
+        _check(lib().grass_khop_batch(self._h, arr, R, seed, C.byref(out),
+                                      C.c_void_p(_stream_ptr(stream))), "grass_khop_batch")
+        self.sync(stream)
+        t = out.total
+        return {"offsets": off.cpu().numpy().view(np.uint64)[: H * (R + 1)].reshape(H, R + 1),
+                "src": to_numpy_u32(src)[:t], "dst": to_numpy_u32(dst)[:t], "total": t,
+                "max_hops": out.max_hops}
+
+    @staticmethod
+    def khop_request_edges(res, r):
+        """Request r's edges, hop by hop concatenated (request-major view of a hop-major batch)."""
+        off = res["offsets"]
+        parts_s, parts_d, counts = [], [], []
+        for h in range(off.shape[0]):
+            a, b = int(off[h, r]), int(off[h, r + 1])
+            parts_s.append(res["src"][a:b])
+            parts_d.append(res["dst"][a:b])
+            counts.append(b - a)
+        cat = (lambda p: np.concatenate(p) if p else np.zeros(0, np.uint32))
+        return cat(parts_s), cat(parts_d), counts
 
     def random_walk_batch(self, requests, seed, stream=None, sync=True):
         """requests: list of dicts {walkers, length, instance, starts (optional)}.

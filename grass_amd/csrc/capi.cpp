@@ -71,6 +71,7 @@ void free_graph(grass_graph* g) {
     if (g->d_deg) (void)hipFree(g->d_deg);
     if (g->d_err) (void)hipFree(g->d_err);
     g->scratch.release();
+    g->ht.release();
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
 }
@@ -210,6 +211,19 @@ int grass_random_walk_batch(grass_graph* g, const grass_rw_request* reqs, uint32
     for (uint32_t i = 0; reqs && i < count; ++i)
         if (reqs[i].walkers && !reqs[i].out_dev) return GRASS_EINVAL;
     return random_walk_batch(g, reqs, count, seed, pick(g, stream), nullptr);
+}
+
+int grass_khop_bound(const grass_graph* g, const grass_khop_request* reqs, uint32_t count,
+                     uint64_t* max_edges) {
+    if (!g || (count && !reqs) || !max_edges) return GRASS_EINVAL;
+    return khop_bound(g, reqs, count, max_edges, nullptr, nullptr, nullptr);
+}
+
+int grass_khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t count, uint64_t seed,
+                     grass_khop_output* out, void* stream) {
+    if (!g || (count && !reqs) || !out) return GRASS_EINVAL;
+    DeviceGuard guard(g->device);
+    return khop_batch(g, reqs, count, seed, out, pick(g, stream));
 }
 
 int grass_random_walk_bytes(grass_graph* g, const uint32_t* starts_dev, uint64_t walkers,

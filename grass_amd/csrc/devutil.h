@@ -86,6 +86,15 @@ __device__ __forceinline__ T block_sum(T x, T* lds) {
     return tot;
 }
 
+// p[0..count) = value.  Used instead of hipMemsetAsync for anything a caller might capture in a
+// HIP graph.
+template <typename T>
+__global__ void fill_kernel(T* p, uint64_t count, T value) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
 template <typename T, int BLOCK>
 __device__ __forceinline__ T block_max(T x, T* lds) {
     T s = wave_max(x);

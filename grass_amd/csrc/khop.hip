@@ -1,0 +1,498 @@
+// Batched k-hop neighbourhood sampling without replacement, per-hop dedup.
+//
+// The reference's k-hop app draws `stepSize(k)` neighbours with replacement per transit and
+// keeps no frontier (LE/src/apps/khop/khop.cu:1-24, driven by the transit-parallel engine of
+// LE/src/grass.cu:1299-1900).  Here every request of a batch runs in the same launches, with all
+// counts kept on the device, so a batch of any size costs a fixed number of launches per hop and
+// no host round trip:
+//   frontier 0  = the request's seeds, duplicates dropped (first appearance kept);
+//   hop h       = for each frontier vertex f (in order): c = min(fanout[h], deg f) distinct
+//                 neighbour positions, drawn by Floyd's algorithm with draw k taken from
+//                 Philox(seed, {instance, f, h, k}); edges (f, N_f[pos]) in ascending pos;
+//   frontier h+1 = destinations of hop h, duplicates dropped (first appearance kept).
+// Per hop: one fused count+scan (exclusive scan over items of min(fanout, deg)), a sampling
+// kernel with G lanes per frontier vertex (the c draws are made in parallel; Floyd's
+// "taken?" test is resolved in draw order with wave shuffles; ranks give the sorted order), and
+// a dedup of (request, vertex) keys in a generation-tagged hash table where the smallest edge
+// position wins, followed by a flag scan and a compaction into the next frontier.
+// Output is hop-major: edges of (hop h, request r) are [off[h][r], off[h][r+1]).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "devutil.h"
+#include "philox.h"
+#include "runtime.h"
+#include "scan.h"
+
+namespace grass {
+
+constexpr int kKhBlock = 256;
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+
+struct KhReq {
+    const uint32_t* seeds;
+    uint64_t seed_begin;
+    uint32_t inst;
+    uint32_t hops;
+    uint32_t fan[GRASS_KHOP_MAX_HOPS];
+};
+
+// Largest r in [0, R-1] with off[r] <= p.
+__device__ __forceinline__ uint32_t seg_of(const uint64_t* __restrict__ off, uint32_t R, uint64_t p) {
+    uint32_t lo = 0, hi = R - 1;
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo + 1) / 2;
+        if (off[mid] <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ------------------------------------------------------------------ hash table (dedup)
+// keys: (gen << 48) | (request << 31) | vertex; a slot whose generation is not the current one
+// is free.  vals: ((0xFFFF - gen) << 32) | position, so atomicMin keeps the smallest position of
+// the current generation over any stale value.
+struct HashTab {
+    unsigned long long* keys;
+    unsigned long long* vals;
+    uint64_t mask;
+    uint32_t shift;
+};
+
+__device__ __forceinline__ uint64_t ht_home(const HashTab& t, uint64_t key48) {
+    return ((key48 * 0x9E3779B97F4A7C15ull) >> t.shift) & t.mask;
+}
+
+__global__ void ht_clear_kernel(HashTab t) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= t.mask;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        t.keys[i] = 0ull;
+        t.vals[i] = ~0ull;
+    }
+}
+
+// Value of the p-th entry of a dedup input, kInvalid when absent or past the output capacity.
+struct DedupIn {
+    const uint32_t* arr;
+    const uint64_t* base;  // device offset added to p (hop base); nullptr = 0
+    uint64_t cap;          // entries of arr that exist
+    __device__ __forceinline__ uint32_t at(uint64_t p) const {
+        uint64_t q = (base ? *base : 0) + p;
+        return q < cap ? arr[q] : kInvalid;
+    }
+};
+
+__global__ __launch_bounds__(kKhBlock) void dedup_insert_kernel(DedupIn in, Count E,
+                                                                const uint64_t* __restrict__ eoff_r,
+                                                                uint32_t R, HashTab t, uint32_t gen,
+                                                                uint32_t* err) {
+    const uint64_t n = E.get();
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t v = in.at(p);
+        if (v == kInvalid) continue;
+        uint64_t key48 = ((uint64_t)seg_of(eoff_r, R, p) << 31) | v;
+        unsigned long long full = ((unsigned long long)gen << 48) | key48;
+        uint64_t h = ht_home(t, key48);
+        uint64_t probes = 0;
+        for (;;) {
+            unsigned long long cur = __atomic_load_n(&t.keys[h], __ATOMIC_RELAXED);
+            if (cur == full) break;
+            if ((uint32_t)(cur >> 48) != gen) {
+                unsigned long long prev = atomicCAS(&t.keys[h], cur, full);
+                if (prev == cur || prev == full) break;
+                continue;  // the slot was taken meanwhile: look at it again
+            }
+            h = (h + 1) & t.mask;
+            if (++probes > t.mask) {  // cannot happen while slots > entries; never spin
+                atomicCAS(err, kErrNone, kErrBadNeighbor);
+                break;
+            }
+        }
+        atomicMin(&t.vals[h], ((unsigned long long)(0xFFFFu - gen) << 32) | (uint32_t)p);
+    }
+}
+
+// flag[p] = 1 when entry p is the first occurrence of its (request, vertex) key.
+__global__ __launch_bounds__(kKhBlock) void dedup_mark_kernel(DedupIn in, Count E,
+                                                              const uint64_t* __restrict__ eoff_r,
+                                                              uint32_t R, HashTab t, uint32_t gen,
+                                                              uint32_t* __restrict__ flag) {
+    const uint64_t n = E.get();
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t v = in.at(p);
+        uint32_t first = 0;
+        if (v != kInvalid) {
+            uint64_t key48 = ((uint64_t)seg_of(eoff_r, R, p) << 31) | v;
+            unsigned long long full = ((unsigned long long)gen << 48) | key48;
+            uint64_t h = ht_home(t, key48);
+            for (uint64_t probes = 0; probes <= t.mask; ++probes) {
+                if (t.keys[h] == full) {
+                    first = (uint32_t)t.vals[h] == (uint32_t)p;
+                    break;
+                }
+                h = (h + 1) & t.mask;
+            }
+        }
+        flag[p] = first;
+    }
+}
+
+struct FlagGen {
+    const uint32_t* flag;
+    __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return flag[i]; }
+};
+
+__global__ __launch_bounds__(kKhBlock) void compact_kernel(DedupIn in, Count E,
+                                                           const uint64_t* __restrict__ eoff_r,
+                                                           uint32_t R,
+                                                           const uint32_t* __restrict__ flag,
+                                                           const uint64_t* __restrict__ fpos,
+                                                           uint32_t* __restrict__ fv,
+                                                           uint32_t* __restrict__ fr) {
+    const uint64_t n = E.get();
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        if (!flag[p]) continue;
+        uint64_t q = fpos[p];
+        fv[q] = in.at(p);
+        fr[q] = seg_of(eoff_r, R, p);
+    }
+}
+
+// foff[r] = first frontier item of request r; icount = frontier size.
+__global__ void front_offsets_kernel(const uint64_t* __restrict__ eoff_r, uint32_t R,
+                                     const uint64_t* __restrict__ fpos, uint64_t* __restrict__ foff,
+                                     uint64_t* icount) {
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r <= R; r += gridDim.x * blockDim.x) {
+        uint64_t x = fpos[eoff_r[r]];
+        foff[r] = x;
+        if (r == R) *icount = x;
+    }
+}
+
+// ------------------------------------------------------------------ seeds -> virtual edges
+__global__ __launch_bounds__(kKhBlock) void seed_gather_kernel(const KhReq* __restrict__ reqs,
+                                                               const uint64_t* __restrict__ sbeg,
+                                                               uint32_t R, uint64_t S, uint32_t n,
+                                                               uint32_t* __restrict__ out,
+                                                               uint32_t* err) {
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < S;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t r = seg_of(sbeg, R, p);
+        uint32_t v = reqs[r].seeds[p - sbeg[r]];
+        if (v >= n) {
+            atomicCAS(err, kErrNone, kErrBadStart);
+            v = kInvalid;
+        }
+        out[p] = v;
+    }
+}
+
+// ------------------------------------------------------------------ per hop
+struct CountGen {
+    const uint32_t* fv;
+    const uint32_t* fr;
+    const KhReq* reqs;
+    const uint32_t* deg;
+    uint32_t h;
+    __device__ __forceinline__ uint64_t operator()(uint64_t i) const {
+        const KhReq& q = reqs[fr[i]];
+        uint32_t f = h < q.hops ? q.fan[h] : 0u;
+        uint32_t d = deg[fv[i]];
+        return d < f ? d : f;
+    }
+};
+
+// eoff_r[r] = first edge (within hop h) of request r; out_off[h][r] = hop base + eoff_r[r];
+// hb[h+1] = hb[h] + edges of hop h.
+__global__ void hop_offsets_kernel(const uint64_t* __restrict__ foff, uint32_t R,
+                                   const uint64_t* __restrict__ eoff, uint64_t* __restrict__ eoff_r,
+                                   uint64_t* hb, uint32_t h, uint64_t* __restrict__ out_off) {
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r <= R; r += gridDim.x * blockDim.x) {
+        uint64_t e = eoff[foff[r]];
+        eoff_r[r] = e;
+        out_off[(uint64_t)h * (R + 1) + r] = hb[h] + e;
+        if (r == R) hb[h + 1] = hb[h] + e;
+    }
+}
+
+template <int FMT, int G>
+__global__ __launch_bounds__(kKhBlock) void sample_kernel(DevGraph g, const uint32_t* __restrict__ fv,
+                                                          const uint32_t* __restrict__ fr,
+                                                          const uint64_t* __restrict__ icount,
+                                                          const KhReq* __restrict__ reqs, uint32_t h,
+                                                          uint64_t seed,
+                                                          const uint64_t* __restrict__ eoff,
+                                                          const uint64_t* __restrict__ hb,
+                                                          uint32_t* __restrict__ osrc,
+                                                          uint32_t* __restrict__ odst, uint64_t cap,
+                                                          uint32_t* err) {
+    const int lane = lane_id();
+    const int gl = lane % G;
+    const int gbase = lane - gl;
+    const uint64_t I = *icount;
+    const uint64_t hbase = hb[h];
+    const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / G;
+    for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;; i += groups) {
+        const bool has = i < I;
+        if (!__any(has)) break;  // wave-uniform exit
+        uint32_t v = 0, c = 0, d = 0, inst = 0;
+        uint64_t ebase = 0;
+        if (has) {
+            v = fv[i];
+            const KhReq& q = reqs[fr[i]];
+            uint32_t f = h < q.hops ? q.fan[h] : 0u;
+            d = g.deg[v];
+            c = d < f ? d : f;
+            inst = q.inst;
+            ebase = hbase + eoff[i];
+        }
+        const bool all = c == d;  // every neighbour: positions 0..d-1, no draws
+        const uint32_t j = d - c + gl;
+        uint32_t t = 0;
+        if (gl < (int)c && !all) t = (uint32_t)uniform(seed, inst, v, h, gl, (uint64_t)j + 1);
+        // Floyd: x_k = t_k unless t_k equals an earlier x, then j_k; resolved in draw order
+        bool dup = false;
+#pragma unroll 4
+        for (int jj = 0; jj < G; ++jj) {
+            uint32_t xj = __shfl(dup ? j : t, gbase + jj, kWave);
+            if (gl > jj && jj < (int)c && t == xj) dup = true;
+        }
+        const uint32_t x = all ? (uint32_t)gl : (dup ? j : t);
+        uint32_t rank = all ? (uint32_t)gl : 0u;
+        if (!all) {
+#pragma unroll 4
+            for (int jj = 0; jj < G; ++jj) {
+                uint32_t xj = __shfl(x, gbase + jj, kWave);
+                if (jj < (int)c && xj < x) ++rank;
+            }
+        }
+        if (gl < (int)c) {
+            Hdr hd;
+            parse_header<FMT>(g.w, g.start[v], hd);
+            uint32_t nb = get_neighbor<FMT>(g.w, hd, v, x);
+            if (nb >= g.n) {  // never let a bad id reach the next frontier
+                atomicCAS(err, kErrNone, kErrBadNeighbor);
+                nb = kInvalid;
+            }
+            const uint64_t p = ebase + rank;
+            if (p < cap) {
+                osrc[p] = v;
+                odst[p] = nb;
+            }
+        }
+    }
+}
+
+int khop_bound(const grass_graph* g, const grass_khop_request* reqs, uint32_t count,
+               uint64_t* edges, uint64_t* items, uint64_t* dedup, uint32_t* max_hops) {
+    uint64_t E = 0, I = 0, D = 0;
+    uint32_t H = 0;
+    std::vector<uint64_t> Ih(GRASS_KHOP_MAX_HOPS, 0), Eh(GRASS_KHOP_MAX_HOPS, 0);
+    for (uint32_t r = 0; r < count; ++r) {
+        const grass_khop_request& q = reqs[r];
+        if (q.hops > GRASS_KHOP_MAX_HOPS || (q.num_seeds && !q.seeds_dev)) return GRASS_EINVAL;
+        H = std::max(H, q.hops);
+        for (uint32_t h = 0; h < q.hops; ++h)
+            if (q.fanout[h] > 64) return GRASS_EINVAL;
+    }
+    // per request: b = frontier bound, e = edge bound of each hop; a request whose hops are
+    // done still holds the frontier its last hop produced (with fanout 0 from then on)
+    for (uint32_t r = 0; r < count; ++r) {
+        const grass_khop_request& q = reqs[r];
+        uint64_t b = std::min<uint64_t>(q.num_seeds, g->n);
+        for (uint32_t h = 0; h < H; ++h) {
+            Ih[h] += b;
+            uint64_t e = h < q.hops ? b * std::min<uint64_t>(q.fanout[h], g->max_degree) : 0;
+            Eh[h] += e;
+            b = std::min<uint64_t>(e, g->n);
+        }
+    }
+    uint64_t S = 0;
+    for (uint32_t r = 0; r < count; ++r) S += reqs[r].num_seeds;
+    D = S;
+    for (uint32_t h = 0; h < H; ++h) {
+        E += Eh[h];
+        I = std::max(I, Ih[h]);
+        if (h + 1 < H) D = std::max(D, Eh[h]);
+    }
+    I = std::max(I, S);
+    if (edges) *edges = E;
+    if (items) *items = I;
+    if (dedup) *dedup = D;
+    if (max_hops) *max_hops = H;
+    return GRASS_OK;
+}
+
+template <int FMT>
+static void launch_sample(grass_graph* g, bool wide, unsigned grid, hipStream_t s, const uint32_t* fv,
+                          const uint32_t* fr, const uint64_t* ic, const KhReq* reqs, uint32_t h,
+                          uint64_t seed, const uint64_t* eoff, const uint64_t* hb, uint32_t* osrc,
+                          uint32_t* odst, uint64_t cap) {
+    if (wide)
+        hipLaunchKernelGGL((sample_kernel<FMT, 64>), dim3(grid), dim3(kKhBlock), 0, s, g->dev(), fv, fr,
+                           ic, reqs, h, seed, eoff, hb, osrc, odst, cap, g->d_err);
+    else
+        hipLaunchKernelGGL((sample_kernel<FMT, 32>), dim3(grid), dim3(kKhBlock), 0, s, g->dev(), fv, fr,
+                           ic, reqs, h, seed, eoff, hb, osrc, odst, cap, g->d_err);
+}
+
+static hipError_t ensure_table(grass_graph* g, uint64_t entries, hipStream_t s, HashTab& t) {
+    uint64_t slots = 1024;
+    while (slots < 2 * entries + 2) slots <<= 1;
+    if (slots > g->ht_slots) {
+        hipError_t e = g->ht.reserve(slots * 16);
+        if (e != hipSuccess) return e;
+        g->ht_slots = slots;
+        g->ht_gen = 0xFFFF;  // force a clear
+    }
+    t.keys = (unsigned long long*)g->ht.ptr;
+    t.vals = t.keys + g->ht_slots;
+    t.mask = g->ht_slots - 1;
+    uint32_t lg = 0;
+    while ((1ull << lg) < g->ht_slots) ++lg;
+    t.shift = 64 - lg;
+    return hipSuccess;
+}
+
+static uint32_t next_gen(grass_graph* g, const HashTab& t, hipStream_t s) {
+    if (g->ht_gen >= 0xFFFF) {
+        hipLaunchKernelGGL(ht_clear_kernel, dim3(4096), dim3(256), 0, s, t);
+        g->ht_gen = 0;
+    }
+    return ++g->ht_gen;
+}
+
+int khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uint64_t seed,
+               grass_khop_output* out, hipStream_t s) {
+    out->total = 0;
+    out->max_hops = 0;
+    if (R == 0) return GRASS_OK;
+    if (R > (1u << 17)) return GRASS_EINVAL;
+    uint64_t Ebound, Ibound, Dbound;
+    uint32_t H;
+    int rc = khop_bound(g, reqs, R, &Ebound, &Ibound, &Dbound, &H);
+    if (rc) return rc;
+    out->max_hops = H;
+    if (H == 0) return GRASS_OK;
+    if (!out->offsets_dev || (out->capacity && (!out->src_dev || !out->dst_dev))) return GRASS_EINVAL;
+    if (Dbound >= 0xFFFFFFFFull) return GRASS_EINVAL;  // positions are 32-bit in the dedup table
+
+    std::vector<KhReq> hreq(R);
+    std::vector<uint64_t> sbeg(R + 1, 0);
+    for (uint32_t r = 0; r < R; ++r) {
+        KhReq q{};
+        q.seeds = reqs[r].seeds_dev;
+        q.seed_begin = sbeg[r];
+        q.inst = reqs[r].instance;
+        q.hops = reqs[r].hops;
+        for (int h = 0; h < GRASS_KHOP_MAX_HOPS; ++h) q.fan[h] = h < (int)reqs[r].hops ? reqs[r].fanout[h] : 0;
+        hreq[r] = q;
+        sbeg[r + 1] = sbeg[r] + reqs[r].num_seeds;
+    }
+    const uint64_t S = sbeg[R];
+    const uint64_t Imax = std::max<uint64_t>(Ibound, 1), Dmax = std::max<uint64_t>(Dbound, 1);
+    const uint64_t nbI = scan_blocks(Imax), nbD = scan_blocks(Dmax);
+
+    std::lock_guard<std::mutex> lock(g->mu);
+    size_t need = Carve::need(R * sizeof(KhReq)) + Carve::need((R + 1) * 8) * 4 +
+                  Carve::need(S * 4 + 4) + Carve::need(Imax * 4) * 4 + Carve::need((Imax + 1) * 8) +
+                  Carve::need(Dmax * 4) + Carve::need((Dmax + 1) * 8) +
+                  Carve::need(std::max(nbI, nbD) * 8) + Carve::need(64 * 8);
+    hipError_t e = g->scratch.reserve(need);
+    if (e != hipSuccess) return GRASS_ENOMEM;
+    Carve cv(g->scratch.ptr);
+    KhReq* d_req = cv.take<KhReq>(R);
+    uint64_t* d_sbeg = cv.take<uint64_t>(R + 1);
+    uint64_t* d_foff[2] = {cv.take<uint64_t>(R + 1), cv.take<uint64_t>(R + 1)};
+    uint64_t* d_eoff_r = cv.take<uint64_t>(R + 1);
+    uint32_t* d_seedv = cv.take<uint32_t>(S + 1);
+    uint32_t* d_fv[2] = {cv.take<uint32_t>(Imax), cv.take<uint32_t>(Imax)};
+    uint32_t* d_fr[2] = {cv.take<uint32_t>(Imax), cv.take<uint32_t>(Imax)};
+    uint64_t* d_eoff = cv.take<uint64_t>(Imax + 1);
+    uint32_t* d_flag = cv.take<uint32_t>(Dmax);
+    uint64_t* d_fpos = cv.take<uint64_t>(Dmax + 1);
+    uint64_t* d_part = cv.take<uint64_t>(std::max(nbI, nbD));
+    uint64_t* d_sc = cv.take<uint64_t>(64);  // [0..1] item counts, [2] edges total, [3] flags total,
+                                             // [8..8+H] hop bases
+    uint64_t* d_icount[2] = {d_sc + 0, d_sc + 1};
+    uint64_t* d_etot = d_sc + 2;
+    uint64_t* d_ftot = d_sc + 3;
+    uint64_t* d_hb = d_sc + 8;
+
+    HashTab ht;
+    e = ensure_table(g, Dmax, s, ht);
+    if (e != hipSuccess) return GRASS_ENOMEM;
+
+    e = hipMemcpyAsync(d_req, hreq.data(), R * sizeof(KhReq), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_sbeg, sbeg.data(), (R + 1) * 8, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return status_of(e);
+    hipLaunchKernelGGL(fill_kernel<uint64_t>, dim3(1), dim3(64), 0, s, d_sc, (uint64_t)64, (uint64_t)0);
+
+    const unsigned gridE = 2048;
+    // frontier 0: seeds, deduplicated
+    const uint32_t n32 = (uint32_t)g->n;
+    if (S) hipLaunchKernelGGL(seed_gather_kernel, dim3(gridE), dim3(kKhBlock), 0, s, d_req, d_sbeg, R, S, n32,
+                              d_seedv, g->d_err);
+    {
+        DedupIn in{d_seedv, nullptr, S};
+        uint32_t gen = next_gen(g, ht, s);
+        hipLaunchKernelGGL(dedup_insert_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, Count{S, nullptr},
+                           d_sbeg, R, ht, gen, g->d_err);
+        hipLaunchKernelGGL(dedup_mark_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, Count{S, nullptr},
+                           d_sbeg, R, ht, gen, d_flag);
+        e = exclusive_scan(FlagGen{d_flag}, Count{S, nullptr}, Dmax, d_fpos, d_part, d_ftot, s);
+        if (e != hipSuccess) return status_of(e);
+        hipLaunchKernelGGL(compact_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, Count{S, nullptr}, d_sbeg, R,
+                           d_flag, d_fpos, d_fv[0], d_fr[0]);
+        hipLaunchKernelGGL(front_offsets_kernel, dim3((R + 256) / 256), dim3(256), 0, s, d_sbeg, R, d_fpos,
+                           d_foff[0], d_icount[0]);
+    }
+    bool wide = false;
+    for (uint32_t r = 0; r < R; ++r)
+        for (uint32_t h = 0; h < reqs[r].hops; ++h) wide = wide || reqs[r].fanout[h] > 32;
+    const void* skern = g->fmt == kLE ? (wide ? (const void*)sample_kernel<kLE, 64> : (const void*)sample_kernel<kLE, 32>)
+                                      : (wide ? (const void*)sample_kernel<kGT, 64> : (const void*)sample_kernel<kGT, 32>);
+    const unsigned gridS = persistent_grid(skern, kKhBlock);
+    for (uint32_t h = 0; h < H; ++h) {
+        const int cur = h & 1, nxt = cur ^ 1;
+        CountGen cg{d_fv[cur], d_fr[cur], d_req, g->d_deg, h};
+        e = exclusive_scan(cg, Count{0, d_icount[cur]}, Imax, d_eoff, d_part, d_etot, s);
+        if (e != hipSuccess) return status_of(e);
+        hipLaunchKernelGGL(hop_offsets_kernel, dim3((R + 256) / 256), dim3(256), 0, s, d_foff[cur], R, d_eoff,
+                           d_eoff_r, d_hb, h, out->offsets_dev);
+        if (g->fmt == kLE)
+            launch_sample<kLE>(g, wide, gridS, s, d_fv[cur], d_fr[cur], d_icount[cur], d_req, h, seed, d_eoff,
+                               d_hb, out->src_dev, out->dst_dev, out->capacity);
+        else
+            launch_sample<kGT>(g, wide, gridS, s, d_fv[cur], d_fr[cur], d_icount[cur], d_req, h, seed, d_eoff,
+                               d_hb, out->src_dev, out->dst_dev, out->capacity);
+        if (h + 1 == H) break;
+        DedupIn in{out->dst_dev, d_hb + h, out->capacity};
+        Count E{0, d_etot};
+        uint32_t gen = next_gen(g, ht, s);
+        hipLaunchKernelGGL(dedup_insert_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, E, d_eoff_r, R, ht, gen,
+                           g->d_err);
+        hipLaunchKernelGGL(dedup_mark_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, E, d_eoff_r, R, ht, gen,
+                           d_flag);
+        e = exclusive_scan(FlagGen{d_flag}, E, Dmax, d_fpos, d_part, d_ftot, s);
+        if (e != hipSuccess) return status_of(e);
+        hipLaunchKernelGGL(compact_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, E, d_eoff_r, R, d_flag, d_fpos,
+                           d_fv[nxt], d_fr[nxt]);
+        hipLaunchKernelGGL(front_offsets_kernel, dim3((R + 256) / 256), dim3(256), 0, s, d_eoff_r, R, d_fpos,
+                           d_foff[nxt], d_icount[nxt]);
+    }
+    e = hipGetLastError();
+    uint64_t total = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, d_hb + H, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return status_of(e);
+    out->total = total;
+    return total > out->capacity ? GRASS_ECAPACITY : GRASS_OK;
+}
+
+}  // namespace grass

@@ -74,6 +74,9 @@ struct grass_graph {
     hipStream_t stream = nullptr;
     std::mutex mu;
     grass::Scratch scratch;  // per-call working memory (decode, samplers)
+    grass::Scratch ht;       // k-hop dedup hash table (generation-tagged, reused across calls)
+    uint64_t ht_slots = 0;
+    uint32_t ht_gen = 0xFFFF;
 
     grass::DevGraph dev() const {
         grass::DevGraph d;
@@ -89,6 +92,17 @@ struct grass_graph {
 namespace grass {
 
 int status_of(hipError_t e);
+
+// Workgroups that fill the current device once at the kernel's occupancy (persistent grids).
+inline unsigned persistent_grid(const void* kernel, int block) {
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, 0);
+    if (cus <= 0) cus = 1;
+    if (per <= 0) per = 1;
+    return (unsigned)(cus * per);
+}
 
 // loader.hip
 struct ByteSource {
@@ -109,5 +123,11 @@ int random_walk(grass_graph* g, const uint32_t* starts, uint64_t walkers, uint32
                 uint64_t* bytes);
 int random_walk_batch(grass_graph* g, const grass_rw_request* reqs, uint32_t count, uint64_t seed,
                       hipStream_t stream, uint64_t* bytes);
+
+// khop.hip
+int khop_bound(const grass_graph* g, const grass_khop_request* reqs, uint32_t count,
+               uint64_t* edges, uint64_t* items, uint64_t* dedup, uint32_t* max_hops);
+int khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t count, uint64_t seed,
+               grass_khop_output* out, hipStream_t stream);
 
 }  // namespace grass

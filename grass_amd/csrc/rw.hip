@@ -95,10 +95,6 @@ struct RwDesc {
     uint32_t instance;
 };
 
-__global__ void zero_u64_kernel(unsigned long long* p, int count) {
-    if ((int)threadIdx.x < count) p[threadIdx.x] = 0;
-}
-
 template <int FMT, bool COUNT>
 __global__ __launch_bounds__(kRwBlock) void rw_batch_kernel(DevGraph g, const RwDesc* __restrict__ reqs,
                                                             uint32_t R, uint64_t ngran,
@@ -161,22 +157,12 @@ int random_walk(grass_graph* g, const uint32_t* starts, uint64_t W, uint32_t L, 
     std::lock_guard<std::mutex> lock(g->mu);
     if (g->scratch.reserve(Carve::need(8)) != hipSuccess) return GRASS_ENOMEM;
     unsigned long long* d_bytes = (unsigned long long*)g->scratch.ptr;
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, d_bytes, 1);
+    hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, s, d_bytes, (uint64_t)1, 0ull);
     launch_rw<true>(g, grid, s, starts, W, L, seed, inst, nullptr, d_bytes);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(bytes, d_bytes, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     return status_of(e);
-}
-
-static unsigned persistent_grid(const void* kernel, int block) {
-    int dev = 0, cus = 0, per = 0;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, 0);
-    if (cus <= 0) cus = 1;
-    if (per <= 0) per = 1;
-    return (unsigned)(cus * per);
 }
 
 template <int FMT, bool COUNT>
@@ -222,7 +208,7 @@ int random_walk_batch(grass_graph* g, const grass_rw_request* reqs, uint32_t cou
     unsigned long long* d_ctr = c.take<unsigned long long>(2);  // [0] queue head, [1] bytes
     e = hipMemcpyAsync(d_desc, desc.data(), desc.size() * sizeof(RwDesc), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return status_of(e);
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, d_ctr, 2);
+    hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, s, d_ctr, (uint64_t)2, 0ull);
     const uint32_t R = (uint32_t)desc.size();
     if (g->fmt == kLE) {
         if (bytes) launch_rw_batch<kLE, true>(g, s, d_desc, R, gran, d_ctr, seed, d_ctr + 1);

@@ -119,6 +119,47 @@ int grass_random_walk_bytes(grass_graph *g, const uint32_t *starts_dev, uint64_t
 int grass_random_walk_batch_bytes(grass_graph *g, const grass_rw_request *reqs, uint32_t count,
                                   uint64_t seed, void *stream, uint64_t *bytes);
 
+/* ------------------------------------------------------------------ k-hop sampling */
+
+#define GRASS_KHOP_MAX_HOPS 8
+
+/* One k-hop request.  Frontier 0 is the seeds with duplicates dropped (first appearance kept);
+ * hop h draws, for every frontier vertex f in order, min(fanout[h], deg f) distinct neighbour
+ * positions with Floyd's algorithm (draw k from Philox(seed, {instance, f, h, k})) and emits
+ * (f, neighbour) in ascending position; frontier h+1 is hop h's destinations with duplicates
+ * dropped (first appearance kept).  fanout[h] <= 64. */
+typedef struct {
+    const uint32_t *seeds_dev;
+    uint64_t num_seeds;
+    uint32_t instance;
+    uint32_t hops;
+    uint32_t fanout[GRASS_KHOP_MAX_HOPS];
+} grass_khop_request;
+
+/* Output of a batch, hop-major: the edges of (hop h, request r) are
+ * [offsets_dev[h*(count+1)+r], offsets_dev[h*(count+1)+r+1]) of src_dev/dst_dev.
+ * offsets_dev holds max_hops*(count+1) entries (max_hops = the largest `hops` of the batch). */
+typedef struct {
+    uint32_t *src_dev;
+    uint32_t *dst_dev;
+    uint64_t capacity;     /* entries of src_dev and dst_dev */
+    uint64_t *offsets_dev;
+    uint64_t total;        /* out: edges of the batch (may exceed capacity: GRASS_ECAPACITY) */
+    uint32_t max_hops;     /* out */
+} grass_khop_output;
+
+/* Upper bound on the edges a batch can produce (for sizing `capacity`). */
+int grass_khop_bound(const grass_graph *g, const grass_khop_request *reqs, uint32_t count,
+                     uint64_t *max_edges);
+
+/* Runs every request of the batch in the same launches (a fixed number per hop whatever the
+ * batch size; counts stay on the device).  Each request's edges are identical to running it
+ * alone.  Replaces KHopApp (LE/src/apps/khop/khop.cu:1-69: fanout [10,25] with replacement)
+ * driven by doTransitParallelSampling (LE/src/grass.cu:1299-1900).  Synchronises to report
+ * `total`. */
+int grass_khop_batch(grass_graph *g, const grass_khop_request *reqs, uint32_t count, uint64_t seed,
+                     grass_khop_output *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
