@@ -48,11 +48,15 @@ def main():
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--fmt", default="gt", choices=["le", "gt"])
     ap.add_argument("--rw-requests", type=int, default=256)
+    ap.add_argument("--khop-requests", type=int, default=256)
     ap.add_argument("--threads", type=int, default=16, help="host threads for ingest")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--decode", action="store_true", help="also time the full decode to CSR")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 256 MiB device copy (a known-size kernel that scales rocprof "
+                         "FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -68,6 +72,14 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(0)
     seed = 42
+
+    if args.calib_copy:
+        a = torch.ones(64 << 20, dtype=torch.int32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        log("calibration copy: 256 MiB read + 256 MiB written")
 
     t0 = time.time()
     gb, ob, csr = ingest.rmat_graph(args.scale, 16, fmt=args.fmt, seed=seed, threads=args.threads)
@@ -87,16 +99,49 @@ def main():
     stream = torch.cuda.current_stream()
     sptr = stream.cuda_stream
     L = grass_amd.lib()
+    C = grass_amd.C
+
+    # 2-hop requests: seed counts lognormal(6, 1.2) clipped to [1, 8192], fanout [25, 10]
+    R2 = args.khop_requests
+    kseeds = [rng.integers(0, g.n, size=int(np.clip(round(x), 1, 8192))).astype(np.uint32)
+              for x in rng.lognormal(6.0, 1.2, size=R2)]
+    kfans = [[25, 10]] * R2
+    karr, kkeep = g.khop_request_array(kseeds, kfans, instances=list(range(10000, 10000 + R2)))
+    kcap = g.khop_bound(karr, R2) if R2 else 0
+    ksrc = torch.empty(max(kcap, 1), dtype=torch.int32, device=dev)
+    kdst = torch.empty(max(kcap, 1), dtype=torch.int32, device=dev)
+    koff = torch.zeros(max(2 * (R2 + 1), 1), dtype=torch.int64, device=dev)
+    kout = grass_amd.KhopOutput(ksrc.data_ptr(), kdst.data_ptr(), kcap, koff.data_ptr(), 0, 0)
+    # one request at a time (plain): its own output slice of the same buffers
+    kplain = []
+    for i in range(R2):
+        a1 = (grass_amd.KhopRequest * 1)(karr[i])
+        kplain.append(a1)
+    khop_edges = 0
+
+    def khop_batched():
+        grass_amd._check(L.grass_khop_batch(g.handle, karr, R2, seed, C.byref(kout), C.c_void_p(sptr)),
+                         "khop batch")
+
+    def khop_plain():
+        for a1 in kplain:
+            o1 = grass_amd.KhopOutput(ksrc.data_ptr(), kdst.data_ptr(), kcap, koff.data_ptr(), 0, 0)
+            grass_amd._check(L.grass_khop_batch(g.handle, a1, 1, seed, C.byref(o1), C.c_void_p(sptr)),
+                             "khop plain")
 
     def batched_step():
         grass_amd._check(L.grass_random_walk_batch(g.handle, arr, len(reqs), seed,
-                                                   grass_amd.C.c_void_p(sptr)), "batch")
+                                                   C.c_void_p(sptr)), "batch")
+        if R2:
+            khop_batched()
 
     def plain_step():
         for r, o in zip(reqs, outs):
             grass_amd._check(L.grass_random_walk(g.handle, None, r["walkers"], r["length"], seed,
-                                                 r["instance"], grass_amd.C.c_void_p(o.data_ptr()),
-                                                 grass_amd.C.c_void_p(sptr)), "plain")
+                                                 r["instance"], C.c_void_p(o.data_ptr()),
+                                                 C.c_void_p(sptr)), "plain")
+        if R2:
+            khop_plain()
 
     def timed(fn, steps, warmup, events=False):
         for _ in range(warmup):
@@ -123,24 +168,36 @@ def main():
     batched_step()
     torch.cuda.synchronize()
     ref0 = outs[0].clone()
-    plain_step()
-    torch.cuda.synchronize()
-    assert torch.equal(ref0, outs[0]), "batched and plain outputs differ"
+    khop_edges = kout.total
+    kref = (kdst[:khop_edges].clone(), koff.clone()) if R2 else None
+    if not args.no_plain:
+        plain_step()
+        torch.cuda.synchronize()
+        assert torch.equal(ref0, outs[0]), "batched and plain walk outputs differ"
     g.sync()
+    log(f"2-hop batch: {R2} requests, {sum(len(s) for s in kseeds)} seeds, {khop_edges} edges")
 
+    # the two request kinds are timed separately (HIP events on the launching stream), then together
+    _, ev_rw = timed(lambda: L.grass_random_walk_batch(g.handle, arr, len(reqs), seed, C.c_void_p(sptr)),
+                     args.steps, args.warmup, events=True)
+    ev_kh = [0.0]
+    if R2:
+        _, ev_kh = timed(khop_batched, args.steps, args.warmup, events=True)
     wall_b, ev_b = timed(batched_step, args.steps, args.warmup, events=True)
+    edges_step = rw_edges + khop_edges
     ms_step = wall_b / args.steps * 1e3
-    value = rw_edges * args.steps / wall_b
-    log(f"batched: {ms_step:.3f} ms/step, {value/1e9:.3f} G edges/s")
+    value = edges_step * args.steps / wall_b
+    log(f"batched: {ms_step:.3f} ms/step, {value/1e9:.3f} G edges/s "
+        f"(rw {np.mean(ev_rw):.3f} ms for {rw_edges} edges, 2-hop {np.mean(ev_kh):.3f} ms for {khop_edges})")
     plain = None
     if not args.no_plain:
-        wall_p, _ = timed(plain_step, args.steps, args.warmup)
-        plain = rw_edges * args.steps / wall_p
-        log(f"plain: {wall_p/args.steps*1e3:.3f} ms/step, {plain/1e9:.3f} G edges/s")
+        wall_p, _ = timed(plain_step, max(1, args.steps // 2), 1)
+        plain = edges_step * max(1, args.steps // 2) / wall_p
+        log(f"plain: {wall_p/max(1, args.steps // 2)*1e3:.3f} ms/step, {plain/1e9:.3f} G edges/s")
 
     # roofline of the dominant kernel (the persistent random-walk kernel)
     bytes_launch = g.random_walk_batch_bytes(reqs, seed)
-    kern_ms = float(np.mean(ev_b))
+    kern_ms = float(np.mean(ev_rw))
     achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
     roof = {"kernel": "rw_batch_kernel", "bound": "hbm", "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -198,11 +255,14 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32",
         "data": f"synthetic R-MAT s{args.scale} ef16 (.57,.19,.19,.05) seed 42, graph_sort, "
-                f"{args.fmt.upper()} compressor; Philox-drawn walk starts",
-        "config": {"workload": f"batched random-walk requests on R-MAT-{args.scale} "
+                f"{args.fmt.upper()} compressor; Philox-drawn walk starts, uniform 2-hop seeds",
+        "config": {"workload": f"batched random-walk + 2-hop [25,10] requests on R-MAT-{args.scale} "
                                f"({args.fmt.upper()})", "scale": args.scale, "edge_factor": 16,
                    "format": args.fmt, "n": g.n, "m": g.m, "rw_requests": len(reqs),
-                   "rw_edges_per_step": rw_edges, "khop_requests": 0},
+                   "rw_edges_per_step": rw_edges, "khop_requests": R2,
+                   "khop_seeds": int(sum(len(s) for s in kseeds)),
+                   "khop_edges_per_step": khop_edges},
+        "rw_ms": round(float(np.mean(ev_rw)), 4), "khop_ms": round(float(np.mean(ev_kh)), 4),
         "plain_value": plain, "batched_over_plain": (value / plain) if plain else None,
         "p99_ms": None,
         "roofline": roof, "cpu_baseline": cpu,

@@ -69,6 +69,7 @@ void free_graph(grass_graph* g) {
     if (g->d_words) (void)hipFree(g->d_words);
     if (g->d_start) (void)hipFree(g->d_start);
     if (g->d_deg) (void)hipFree(g->d_deg);
+    if (g->d_hdr) (void)hipFree(g->d_hdr);
     if (g->d_err) (void)hipFree(g->d_err);
     g->scratch.release();
     g->ht.release();

@@ -29,6 +29,7 @@ struct DevGraph {
     const uint32_t* __restrict__ w;      // big-endian word view of .graph, padded
     const uint64_t* __restrict__ start;  // bit start of record v, n + 1 entries
     const uint32_t* __restrict__ deg;    // degree of v, decoded and validated at load
+    const uint4* __restrict__ hdr;       // decoded header of v (see pack_hdr), built at load
     uint32_t n;
     int fmt;
 };
@@ -41,6 +42,53 @@ struct Hdr {
     uint32_t C, q, R;  // LE: chunks, q = floor((d-1)/C); GT: blocks, q = floor(d/C); R remainder
     uint32_t hbits;    // header length in bits
 };
+
+// Decoded-header cache: one 16-byte entry per vertex, so that a sampling step reads one aligned
+// dwordx4 instead of walking four dependent gamma codes through the record.
+//   x = payload bit [31:0]          (LE d == 1: the neighbour itself)
+//   y = payload bit [39:32] | f1 << 8 | f2 << 14 | header bits << 20
+//   z = d
+//   w = C | first_pos << 16          (C, first_pos <= sqrt(2^31) + 2 < 2^16)
+// Payload bits must be < 2^40 (a .graph file below 128 GiB); f1, f2 <= 31; header <= 252 bits.
+__host__ __device__ __forceinline__ uint4 pack_hdr(const Hdr& h, int fmt) {
+    uint4 e;
+    e.z = h.d;
+    if (fmt == kLE && h.d == 1) {
+        e.x = h.f1;
+        e.y = h.hbits << 20;
+        e.w = 0;
+        return e;
+    }
+    e.x = (uint32_t)h.pay;
+    e.y = (uint32_t)(h.pay >> 32) | (h.f1 << 8) | (h.f2 << 14) | (h.hbits << 20);
+    e.w = h.C | (h.fp << 16);
+    return e;
+}
+
+template <int FMT>
+__device__ __forceinline__ void unpack_hdr(uint4 e, Hdr& h) {
+    h.d = e.z;
+    h.hbits = e.y >> 20;
+    if (FMT == kLE && h.d == 1) {
+        h.f1 = e.x;
+        h.f2 = h.fp = h.C = h.q = h.R = 0;
+        h.pay = 0;
+        return;
+    }
+    h.pay = ((uint64_t)(e.y & 0xFFu) << 32) | e.x;
+    h.f1 = (e.y >> 8) & 63u;
+    h.f2 = (e.y >> 14) & 63u;
+    h.C = e.w & 0xFFFFu;
+    h.fp = e.w >> 16;
+    const uint32_t D = FMT == kLE ? h.d - 1 : h.d;
+    h.q = h.C ? D / h.C : 0u;
+    h.R = D - h.C * h.q;
+}
+
+template <int FMT>
+__device__ __forceinline__ void load_hdr(const DevGraph& g, uint32_t v, Hdr& h) {
+    unpack_hdr<FMT>(g.hdr[v], h);
+}
 
 __host__ __device__ __forceinline__ uint32_t isqrt_u32(uint32_t x) {
     uint32_t s = (uint32_t)sqrtf((float)x);
@@ -77,6 +125,14 @@ __device__ __forceinline__ uint32_t peek32(const uint32_t* __restrict__ w, uint6
     const uint32_t* p = w + (b >> 5);
     uint64_t x = ((uint64_t)p[0] << 32) | p[1];
     return (uint32_t)(x >> (32 - (uint32_t)(b & 31)));
+}
+
+// 64-bit window at bit b (three words).
+__device__ __forceinline__ uint64_t peek64(const uint32_t* __restrict__ w, uint64_t b) {
+    const uint32_t* p = w + (b >> 5);
+    const uint32_t s = (uint32_t)(b & 31);
+    uint64_t x = ((uint64_t)p[0] << 32) | p[1];
+    return s ? (x << s) | (p[2] >> (32 - s)) : x;
 }
 
 // nbits in [1, 32].
@@ -172,14 +228,15 @@ __device__ __forceinline__ uint32_t le_get(const uint32_t* __restrict__ w, const
     uint32_t c, i, len;
     le_locate(h, k, c, i, len);
     const uint32_t rb = h.f1;
-    uint64_t rbit = h.pay + (uint64_t)c * rb;
-    uint32_t r1 = read_bits(w, rbit, rb);
+    // rocks c and c+1 are adjacent fields of rb <= 31 bits: one 64-bit window holds both
+    const uint64_t win = peek64(w, h.pay + (uint64_t)c * rb);
+    uint32_t r1 = (uint32_t)(win >> (64 - rb));
     uint32_t a = c < h.fp ? v - r1 : v + r1;
     if (i == 0) {
         if (fbits) *fbits = rb;
         return a;
     }
-    uint32_t r2 = read_bits(w, rbit + rb, rb);
+    uint32_t r2 = (uint32_t)((win << rb) >> (64 - rb));
     uint32_t z = c + 1 < h.fp ? v - r2 : v + r2;
     uint32_t j = (h.q - 1) * c + (c < h.R ? c : h.R) + i - 1;
     uint64_t so = h.pay + (uint64_t)(h.C + 1) * rb + (uint64_t)j * (h.f2 + 1);

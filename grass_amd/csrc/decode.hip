@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
     uint64_t eb = e0;
     if (FMT == kGT) {
         Hdr h;
-        parse_header<kGT>(g.w, g.start[vlo], h);
+        load_hdr<kGT>(g, vlo, h);
         uint32_t blk, i;
         gt_locate(h, (uint32_t)(e0 - rowptr[vlo]), blk, i);
         eb = e0 - i;
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
         const uint32_t v = owner_of(rowptr, vlo, vhi, ee);
         const uint32_t k = (uint32_t)(ee - rowptr[v]);
         Hdr h;
-        parse_header<FMT>(g.w, g.start[v], h);
+        load_hdr<FMT>(g, v, h);
         if (FMT == kLE) {
             uint32_t x = le_get(g.w, h, v, k);
             if (act && e >= e0) col[e] = x;

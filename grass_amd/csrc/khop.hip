@@ -273,7 +273,7 @@ __global__ __launch_bounds__(kKhBlock) void sample_kernel(DevGraph g, const uint
         }
         if (gl < (int)c) {
             Hdr hd;
-            parse_header<FMT>(g.w, g.start[v], hd);
+            load_hdr<FMT>(g, v, hd);
             uint32_t nb = get_neighbor<FMT>(g.w, hd, v, x);
             if (nb >= g.n) {  // never let a bad id reach the next frontier
                 atomicCAS(err, kErrNone, kErrBadNeighbor);

@@ -46,6 +46,7 @@ __global__ __launch_bounds__(kValBlock) void validate_kernel(const uint32_t* __r
                                                              const uint64_t* __restrict__ start,
                                                              uint32_t n, uint64_t gbits,
                                                              uint32_t* __restrict__ deg,
+                                                             uint4* __restrict__ hdr,
                                                              unsigned long long* stats) {
     __shared__ unsigned long long lds[kValBlock / kWave];
     unsigned long long sum = 0, mx = 0, bad = 0;
@@ -84,9 +85,12 @@ __global__ __launch_bounds__(kValBlock) void validate_kernel(const uint32_t* __r
         if (!ok) {
             ++bad;
             atomicMin(&stats[3], (unsigned long long)v);
-            h.d = 0;
+            h.d = h.f1 = h.f2 = h.fp = h.C = 0;
+            h.pay = b0;
         }
+        h.hbits = (uint32_t)(h.pay - b0);
         deg[v] = h.d;
+        hdr[v] = pack_hdr(h, FMT);
         sum += h.d;
         mx = h.d > mx ? h.d : mx;
     }
@@ -169,6 +173,7 @@ uint64_t host_field(const uint8_t* tailb, uint64_t tlen, uint64_t file_bytes, ui
 int load_graph(grass_graph* g, ByteSource& gsrc, ByteSource& osrc) {
     const uint64_t gbytes = gsrc.size(), obytes = osrc.size();
     if (obytes < 2 || gbytes < 1) return GRASS_EFORMAT;
+    if (gbytes >= (1ull << 37)) return GRASS_EFORMAT;  // payload bits must fit the 40-bit cache field
     hipStream_t s = g->stream;
     const uint64_t gwords = (gbytes + 3) / 4 + kPadWords;
     const uint64_t owords = (obytes + 3) / 4 + kPadWords;
@@ -214,6 +219,7 @@ int load_graph(grass_graph* g, ByteSource& gsrc, ByteSource& osrc) {
 
     if (hipMalloc(&g->d_start, (n + 1) * 8) != hipSuccess) return fail(GRASS_ENOMEM);
     if (hipMalloc(&g->d_deg, n * 4) != hipSuccess) return fail(GRASS_ENOMEM);
+    if (hipMalloc(&g->d_hdr, n * 16) != hipSuccess) return fail(GRASS_ENOMEM);
     const uint64_t nb = scan_blocks(n);
     uint64_t* d_tmp = nullptr;  // partials[nb], total, stats[4]
     if (hipMalloc(&d_tmp, (nb + 8) * 8) != hipSuccess) return fail(GRASS_ENOMEM);
@@ -228,10 +234,10 @@ int load_graph(grass_graph* g, ByteSource& gsrc, ByteSource& osrc) {
         unsigned grid = (unsigned)std::min<uint64_t>((n + kValBlock - 1) / kValBlock, 16384);
         if (g->fmt == kLE)
             hipLaunchKernelGGL(validate_kernel<kLE>, dim3(grid), dim3(kValBlock), 0, s, g->d_words,
-                               g->d_start, (uint32_t)n, 8 * gbytes, g->d_deg, d_stats);
+                               g->d_start, (uint32_t)n, 8 * gbytes, g->d_deg, g->d_hdr, d_stats);
         else
             hipLaunchKernelGGL(validate_kernel<kGT>, dim3(grid), dim3(kValBlock), 0, s, g->d_words,
-                               g->d_start, (uint32_t)n, 8 * gbytes, g->d_deg, d_stats);
+                               g->d_start, (uint32_t)n, 8 * gbytes, g->d_deg, g->d_hdr, d_stats);
         e = hipGetLastError();
     }
     uint64_t total = 0;
@@ -248,7 +254,7 @@ int load_graph(grass_graph* g, ByteSource& gsrc, ByteSource& osrc) {
     g->total_bits = total;
     g->m = stats[0];
     g->max_degree = stats[1];
-    g->device_bytes = gwords * 4 + (n + 1) * 8 + n * 4;
+    g->device_bytes = gwords * 4 + (n + 1) * 8 + n * 4 + n * 16;
     return GRASS_OK;
 }
 

@@ -70,6 +70,7 @@ struct grass_graph {
     uint32_t* d_words = nullptr;
     uint64_t* d_start = nullptr;
     uint32_t* d_deg = nullptr;
+    uint4* d_hdr = nullptr;  // decoded-header cache, 16 B per vertex
     uint32_t* d_err = nullptr;  // [0] first error kind, [1] detail
     hipStream_t stream = nullptr;
     std::mutex mu;
@@ -83,6 +84,7 @@ struct grass_graph {
         d.w = d_words;
         d.start = d_start;
         d.deg = d_deg;
+        d.hdr = d_hdr;
         d.n = (uint32_t)n;
         d.fmt = fmt;
         return d;

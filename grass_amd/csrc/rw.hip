@@ -1,11 +1,19 @@
 // Uniform random walks over the compressed graph.
 //
-// One lane per walker; every step is one header parse plus one random-access decode of the
-// current vertex's record, so no transit grouping, sort or host round trip is needed between
-// steps (the reference regroups walkers by transit with a radix sort, run-length encode and
-// three D2H copies per step, LE/src/grass.cu:1453-1861).  Step s of walker w draws its
-// neighbour index from Philox(seed, {instance, w, s, 0}) (DeepWalkApp::next,
-// LE/src/apps/randomwalk/randomWalks.cu:62-79, with cuRAND replaced).
+// One lane per walker.  A step reads the current vertex's 16-byte decoded header (built at
+// load) and then only the payload fields of the drawn neighbour, so no transit grouping, sort
+// or host round trip is needed between steps (the reference regroups walkers by transit with a
+// radix sort, run-length encode and three D2H copies per step, LE/src/grass.cu:1453-1861).
+// Step s of walker w draws its neighbour index from Philox(seed, {instance, w, s, 0})
+// (DeepWalkApp::next, LE/src/apps/randomwalk/randomWalks.cu:62-79, with cuRAND replaced).
+//
+// The 64 walkers of a wave belong to one request and so have one length: the step loop is
+// wave-uniform, which allows
+//   * staged output: each lane puts its walk values in an LDS tile of kStage columns and the
+//     wave writes whole 64-byte row segments (the walk matrix is row-major, so one column
+//     written straight from 64 lanes would touch 64 lines);
+//   * GT gap runs: a neighbour deep inside a long GT block needs the sum of i gaps; runs longer
+//     than kCoopMin are summed by the whole wave, 64 gaps per pass, one run at a time.
 //
 // plain:   one launch, one request (grass_random_walk);
 // batched: many requests of different walker counts and lengths in one persistent launch;
@@ -28,41 +36,113 @@
 namespace grass {
 
 constexpr int kRwBlock = 256;
+constexpr int kRwWaves = kRwBlock / kWave;
+constexpr uint32_t kStage = 16;             // walk columns staged per flush (64 bytes a row)
+constexpr uint32_t kPitch = kStage + 1;     // LDS row pitch (odd: spreads lanes over banks)
+constexpr uint32_t kCoopMin = 48;           // GT gap runs longer than this use the whole wave
 
 __device__ __forceinline__ uint64_t bytes_of_bits(uint32_t bits) { return (bits + 7) >> 3; }
 
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Writes staged columns [c0, c0 + ncols) of the wave's rows: 16 lanes per row, 4 rows per store.
+__device__ __forceinline__ void flush_tile(const uint32_t* __restrict__ tile, uint32_t* __restrict__ row0,
+                                           uint64_t stride, uint32_t nrows, uint32_t c0, uint32_t ncols) {
+    wave_sync();
+    const int lane = lane_id();
+    const uint32_t col = lane & 15, r4 = lane >> 4;
+    for (uint32_t r = r4; r < (uint32_t)kWave; r += 4)
+        if (r < nrows && col < ncols) row0[r * stride + c0 + col] = tile[r * kPitch + col];
+    wave_sync();
+}
+
+// Walks the 64 walkers of one wave (lane = walker); all lanes call it with the same L.
+// row0 is the output row of lane 0's walker; rows are consecutive with stride L + 1.
 template <int FMT, bool COUNT>
-__device__ __forceinline__ uint64_t walk_one(const DevGraph& g, uint32_t v, uint32_t L,
-                                             uint64_t seed, uint32_t inst, uint32_t w,
-                                             uint32_t* __restrict__ row, uint32_t* __restrict__ err) {
-    uint64_t acc = 4;
-    if (!COUNT) row[0] = v;
-    uint32_t s = 0;
-    if (v >= g.n) {
+__device__ __forceinline__ uint64_t walk_wave(const DevGraph& g, bool active, uint32_t v, uint32_t L,
+                                              uint64_t seed, uint32_t inst, uint32_t w,
+                                              uint32_t* __restrict__ row0, uint32_t nrows,
+                                              uint32_t* __restrict__ tile, uint32_t* __restrict__ err) {
+    const int lane = lane_id();
+    const uint64_t stride = (uint64_t)L + 1;
+    uint64_t acc = active ? 4 : 0;
+    bool live = active;
+    if (active && v >= g.n) {
         atomicCAS(err, kErrNone, kErrBadStart);
-        if (!COUNT) row[0] = g.n;
-    } else {
-        for (; s < L; ++s) {
+        live = false;
+    }
+    if (!COUNT) {
+        tile[lane * kPitch] = live ? v : g.n;
+        if (L == 0) flush_tile(tile, row0, stride, nrows, 0, 1);
+    }
+    for (uint32_t s = 0; s < L; ++s) {
+        uint32_t nv = 0;
+        uint64_t job_bit = 0;
+        uint32_t job_len = 0, job_w = 0;
+        if (live) {
             Hdr h;
-            parse_header<FMT>(g.w, g.start[v], h);
+            load_hdr<FMT>(g, v, h);
             if (COUNT) acc += 8 + bytes_of_bits(h.hbits);
-            if (h.d == 0) break;
-            U4 r = philox4x32_10(inst, w, s, 0, seed);
-            uint32_t k = (uint32_t)uniform_index(r.x, r.y, h.d);
-            uint32_t nv, fb = 0;
-            if (FMT == kLE) nv = le_get(g.w, h, v, k, COUNT ? &fb : nullptr);
-            else nv = gt_get(g.w, h, v, k, COUNT ? &fb : nullptr);
+            if (h.d == 0) {
+                live = false;
+            } else {
+                U4 r = philox4x32_10(inst, w, s, 0, seed);
+                const uint32_t k = (uint32_t)uniform_index(r.x, r.y, h.d);
+                if (FMT == kLE) {
+                    uint32_t fb = 0;
+                    nv = le_get(g.w, h, v, k, COUNT ? &fb : nullptr);
+                    if (COUNT) acc += bytes_of_bits(fb) + 4;
+                } else {
+                    uint32_t blk, i;
+                    gt_locate(h, k, blk, i);
+                    const uint64_t bo = gt_block_bit(h, blk);
+                    nv = gt_head(g.w, h, v, blk, bo);
+                    if (i > kCoopMin) {
+                        job_bit = bo + h.f1;
+                        job_len = i;
+                        job_w = h.f2;
+                    } else if (i) {
+                        nv += sum_fields(g.w, bo + h.f1, i, h.f2) + i;
+                    }
+                    if (COUNT) acc += bytes_of_bits(h.f1 + i * h.f2) + 4;
+                }
+            }
+        }
+        if (FMT == kGT) {
+            // long gap runs: the wave sums each lane's run in turn, 64 gaps per pass
+            uint64_t jobs = __ballot(job_len != 0);
+            while (jobs) {
+                const int l = __ffsll((long long)jobs) - 1;
+                jobs &= jobs - 1;
+                const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)job_bit, l);
+                const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(job_bit >> 32), l);
+                const uint32_t len = __builtin_amdgcn_readlane(job_len, l);
+                const uint32_t wd = __builtin_amdgcn_readlane(job_w, l);
+                const uint64_t base = ((uint64_t)hi << 32) | lo;
+                uint32_t part = 0;
+                for (uint32_t j = lane; j < len; j += kWave) part += read_bits(g.w, base + (uint64_t)j * wd, wd);
+                part = wave_sum(part);
+                if (lane == l) nv += part + len;
+            }
+        }
+        if (live) {
             if (nv >= g.n) {
                 atomicCAS(err, kErrNone, kErrBadNeighbor);
-                break;
+                live = false;
+            } else {
+                v = nv;
             }
-            if (COUNT) acc += bytes_of_bits(fb) + 4;
-            v = nv;
-            if (!COUNT) row[s + 1] = v;
+        }
+        if (!COUNT) {
+            const uint32_t col = s + 1, cc = col % kStage;
+            tile[lane * kPitch + cc] = live ? v : g.n;
+            if (cc == kStage - 1 || col == L) flush_tile(tile, row0, stride, nrows, col - cc, cc + 1);
         }
     }
-    if (!COUNT)
-        for (; s < L; ++s) row[s + 1] = g.n;
     return acc;
 }
 
@@ -72,15 +152,19 @@ __global__ __launch_bounds__(kRwBlock) void rw_kernel(DevGraph g, const uint32_t
                                                       uint32_t inst, uint32_t* __restrict__ out,
                                                       uint32_t* __restrict__ err,
                                                       unsigned long long* bytes) {
+    __shared__ uint32_t tiles[kRwWaves][kWave * kPitch];
     const uint64_t w = (uint64_t)blockIdx.x * kRwBlock + threadIdx.x;
-    uint64_t acc = 0;
-    if (w < W) {
-        uint32_t v = starts ? starts[w] : (uint32_t)uniform(seed, inst, (uint32_t)w, kTagStart, 0, g.n);
-        acc = walk_one<FMT, COUNT>(g, v, L, seed, inst, (uint32_t)w,
-                                   COUNT ? nullptr : out + w * (uint64_t)(L + 1), err);
-        if (starts) acc += 4;
-    }
+    const uint64_t w0 = w & ~(uint64_t)(kWave - 1);
+    if (w0 >= W) return;  // whole wave past the end
+    const bool active = w < W;
+    uint32_t v = 0;
+    if (active) v = starts ? starts[w] : (uint32_t)uniform(seed, inst, (uint32_t)w, kTagStart, 0, g.n);
+    const uint32_t nrows = (uint32_t)std::min<uint64_t>(kWave, W - w0);
+    uint64_t acc = walk_wave<FMT, COUNT>(g, active, v, L, seed, inst, (uint32_t)w,
+                                         COUNT ? nullptr : out + w0 * ((uint64_t)L + 1), nrows,
+                                         tiles[threadIdx.x / kWave], err);
     if (COUNT) {
+        if (active && starts) acc += 4;
         acc = wave_sum(acc);
         if (lane_id() == 0) atomicAdd(bytes, (unsigned long long)acc);
     }
@@ -101,6 +185,8 @@ __global__ __launch_bounds__(kRwBlock) void rw_batch_kernel(DevGraph g, const Rw
                                                             unsigned long long* queue, uint64_t seed,
                                                             uint32_t* __restrict__ err,
                                                             unsigned long long* bytes) {
+    __shared__ uint32_t tiles[kRwWaves][kWave * kPitch];
+    uint32_t* tile = tiles[threadIdx.x / kWave];
     const int lane = lane_id();
     uint64_t acc = 0;
     for (;;) {
@@ -115,14 +201,17 @@ __global__ __launch_bounds__(kRwBlock) void rw_batch_kernel(DevGraph g, const Rw
             else hi = mid - 1;
         }
         const RwDesc d = reqs[lo];
-        const uint64_t w = (gi - d.gran_begin) * kWave + lane;
-        if (w < d.walkers) {
-            uint32_t v = d.starts ? d.starts[w]
-                                  : (uint32_t)uniform(seed, d.instance, (uint32_t)w, kTagStart, 0, g.n);
-            acc += walk_one<FMT, COUNT>(g, v, d.length, seed, d.instance, (uint32_t)w,
-                                        COUNT ? nullptr : d.out + w * (uint64_t)(d.length + 1), err);
-            if (d.starts) acc += 4;
-        }
+        const uint64_t w0 = (gi - d.gran_begin) * kWave;
+        const uint64_t w = w0 + lane;
+        const bool active = w < d.walkers;
+        uint32_t v = 0;
+        if (active)
+            v = d.starts ? d.starts[w] : (uint32_t)uniform(seed, d.instance, (uint32_t)w, kTagStart, 0, g.n);
+        const uint32_t nrows = (uint32_t)std::min<uint64_t>(kWave, d.walkers - w0);
+        acc += walk_wave<FMT, COUNT>(g, active, v, d.length, seed, d.instance, (uint32_t)w,
+                                     COUNT ? nullptr : d.out + w0 * ((uint64_t)d.length + 1), nrows,
+                                     tile, err);
+        if (COUNT && active && d.starts) acc += 4;
     }
     if (COUNT) {
         acc = wave_sum(acc);
