@@ -74,12 +74,14 @@ def main():
     seed = 42
 
     if args.calib_copy:
+        # known-size kernels: a fill writes 256 MiB, a sum reads 256 MiB, a copy does both
         a = torch.ones(64 << 20, dtype=torch.int32, device=dev)
+        s = a.sum()
         b = torch.empty_like(a)
         b.copy_(a)
         torch.cuda.synchronize()
-        del a, b
-        log("calibration copy: 256 MiB read + 256 MiB written")
+        del a, b, s
+        log("calibration: fill 256 MiB, sum over 256 MiB, copy 256 MiB")
 
     t0 = time.time()
     gb, ob, csr = ingest.rmat_graph(args.scale, 16, fmt=args.fmt, seed=seed, threads=args.threads)

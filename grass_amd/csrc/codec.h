@@ -313,4 +313,51 @@ __device__ __forceinline__ uint32_t get_neighbor(const uint32_t* __restrict__ w,
     return FMT == kLE ? le_get(w, h, v, k) : gt_get(w, h, v, k);
 }
 
+// GT gap runs longer than this are summed by the whole wave instead of by their lane.
+constexpr uint32_t kCoopMin = 48;
+
+// Split GT random access: the head of k's block (into `val`) and, when k sits i > 0 places into
+// the block, either the gap sum done here (i <= kCoopMin) or a deferred run (job_*) for
+// coop_gap_sums.  fbits receives the payload bits of the access.
+__device__ __forceinline__ void gt_get_split(const uint32_t* __restrict__ w, const Hdr& h, uint32_t v,
+                                             uint32_t k, uint32_t& val, uint64_t& job_bit,
+                                             uint32_t& job_len, uint32_t& job_w, uint32_t& fbits) {
+    uint32_t blk, i;
+    gt_locate(h, k, blk, i);
+    const uint64_t bo = gt_block_bit(h, blk);
+    val = gt_head(w, h, v, blk, bo);
+    job_len = 0;
+    if (i > kCoopMin) {
+        job_bit = bo + h.f1;
+        job_len = i;
+        job_w = h.f2;
+    } else if (i) {
+        val += sum_fields(w, bo + h.f1, i, h.f2) + i;
+    }
+    fbits = h.f1 + i * h.f2;
+}
+
+// Completes the deferred runs of gt_get_split: every lane of the wave must call it.  Runs are
+// taken one at a time, the 64 lanes reading 64 consecutive gaps per pass (coalesced), and the
+// sum (plus one per gap) is added to the owner's val.
+__device__ __forceinline__ void coop_gap_sums(const uint32_t* __restrict__ w, uint64_t job_bit,
+                                              uint32_t job_len, uint32_t job_w, uint32_t& val) {
+    const int lane = threadIdx.x & 63;
+    uint64_t jobs = __ballot(job_len != 0);
+    while (jobs) {
+        const int l = __ffsll((long long)jobs) - 1;
+        jobs &= jobs - 1;
+        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)job_bit, l);
+        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(job_bit >> 32), l);
+        const uint32_t len = __builtin_amdgcn_readlane(job_len, l);
+        const uint32_t wd = __builtin_amdgcn_readlane(job_w, l);
+        const uint64_t base = ((uint64_t)hi << 32) | lo;
+        uint32_t part = 0;
+        for (uint32_t j = lane; j < len; j += 64) part += read_bits(w, base + (uint64_t)j * wd, wd);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+        if (lane == l) val += part + len;
+    }
+}
+
 }  // namespace grass

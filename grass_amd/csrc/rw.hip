@@ -39,7 +39,6 @@ constexpr int kRwBlock = 256;
 constexpr int kRwWaves = kRwBlock / kWave;
 constexpr uint32_t kStage = 16;             // walk columns staged per flush (64 bytes a row)
 constexpr uint32_t kPitch = kStage + 1;     // LDS row pitch (odd: spreads lanes over banks)
-constexpr uint32_t kCoopMin = 48;           // GT gap runs longer than this use the whole wave
 
 __device__ __forceinline__ uint64_t bytes_of_bits(uint32_t bits) { return (bits + 7) >> 3; }
 
@@ -97,38 +96,14 @@ __device__ __forceinline__ uint64_t walk_wave(const DevGraph& g, bool active, ui
                     nv = le_get(g.w, h, v, k, COUNT ? &fb : nullptr);
                     if (COUNT) acc += bytes_of_bits(fb) + 4;
                 } else {
-                    uint32_t blk, i;
-                    gt_locate(h, k, blk, i);
-                    const uint64_t bo = gt_block_bit(h, blk);
-                    nv = gt_head(g.w, h, v, blk, bo);
-                    if (i > kCoopMin) {
-                        job_bit = bo + h.f1;
-                        job_len = i;
-                        job_w = h.f2;
-                    } else if (i) {
-                        nv += sum_fields(g.w, bo + h.f1, i, h.f2) + i;
-                    }
-                    if (COUNT) acc += bytes_of_bits(h.f1 + i * h.f2) + 4;
+                    uint32_t fb;
+                    gt_get_split(g.w, h, v, k, nv, job_bit, job_len, job_w, fb);
+                    if (COUNT) acc += bytes_of_bits(fb) + 4;
                 }
             }
         }
-        if (FMT == kGT) {
-            // long gap runs: the wave sums each lane's run in turn, 64 gaps per pass
-            uint64_t jobs = __ballot(job_len != 0);
-            while (jobs) {
-                const int l = __ffsll((long long)jobs) - 1;
-                jobs &= jobs - 1;
-                const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)job_bit, l);
-                const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(job_bit >> 32), l);
-                const uint32_t len = __builtin_amdgcn_readlane(job_len, l);
-                const uint32_t wd = __builtin_amdgcn_readlane(job_w, l);
-                const uint64_t base = ((uint64_t)hi << 32) | lo;
-                uint32_t part = 0;
-                for (uint32_t j = lane; j < len; j += kWave) part += read_bits(g.w, base + (uint64_t)j * wd, wd);
-                part = wave_sum(part);
-                if (lane == l) nv += part + len;
-            }
-        }
+        // long GT gap runs: the wave sums each lane's run in turn, 64 gaps per pass
+        if (FMT == kGT) coop_gap_sums(g.w, job_bit, job_len, job_w, nv);
         if (live) {
             if (nv >= g.n) {
                 atomicCAS(err, kErrNone, kErrBadNeighbor);
