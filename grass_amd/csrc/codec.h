@@ -30,6 +30,7 @@ struct DevGraph {
     const uint64_t* __restrict__ start;  // bit start of record v, n + 1 entries
     const uint32_t* __restrict__ deg;    // degree of v, decoded and validated at load
     const uint4* __restrict__ hdr;       // decoded header of v (see pack_hdr), built at load
+    const uint32_t* __restrict__ cp;     // GT block checkpoints (see kGtCheck), built at load
     uint32_t n;
     int fmt;
 };
@@ -41,7 +42,14 @@ struct Hdr {
     uint32_t f1, f2, fp;
     uint32_t C, q, R;  // LE: chunks, q = floor((d-1)/C); GT: blocks, q = floor(d/C); R remainder
     uint32_t hbits;    // header length in bits
+    uint64_t cpb;      // GT: index of this vertex's first checkpoint
 };
+
+// GT checkpoints: for every block, the absolute id at in-block positions 32, 64, ... (K =
+// floor(q / 32) slots per block, K * C per vertex).  A GT access at in-block position i then
+// reads one checkpoint and at most 31 gaps instead of i gaps (the reference's O(sqrt d) walk,
+// GT/src/bcg.cuh:154-168).  Only vertices with q >= 32 (d >= 1024) have any.
+constexpr uint32_t kGtCheck = 32;
 
 // Decoded-header cache: one 16-byte entry per vertex, so that a sampling step reads one aligned
 // dwordx4 instead of walking four dependent gamma codes through the record.
@@ -85,9 +93,19 @@ __device__ __forceinline__ void unpack_hdr(uint4 e, Hdr& h) {
     h.R = D - h.C * h.q;
 }
 
+// LE entries are 16 bytes; GT entries are 32 bytes (the second half holds the checkpoint base),
+// 32-byte aligned so that both halves share one memory sector.
 template <int FMT>
 __device__ __forceinline__ void load_hdr(const DevGraph& g, uint32_t v, Hdr& h) {
-    unpack_hdr<FMT>(g.hdr[v], h);
+    if (FMT == kGT) {
+        const uint4 e0 = g.hdr[2 * (uint64_t)v];
+        const uint4 e1 = g.hdr[2 * (uint64_t)v + 1];
+        unpack_hdr<FMT>(e0, h);
+        h.cpb = ((uint64_t)e1.y << 32) | e1.x;
+    } else {
+        unpack_hdr<FMT>(g.hdr[v], h);
+        h.cpb = 0;
+    }
 }
 
 __host__ __device__ __forceinline__ uint32_t isqrt_u32(uint32_t x) {
@@ -307,57 +325,38 @@ __device__ __forceinline__ uint32_t gt_get(const uint32_t* __restrict__ w, const
     return val;
 }
 
-template <int FMT>
-__device__ __forceinline__ uint32_t get_neighbor(const uint32_t* __restrict__ w, const Hdr& h,
-                                                 uint32_t v, uint32_t k) {
-    return FMT == kLE ? le_get(w, h, v, k) : gt_get(w, h, v, k);
-}
 
-// GT gap runs longer than this are summed by the whole wave instead of by their lane.
-constexpr uint32_t kCoopMin = 48;
-
-// Split GT random access: the head of k's block (into `val`) and, when k sits i > 0 places into
-// the block, either the gap sum done here (i <= kCoopMin) or a deferred run (job_*) for
-// coop_gap_sums.  fbits receives the payload bits of the access.
-__device__ __forceinline__ void gt_get_split(const uint32_t* __restrict__ w, const Hdr& h, uint32_t v,
-                                             uint32_t k, uint32_t& val, uint64_t& job_bit,
-                                             uint32_t& job_len, uint32_t& job_w, uint32_t& fbits) {
+// GT random access through the block checkpoints: neighbour k of v is the checkpoint at or
+// before its in-block position (or the block head) plus at most kGtCheck - 1 gaps.  fbits
+// receives the bits read: 32 for a checkpoint (or head_bit for a head) plus the gap fields.
+__device__ __forceinline__ uint32_t gt_get_cp(const DevGraph& g, const Hdr& h, uint32_t v, uint32_t k,
+                                              uint32_t* fbits = nullptr) {
     uint32_t blk, i;
     gt_locate(h, k, blk, i);
     const uint64_t bo = gt_block_bit(h, blk);
-    val = gt_head(w, h, v, blk, bo);
-    job_len = 0;
-    if (i > kCoopMin) {
-        job_bit = bo + h.f1;
-        job_len = i;
-        job_w = h.f2;
-    } else if (i) {
-        val += sum_fields(w, bo + h.f1, i, h.f2) + i;
+    const uint32_t j = i / kGtCheck;  // 0 when the vertex has no checkpoints (q < kGtCheck)
+    uint32_t val, rem;
+    uint64_t gaps;
+    if (j) {
+        const uint32_t K = h.q / kGtCheck;
+        val = g.cp[h.cpb + (uint64_t)blk * K + (j - 1)];
+        rem = i - j * kGtCheck;
+        gaps = bo + h.f1 + (uint64_t)(j * kGtCheck) * h.f2;
+    } else {
+        val = gt_head(g.w, h, v, blk, bo);
+        rem = i;
+        gaps = bo + h.f1;
     }
-    fbits = h.f1 + i * h.f2;
+    if (rem) val += sum_fields(g.w, gaps, rem, h.f2) + rem;
+    if (fbits) *fbits = (j ? 32u : h.f1) + rem * h.f2;
+    return val;
 }
 
-// Completes the deferred runs of gt_get_split: every lane of the wave must call it.  Runs are
-// taken one at a time, the 64 lanes reading 64 consecutive gaps per pass (coalesced), and the
-// sum (plus one per gap) is added to the owner's val.
-__device__ __forceinline__ void coop_gap_sums(const uint32_t* __restrict__ w, uint64_t job_bit,
-                                              uint32_t job_len, uint32_t job_w, uint32_t& val) {
-    const int lane = threadIdx.x & 63;
-    uint64_t jobs = __ballot(job_len != 0);
-    while (jobs) {
-        const int l = __ffsll((long long)jobs) - 1;
-        jobs &= jobs - 1;
-        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)job_bit, l);
-        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(job_bit >> 32), l);
-        const uint32_t len = __builtin_amdgcn_readlane(job_len, l);
-        const uint32_t wd = __builtin_amdgcn_readlane(job_w, l);
-        const uint64_t base = ((uint64_t)hi << 32) | lo;
-        uint32_t part = 0;
-        for (uint32_t j = lane; j < len; j += 64) part += read_bits(w, base + (uint64_t)j * wd, wd);
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-        if (lane == l) val += part + len;
-    }
+// Neighbour k (< d) of vertex v with header h.
+template <int FMT>
+__device__ __forceinline__ uint32_t get_neighbor(const DevGraph& g, const Hdr& h, uint32_t v, uint32_t k,
+                                                 uint32_t* fbits = nullptr) {
+    return FMT == kLE ? le_get(g.w, h, v, k, fbits) : gt_get_cp(g, h, v, k, fbits);
 }
 
 }  // namespace grass

@@ -271,17 +271,10 @@ __global__ __launch_bounds__(kKhBlock) void sample_kernel(DevGraph g, const uint
                 if (jj < (int)c && xj < x) ++rank;
             }
         }
-        uint32_t nb = 0, job_len = 0, job_w = 0, fb;
-        uint64_t job_bit = 0;
         if (gl < (int)c) {
             Hdr hd;
             load_hdr<FMT>(g, v, hd);
-            if (FMT == kLE) nb = le_get(g.w, hd, v, x);
-            else gt_get_split(g.w, hd, v, x, nb, job_bit, job_len, job_w, fb);
-        }
-        // positions deep inside long GT blocks: their gap runs are summed by the whole wave
-        if (FMT == kGT) coop_gap_sums(g.w, job_bit, job_len, job_w, nb);
-        if (gl < (int)c) {
+            uint32_t nb = get_neighbor<FMT>(g, hd, v, x);
             if (nb >= g.n) {  // never let a bad id reach the next frontier
                 atomicCAS(err, kErrNone, kErrBadNeighbor);
                 nb = kInvalid;

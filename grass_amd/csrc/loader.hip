@@ -90,7 +90,12 @@ __global__ __launch_bounds__(kValBlock) void validate_kernel(const uint32_t* __r
         }
         h.hbits = (uint32_t)(h.pay - b0);
         deg[v] = h.d;
-        hdr[v] = pack_hdr(h, FMT);
+        if (FMT == kGT) {
+            hdr[2 * (uint64_t)v] = pack_hdr(h, FMT);
+            hdr[2 * (uint64_t)v + 1] = make_uint4(0u, 0u, 0u, 0u);  // checkpoint base, set later
+        } else {
+            hdr[v] = pack_hdr(h, FMT);
+        }
         sum += h.d;
         mx = h.d > mx ? h.d : mx;
     }
@@ -101,6 +106,67 @@ __global__ __launch_bounds__(kValBlock) void validate_kernel(const uint32_t* __r
         atomicAdd(&stats[0], sum);
         atomicMax(&stats[1], mx);
         if (bad) atomicAdd(&stats[2], bad);
+    }
+}
+
+// ------------------------------------------------------------------ GT block checkpoints
+
+// Checkpoint slots of a GT vertex: K = floor(q / kGtCheck) per block, C blocks.
+struct CpCountGen {
+    const uint4* hdr;
+    __device__ __forceinline__ uint64_t operator()(uint64_t v) const {
+        const uint4 e = hdr[2 * v];
+        const uint32_t d = e.z, C = e.w & 0xFFFFu;
+        if (d == 0 || C == 0) return 0;
+        const uint32_t q = d / C;
+        return (uint64_t)C * (q / kGtCheck);
+    }
+};
+
+__global__ void cp_base_kernel(uint4* __restrict__ hdr, const uint64_t* __restrict__ base, uint32_t n) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t b = base[v];
+        hdr[2 * v + 1] = make_uint4((uint32_t)b, (uint32_t)(b >> 32), 0u, 0u);
+    }
+}
+
+// One wave per vertex with checkpoints: the wave decodes the whole list 64 elements at a time
+// (block heads absolute, bodies as gaps, a segmented prefix sum with the running id carried
+// between passes) and stores the id at every in-block position that is a multiple of kGtCheck.
+__global__ __launch_bounds__(256) void gt_checkpoint_kernel(DevGraph g, uint32_t* __restrict__ cp) {
+    const int lane = lane_id();
+    const uint64_t nw = (uint64_t)gridDim.x * 256 / kWave;
+    for (uint64_t v = ((uint64_t)blockIdx.x * 256 + threadIdx.x) / kWave; v < g.n; v += nw) {
+        Hdr h;
+        load_hdr<kGT>(g, (uint32_t)v, h);  // same vertex in every lane: uniform branches
+        if (h.d == 0 || h.q < kGtCheck) continue;
+        const uint32_t K = h.q / kGtCheck;
+        uint32_t carry = 0;
+        for (uint32_t base = 0; base < h.d; base += kWave) {
+            const uint32_t k = base + lane;
+            const bool act = k < h.d;
+            uint32_t blk, i;
+            gt_locate(h, act ? k : h.d - 1, blk, i);
+            const uint64_t bo = gt_block_bit(h, blk);
+            uint32_t x;
+            int head;
+            if (i == 0) {
+                x = gt_head(g.w, h, (uint32_t)v, blk, bo);
+                head = 1;
+            } else {
+                x = read_bits(g.w, bo + h.f1 + (uint64_t)(i - 1) * h.f2, h.f2) + 1;
+                head = 0;
+            }
+            if (lane == 0 && !head) {
+                x += carry;
+                head = 1;
+            }
+            x = wave_segmented_sum(x, head);
+            carry = __shfl(x, kWave - 1, kWave);
+            if (act && i >= kGtCheck && i % kGtCheck == 0)
+                cp[h.cpb + (uint64_t)blk * K + i / kGtCheck - 1] = x;
+        }
     }
 }
 
@@ -170,6 +236,8 @@ uint64_t host_field(const uint8_t* tailb, uint64_t tlen, uint64_t file_bytes, ui
 
 }  // namespace
 
+int build_checkpoints(grass_graph* g, uint64_t* count);
+
 int load_graph(grass_graph* g, ByteSource& gsrc, ByteSource& osrc) {
     const uint64_t gbytes = gsrc.size(), obytes = osrc.size();
     if (obytes < 2 || gbytes < 1) return GRASS_EFORMAT;
@@ -219,7 +287,8 @@ int load_graph(grass_graph* g, ByteSource& gsrc, ByteSource& osrc) {
 
     if (hipMalloc(&g->d_start, (n + 1) * 8) != hipSuccess) return fail(GRASS_ENOMEM);
     if (hipMalloc(&g->d_deg, n * 4) != hipSuccess) return fail(GRASS_ENOMEM);
-    if (hipMalloc(&g->d_hdr, n * 16) != hipSuccess) return fail(GRASS_ENOMEM);
+    const uint64_t hdr_bytes = n * 16 * (g->fmt == kGT ? 2 : 1);
+    if (hipMalloc(&g->d_hdr, hdr_bytes) != hipSuccess) return fail(GRASS_ENOMEM);
     const uint64_t nb = scan_blocks(n);
     uint64_t* d_tmp = nullptr;  // partials[nb], total, stats[4]
     if (hipMalloc(&d_tmp, (nb + 8) * 8) != hipSuccess) return fail(GRASS_ENOMEM);
@@ -254,7 +323,41 @@ int load_graph(grass_graph* g, ByteSource& gsrc, ByteSource& osrc) {
     g->total_bits = total;
     g->m = stats[0];
     g->max_degree = stats[1];
-    g->device_bytes = gwords * 4 + (n + 1) * 8 + n * 4 + n * 16;
+    uint64_t ncp = 0;
+    if (g->fmt == kGT) {
+        int rc2 = build_checkpoints(g, &ncp);
+        if (rc2) return rc2;
+    }
+    g->device_bytes = gwords * 4 + (n + 1) * 8 + n * 4 + hdr_bytes + ncp * 4;
+    return GRASS_OK;
+}
+
+int build_checkpoints(grass_graph* g, uint64_t* count) {
+    hipStream_t s = g->stream;
+    const uint64_t n = g->n, nb = scan_blocks(n);
+    uint64_t* d_base = nullptr;  // n + 1 bases, partials, total
+    if (hipMalloc(&d_base, (n + 1 + nb + 2) * 8) != hipSuccess) return GRASS_ENOMEM;
+    hipError_t e = exclusive_scan(CpCountGen{g->d_hdr}, Count{n, nullptr}, n, d_base, d_base + n + 1,
+                                  d_base + n + 1 + nb, s);
+    const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 16384);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(cp_base_kernel, dim3(grid), dim3(256), 0, s, g->d_hdr, (const uint64_t*)d_base,
+                           (uint32_t)n);
+        e = hipGetLastError();
+    }
+    uint64_t total = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, d_base + n, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d_base);
+    if (e != hipSuccess) return status_of(e);
+    if (hipMalloc(&g->d_cp, std::max<uint64_t>(total, 1) * 4) != hipSuccess) return GRASS_ENOMEM;
+    if (total) {
+        hipLaunchKernelGGL(gt_checkpoint_kernel, dim3(4096), dim3(256), 0, s, g->dev(), g->d_cp);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return status_of(e);
+    }
+    *count = total;
     return GRASS_OK;
 }
 

@@ -70,7 +70,8 @@ struct grass_graph {
     uint32_t* d_words = nullptr;
     uint64_t* d_start = nullptr;
     uint32_t* d_deg = nullptr;
-    uint4* d_hdr = nullptr;  // decoded-header cache, 16 B per vertex
+    uint4* d_hdr = nullptr;  // decoded-header cache, 16 B (LE) or 32 B (GT) per vertex
+    uint32_t* d_cp = nullptr;  // GT block checkpoints
     uint32_t* d_err = nullptr;  // [0] first error kind, [1] detail
     hipStream_t stream = nullptr;
     std::mutex mu;
@@ -85,6 +86,7 @@ struct grass_graph {
         d.start = d_start;
         d.deg = d_deg;
         d.hdr = d_hdr;
+        d.cp = d_cp;
         d.n = (uint32_t)n;
         d.fmt = fmt;
         return d;

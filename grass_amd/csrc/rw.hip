@@ -8,12 +8,10 @@
 // (DeepWalkApp::next, LE/src/apps/randomwalk/randomWalks.cu:62-79, with cuRAND replaced).
 //
 // The 64 walkers of a wave belong to one request and so have one length: the step loop is
-// wave-uniform, which allows
-//   * staged output: each lane puts its walk values in an LDS tile of kStage columns and the
-//     wave writes whole 64-byte row segments (the walk matrix is row-major, so one column
-//     written straight from 64 lanes would touch 64 lines);
-//   * GT gap runs: a neighbour deep inside a long GT block needs the sum of i gaps; runs longer
-//     than kCoopMin are summed by the whole wave, 64 gaps per pass, one run at a time.
+// wave-uniform, which allows staged output: each lane puts its walk values in an LDS tile of
+// kStage columns and the wave writes whole 64-byte row segments (the walk matrix is row-major,
+// so one column written straight from 64 lanes would touch 64 lines).  GT accesses go through
+// the block checkpoints built at load, so a step reads at most 31 gaps.
 //
 // plain:   one launch, one request (grass_random_walk);
 // batched: many requests of different walker counts and lengths in one persistent launch;
@@ -80,8 +78,6 @@ __device__ __forceinline__ uint64_t walk_wave(const DevGraph& g, bool active, ui
     }
     for (uint32_t s = 0; s < L; ++s) {
         uint32_t nv = 0;
-        uint64_t job_bit = 0;
-        uint32_t job_len = 0, job_w = 0;
         if (live) {
             Hdr h;
             load_hdr<FMT>(g, v, h);
@@ -91,19 +87,11 @@ __device__ __forceinline__ uint64_t walk_wave(const DevGraph& g, bool active, ui
             } else {
                 U4 r = philox4x32_10(inst, w, s, 0, seed);
                 const uint32_t k = (uint32_t)uniform_index(r.x, r.y, h.d);
-                if (FMT == kLE) {
-                    uint32_t fb = 0;
-                    nv = le_get(g.w, h, v, k, COUNT ? &fb : nullptr);
-                    if (COUNT) acc += bytes_of_bits(fb) + 4;
-                } else {
-                    uint32_t fb;
-                    gt_get_split(g.w, h, v, k, nv, job_bit, job_len, job_w, fb);
-                    if (COUNT) acc += bytes_of_bits(fb) + 4;
-                }
+                uint32_t fb = 0;
+                nv = get_neighbor<FMT>(g, h, v, k, COUNT ? &fb : nullptr);
+                if (COUNT) acc += bytes_of_bits(fb) + 4;
             }
         }
-        // long GT gap runs: the wave sums each lane's run in turn, 64 gaps per pass
-        if (FMT == kGT) coop_gap_sums(g.w, job_bit, job_len, job_w, nv);
         if (live) {
             if (nv >= g.n) {
                 atomicCAS(err, kErrNone, kErrBadNeighbor);
