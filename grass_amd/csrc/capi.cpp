@@ -72,8 +72,8 @@ void free_graph(grass_graph* g) {
     if (g->d_hdr) (void)hipFree(g->d_hdr);
     if (g->d_cp) (void)hipFree(g->d_cp);
     if (g->d_err) (void)hipFree(g->d_err);
-    g->scratch.release();
-    g->ht.release();
+    g->ws.release();
+    g->ws_bytes.release();
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
 }

@@ -100,9 +100,9 @@ int decode_csr(grass_graph* g, uint64_t* rowptr, uint32_t* col, hipStream_t s) {
     const uint64_t nb = scan_blocks(n);
     {
         std::lock_guard<std::mutex> lock(g->mu);
-        hipError_t e = g->scratch.reserve((nb + 4) * 8);
+        hipError_t e = g->ws.scratch.reserve((nb + 4) * 8);
         if (e != hipSuccess) return GRASS_ENOMEM;
-        uint64_t* part = (uint64_t*)g->scratch.ptr;
+        uint64_t* part = (uint64_t*)g->ws.scratch.ptr;
         e = exclusive_scan(DegGen{g->d_deg}, Count{n, nullptr}, n, rowptr, part, part + nb, s);
         if (e != hipSuccess) return status_of(e);
         // the scratch is reused by the next call on any stream: finish the scan first

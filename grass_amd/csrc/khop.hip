@@ -341,34 +341,46 @@ static void launch_sample(grass_graph* g, bool wide, unsigned grid, hipStream_t 
                            ic, reqs, h, seed, eoff, hb, osrc, odst, cap, g->d_err);
 }
 
-static hipError_t ensure_table(grass_graph* g, uint64_t entries, hipStream_t s, HashTab& t) {
+static hipError_t ensure_table(Workspace& ws, uint64_t entries, HashTab& t) {
     uint64_t slots = 1024;
     while (slots < 2 * entries + 2) slots <<= 1;
-    if (slots > g->ht_slots) {
-        hipError_t e = g->ht.reserve(slots * 16);
+    if (slots > ws.ht_slots) {
+        hipError_t e = ws.ht.reserve(slots * 16);
         if (e != hipSuccess) return e;
-        g->ht_slots = slots;
-        g->ht_gen = 0xFFFF;  // force a clear
+        ws.ht_slots = slots;
+        ws.ht_gen = 0xFFFF;  // force a clear
     }
-    t.keys = (unsigned long long*)g->ht.ptr;
-    t.vals = t.keys + g->ht_slots;
-    t.mask = g->ht_slots - 1;
+    t.keys = (unsigned long long*)ws.ht.ptr;
+    t.vals = t.keys + ws.ht_slots;
+    t.mask = ws.ht_slots - 1;
     uint32_t lg = 0;
-    while ((1ull << lg) < g->ht_slots) ++lg;
+    while ((1ull << lg) < ws.ht_slots) ++lg;
     t.shift = 64 - lg;
     return hipSuccess;
 }
 
-static uint32_t next_gen(grass_graph* g, const HashTab& t, hipStream_t s) {
-    if (g->ht_gen >= 0xFFFF) {
+static uint32_t next_gen(Workspace& ws, const HashTab& t, hipStream_t s) {
+    if (ws.ht_gen >= 0xFFFF) {
         hipLaunchKernelGGL(ht_clear_kernel, dim3(4096), dim3(256), 0, s, t);
-        g->ht_gen = 0;
+        ws.ht_gen = 0;
     }
-    return ++g->ht_gen;
+    return ++ws.ht_gen;
 }
 
 int khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uint64_t seed,
                grass_khop_output* out, hipStream_t s) {
+    std::lock_guard<std::mutex> lock(g->mu);
+    int rc = khop_enqueue(g, reqs, R, seed, out, s, g->ws, nullptr);
+    if (rc || out->max_hops == 0) return rc;
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return status_of(e);
+    out->total = *g->ws.result_slot();
+    return out->total > out->capacity ? GRASS_ECAPACITY : GRASS_OK;
+}
+
+int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uint64_t seed,
+                 grass_khop_output* out, hipStream_t s, Workspace& ws, uint64_t* total_host) {
+    if (total_host) *total_host = 0;
     out->total = 0;
     out->max_hops = 0;
     if (R == 0) return GRASS_OK;
@@ -382,8 +394,11 @@ int khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uint6
     if (!out->offsets_dev || (out->capacity && (!out->src_dev || !out->dst_dev))) return GRASS_EINVAL;
     if (Dbound >= 0xFFFFFFFFull) return GRASS_EINVAL;  // positions are 32-bit in the dedup table
 
-    std::vector<KhReq> hreq(R);
-    std::vector<uint64_t> sbeg(R + 1, 0);
+    // descriptors staged in the workspace's pinned memory (the copy below is truly async)
+    if (ws.reserve_stage(R * sizeof(KhReq) + (R + 1) * 8 + 64) != hipSuccess) return GRASS_ENOMEM;
+    KhReq* hreq = (KhReq*)ws.stage();
+    uint64_t* sbeg = (uint64_t*)(hreq + R);
+    sbeg[0] = 0;
     for (uint32_t r = 0; r < R; ++r) {
         KhReq q{};
         q.seeds = reqs[r].seeds_dev;
@@ -398,14 +413,13 @@ int khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uint6
     const uint64_t Imax = std::max<uint64_t>(Ibound, 1), Dmax = std::max<uint64_t>(Dbound, 1);
     const uint64_t nbI = scan_blocks(Imax), nbD = scan_blocks(Dmax);
 
-    std::lock_guard<std::mutex> lock(g->mu);
     size_t need = Carve::need(R * sizeof(KhReq)) + Carve::need((R + 1) * 8) * 4 +
                   Carve::need(S * 4 + 4) + Carve::need(Imax * 4) * 4 + Carve::need((Imax + 1) * 8) +
                   Carve::need(Dmax * 4) + Carve::need((Dmax + 1) * 8) +
                   Carve::need(std::max(nbI, nbD) * 8) + Carve::need(64 * 8);
-    hipError_t e = g->scratch.reserve(need);
+    hipError_t e = ws.scratch.reserve(need);
     if (e != hipSuccess) return GRASS_ENOMEM;
-    Carve cv(g->scratch.ptr);
+    Carve cv(ws.scratch.ptr);
     KhReq* d_req = cv.take<KhReq>(R);
     uint64_t* d_sbeg = cv.take<uint64_t>(R + 1);
     uint64_t* d_foff[2] = {cv.take<uint64_t>(R + 1), cv.take<uint64_t>(R + 1)};
@@ -425,11 +439,11 @@ int khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uint6
     uint64_t* d_hb = d_sc + 8;
 
     HashTab ht;
-    e = ensure_table(g, Dmax, s, ht);
+    e = ensure_table(ws, Dmax, ht);
     if (e != hipSuccess) return GRASS_ENOMEM;
 
-    e = hipMemcpyAsync(d_req, hreq.data(), R * sizeof(KhReq), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_sbeg, sbeg.data(), (R + 1) * 8, hipMemcpyHostToDevice, s);
+    e = hipMemcpyAsync(d_req, hreq, R * sizeof(KhReq), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_sbeg, sbeg, (R + 1) * 8, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return status_of(e);
     hipLaunchKernelGGL(fill_kernel<uint64_t>, dim3(1), dim3(64), 0, s, d_sc, (uint64_t)64, (uint64_t)0);
 
@@ -440,7 +454,7 @@ int khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uint6
                               d_seedv, g->d_err);
     {
         DedupIn in{d_seedv, nullptr, S};
-        uint32_t gen = next_gen(g, ht, s);
+        uint32_t gen = next_gen(ws, ht, s);
         hipLaunchKernelGGL(dedup_insert_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, Count{S, nullptr},
                            d_sbeg, R, ht, gen, g->d_err);
         hipLaunchKernelGGL(dedup_mark_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, Count{S, nullptr},
@@ -474,7 +488,7 @@ int khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uint6
         if (h + 1 == H) break;
         DedupIn in{out->dst_dev, d_hb + h, out->capacity};
         Count E{0, d_etot};
-        uint32_t gen = next_gen(g, ht, s);
+        uint32_t gen = next_gen(ws, ht, s);
         hipLaunchKernelGGL(dedup_insert_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, E, d_eoff_r, R, ht, gen,
                            g->d_err);
         hipLaunchKernelGGL(dedup_mark_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, E, d_eoff_r, R, ht, gen,
@@ -487,12 +501,9 @@ int khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uint6
                            d_foff[nxt], d_icount[nxt]);
     }
     e = hipGetLastError();
-    uint64_t total = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&total, d_hb + H, 8, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return status_of(e);
-    out->total = total;
-    return total > out->capacity ? GRASS_ECAPACITY : GRASS_OK;
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(total_host ? total_host : ws.result_slot(), d_hb + H, 8, hipMemcpyDeviceToHost, s);
+    return status_of(e);
 }
 
 }  // namespace grass

@@ -36,6 +36,53 @@ struct Scratch {
     }
 };
 
+// Pinned host memory that grows on demand (descriptor staging, results read back).
+struct PinnedBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (ptr) {
+            (void)hipHostFree(ptr);
+            ptr = nullptr;
+            bytes = 0;
+        }
+        size_t cap = need + need / 2 + 4096;
+        hipError_t e = hipHostMalloc(&ptr, cap, hipHostMallocDefault);
+        if (e == hipSuccess) bytes = cap;
+        return e;
+    }
+    void release() {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+// Working memory of one in-flight batch: device scratch for descriptors, frontiers and scans,
+// a private k-hop dedup table (generation-tagged, so it is reused across batches without
+// clearing) and pinned host memory whose first kResultBytes receive the batch's results (the
+// k-hop edge total) and whose remainder stages descriptors.  Batches that may be in flight
+// together use different workspaces; a workspace is reused only after its batch completed.
+struct Workspace {
+    static constexpr size_t kResultBytes = 64;
+    Scratch scratch;
+    Scratch ht;
+    uint64_t ht_slots = 0;
+    uint32_t ht_gen = 0xFFFF;
+    PinnedBuf host;
+    uint64_t* result_slot() { return (uint64_t*)host.ptr; }
+    void* stage() { return (char*)host.ptr + kResultBytes; }
+    hipError_t reserve_stage(size_t bytes) { return host.reserve(kResultBytes + bytes); }
+    void release() {
+        scratch.release();
+        ht.release();
+        host.release();
+        ht_slots = 0;
+        ht_gen = 0xFFFF;
+    }
+};
+
 // Bump allocator over a Scratch region (256-byte aligned slices).
 struct Carve {
     char* base;
@@ -75,10 +122,8 @@ struct grass_graph {
     uint32_t* d_err = nullptr;  // [0] first error kind, [1] detail
     hipStream_t stream = nullptr;
     std::mutex mu;
-    grass::Scratch scratch;  // per-call working memory (decode, samplers)
-    grass::Scratch ht;       // k-hop dedup hash table (generation-tagged, reused across calls)
-    uint64_t ht_slots = 0;
-    uint32_t ht_gen = 0xFFFF;
+    grass::Workspace ws;     // working memory of the synchronous entry points (under mu)
+    grass::Scratch ws_bytes; // device counter of the byte-counting entry points (under mu)
 
     grass::DevGraph dev() const {
         grass::DevGraph d;
@@ -127,11 +172,19 @@ int random_walk(grass_graph* g, const uint32_t* starts, uint64_t walkers, uint32
                 uint64_t* bytes);
 int random_walk_batch(grass_graph* g, const grass_rw_request* reqs, uint32_t count, uint64_t seed,
                       hipStream_t stream, uint64_t* bytes);
+// Enqueues a batch on `stream` with `ws` and returns without waiting; `ws` must not be reused
+// before the stream has passed the batch.
+int rw_batch_enqueue(grass_graph* g, const grass_rw_request* reqs, uint32_t count, uint64_t seed,
+                     hipStream_t stream, Workspace& ws, unsigned long long* bytes_dev);
 
 // khop.hip
 int khop_bound(const grass_graph* g, const grass_khop_request* reqs, uint32_t count,
                uint64_t* edges, uint64_t* items, uint64_t* dedup, uint32_t* max_hops);
 int khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t count, uint64_t seed,
                grass_khop_output* out, hipStream_t stream);
+// As rw_batch_enqueue; the batch's edge total is copied to *total_host (pinned host memory)
+// when the stream reaches it.
+int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t count, uint64_t seed,
+                 grass_khop_output* out, hipStream_t stream, Workspace& ws, uint64_t* total_host);
 
 }  // namespace grass

@@ -207,8 +207,8 @@ int random_walk(grass_graph* g, const uint32_t* starts, uint64_t W, uint32_t L, 
         return status_of(hipGetLastError());
     }
     std::lock_guard<std::mutex> lock(g->mu);
-    if (g->scratch.reserve(Carve::need(8)) != hipSuccess) return GRASS_ENOMEM;
-    unsigned long long* d_bytes = (unsigned long long*)g->scratch.ptr;
+    if (g->ws_bytes.reserve(64) != hipSuccess) return GRASS_ENOMEM;
+    unsigned long long* d_bytes = (unsigned long long*)g->ws_bytes.ptr;
     hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, s, d_bytes, (uint64_t)1, 0ull);
     launch_rw<true>(g, grid, s, starts, W, L, seed, inst, nullptr, d_bytes);
     hipError_t e = hipGetLastError();
@@ -226,15 +226,14 @@ static void launch_rw_batch(grass_graph* g, hipStream_t s, const RwDesc* d, uint
                        R, gran, queue, seed, g->d_err, bytes);
 }
 
-int random_walk_batch(grass_graph* g, const grass_rw_request* reqs, uint32_t count, uint64_t seed,
-                      hipStream_t s, uint64_t* bytes) {
-    if (bytes) *bytes = 0;
+int rw_batch_enqueue(grass_graph* g, const grass_rw_request* reqs, uint32_t count, uint64_t seed,
+                     hipStream_t s, Workspace& ws, unsigned long long* bytes_dev) {
     if (count == 0) return GRASS_OK;
     if (!reqs) return GRASS_EINVAL;
     std::vector<uint32_t> order;
     for (uint32_t i = 0; i < count; ++i) {
         if (reqs[i].walkers == 0) continue;
-        if ((!reqs[i].out_dev && !bytes) || reqs[i].walkers > 0xFFFFFFFFull ||
+        if ((!reqs[i].out_dev && !bytes_dev) || reqs[i].walkers > 0xFFFFFFFFull ||
             reqs[i].length == 0xFFFFFFFFu)
             return GRASS_EINVAL;
         order.push_back(i);
@@ -244,35 +243,53 @@ int random_walk_batch(grass_graph* g, const grass_rw_request* reqs, uint32_t cou
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
         return reqs[a].length > reqs[b].length;
     });
-    std::vector<RwDesc> desc(order.size());
+    const uint32_t R = (uint32_t)order.size();
+    // descriptors staged in the workspace's pinned memory (the copy below is truly async)
+    if (ws.reserve_stage(R * sizeof(RwDesc)) != hipSuccess) return GRASS_ENOMEM;
+    RwDesc* desc = (RwDesc*)ws.stage();
     uint64_t gran = 0;
-    for (size_t j = 0; j < order.size(); ++j) {
+    for (uint32_t j = 0; j < R; ++j) {
         const grass_rw_request& r = reqs[order[j]];
         desc[j] = RwDesc{r.walkers, gran, r.starts_dev, r.out_dev, r.length, r.instance};
         gran += (r.walkers + kWave - 1) / kWave;
     }
-    std::lock_guard<std::mutex> lock(g->mu);
-    const size_t need = Carve::need(desc.size() * sizeof(RwDesc)) + Carve::need(64);
-    hipError_t e = g->scratch.reserve(need);
+    const size_t need = Carve::need(R * sizeof(RwDesc)) + Carve::need(64);
+    hipError_t e = ws.scratch.reserve(need);
     if (e != hipSuccess) return GRASS_ENOMEM;
-    Carve c(g->scratch.ptr);
-    RwDesc* d_desc = c.take<RwDesc>(desc.size());
-    unsigned long long* d_ctr = c.take<unsigned long long>(2);  // [0] queue head, [1] bytes
-    e = hipMemcpyAsync(d_desc, desc.data(), desc.size() * sizeof(RwDesc), hipMemcpyHostToDevice, s);
+    Carve c(ws.scratch.ptr);
+    RwDesc* d_desc = c.take<RwDesc>(R);
+    unsigned long long* d_queue = c.take<unsigned long long>(1);
+    e = hipMemcpyAsync(d_desc, desc, R * sizeof(RwDesc), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return status_of(e);
-    hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, s, d_ctr, (uint64_t)2, 0ull);
-    const uint32_t R = (uint32_t)desc.size();
+    hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, s, d_queue, (uint64_t)1, 0ull);
     if (g->fmt == kLE) {
-        if (bytes) launch_rw_batch<kLE, true>(g, s, d_desc, R, gran, d_ctr, seed, d_ctr + 1);
-        else launch_rw_batch<kLE, false>(g, s, d_desc, R, gran, d_ctr, seed, nullptr);
+        if (bytes_dev) launch_rw_batch<kLE, true>(g, s, d_desc, R, gran, d_queue, seed, bytes_dev);
+        else launch_rw_batch<kLE, false>(g, s, d_desc, R, gran, d_queue, seed, nullptr);
     } else {
-        if (bytes) launch_rw_batch<kGT, true>(g, s, d_desc, R, gran, d_ctr, seed, d_ctr + 1);
-        else launch_rw_batch<kGT, false>(g, s, d_desc, R, gran, d_ctr, seed, nullptr);
+        if (bytes_dev) launch_rw_batch<kGT, true>(g, s, d_desc, R, gran, d_queue, seed, bytes_dev);
+        else launch_rw_batch<kGT, false>(g, s, d_desc, R, gran, d_queue, seed, nullptr);
     }
-    e = hipGetLastError();
-    if (e == hipSuccess && bytes) e = hipMemcpyAsync(bytes, d_ctr + 1, 8, hipMemcpyDeviceToHost, s);
-    // the descriptors live in shared scratch: finish before another call may reuse it
+    return status_of(hipGetLastError());
+}
+
+int random_walk_batch(grass_graph* g, const grass_rw_request* reqs, uint32_t count, uint64_t seed,
+                      hipStream_t s, uint64_t* bytes) {
+    if (bytes) *bytes = 0;
+    std::lock_guard<std::mutex> lock(g->mu);
+    unsigned long long* d_bytes = nullptr;
+    if (bytes) {
+        // a counter of its own, ahead of the workspace's scratch that the batch carves
+        if (g->ws_bytes.reserve(64) != hipSuccess) return GRASS_ENOMEM;
+        d_bytes = (unsigned long long*)g->ws_bytes.ptr;
+        hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, s, d_bytes, (uint64_t)1, 0ull);
+    }
+    int rc = rw_batch_enqueue(g, reqs, count, seed, s, g->ws, d_bytes);
+    if (rc) return rc;
+    hipError_t e = hipSuccess;
+    if (bytes) e = hipMemcpyAsync(g->ws.result_slot(), d_bytes, 8, hipMemcpyDeviceToHost, s);
+    // the descriptors live in the graph's workspace: finish before another call may reuse it
     if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess && bytes) *bytes = *g->ws.result_slot();
     return status_of(e);
 }
 

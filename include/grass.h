@@ -160,6 +160,90 @@ int grass_khop_bound(const grass_graph *g, const grass_khop_request *reqs, uint3
 int grass_khop_batch(grass_graph *g, const grass_khop_request *reqs, uint32_t count, uint64_t seed,
                      grass_khop_output *out, void *stream);
 
+/* ------------------------------------------------------------------ serving
+ * No reference counterpart (the reference runs one compile-time app over every vertex,
+ * LE/src/grass.cu:1924-1992).  A session keeps the graph resident and runs a scheduler thread
+ * that does continuous batching: requests that arrive while others are on the GPU are admitted
+ * as soon as HBM for their inputs and outputs is free in the session's arena, grouped into
+ * launch groups (one persistent random-walk launch + one k-hop pipeline) and put on one of
+ * `max_inflight` streams, so groups overlap on the device.  A request's result depends only on
+ * its contents, its instance and the session seed -- never on what it was batched with. */
+
+typedef enum { GRASS_REQ_RW = 0, GRASS_REQ_KHOP = 1 } grass_request_kind;
+
+typedef struct {
+    int32_t kind;              /* grass_request_kind */
+    uint32_t instance;         /* Philox instance */
+    /* random walk */
+    uint64_t walkers;
+    uint32_t length;
+    const uint32_t *starts;    /* host, `walkers` ids; NULL = Philox-drawn starts */
+    /* k-hop */
+    const uint32_t *seeds;     /* host, `num_seeds` ids */
+    uint64_t num_seeds;
+    uint32_t hops;
+    uint32_t fanout[GRASS_KHOP_MAX_HOPS];
+} grass_request;
+
+typedef struct {
+    uint64_t memory_budget;    /* HBM arena for request inputs and outputs (0 = 4 GiB) */
+    uint32_t max_batch;        /* requests per launch group (0 = 512) */
+    uint32_t max_inflight;     /* launch groups on the device at once (0 = 3) */
+    uint64_t seed;
+} grass_session_config;
+
+typedef struct {
+    int32_t kind;
+    int32_t status;            /* GRASS_OK, GRASS_ECANCELLED, or the error of its launch */
+    const uint32_t *walks_dev; /* random walk: [walkers][length+1] */
+    uint64_t walkers;
+    uint32_t length;
+    uint32_t hops;             /* k-hop: edges of hop h are hop_src_dev[h][0..hop_edges[h]) */
+    const uint32_t *hop_src_dev[GRASS_KHOP_MAX_HOPS];
+    const uint32_t *hop_dst_dev[GRASS_KHOP_MAX_HOPS];
+    uint64_t hop_edges[GRASS_KHOP_MAX_HOPS];
+    double submit_us, launch_us, done_us; /* host clock, microseconds since session start */
+} grass_result;
+
+typedef struct {
+    uint64_t submitted, completed, rejected, cancelled, groups;
+    uint64_t sampled_edges;
+    uint64_t memory_in_use, memory_peak;
+} grass_session_counters;
+
+typedef struct grass_session grass_session;
+
+int grass_session_create(grass_graph *g, const grass_session_config *cfg, grass_session **out);
+/* Copies the request (the caller may free its arrays on return).  GRASS_ECAPACITY when the
+ * request could never fit the arena. */
+int grass_session_submit(grass_session *s, const grass_request *req, uint64_t *ticket);
+/* GRASS_OK when done (result ready), GRASS_ENOTREADY, GRASS_ECANCELLED, GRASS_EINVAL (unknown). */
+int grass_session_poll(grass_session *s, uint64_t ticket);
+/* Waits up to timeout_us (< 0: forever); GRASS_ETIMEOUT if still running. */
+int grass_session_wait(grass_session *s, uint64_t ticket, int64_t timeout_us);
+/* Device views of a finished request's output; valid until grass_session_release. */
+int grass_session_result(grass_session *s, uint64_t ticket, grass_result *out);
+/* Copies a finished request's output to host memory: a random walk as its [walkers][length+1]
+ * matrix; a k-hop request as all its sources (hop 0 first) followed by all its destinations.
+ * *bytes receives the size; GRASS_ECAPACITY (nothing copied) when it exceeds `capacity`. */
+int grass_session_fetch(grass_session *s, uint64_t ticket, void *host, uint64_t capacity,
+                        uint64_t *bytes);
+
+/* Returns the request's HBM to the arena and forgets the ticket. */
+int grass_session_release(grass_session *s, uint64_t ticket);
+/* A queued request is dropped; a running one is released when it finishes. */
+int grass_session_cancel(grass_session *s, uint64_t ticket);
+int grass_session_stats(grass_session *s, grass_session_counters *out);
+/* Waits for running groups, then frees everything. */
+int grass_session_destroy(grass_session *s);
+
+/* Open-loop trace replay for latency measurement: submits reqs[i] at arrival_us[i]
+ * (microseconds after the call), releases each result as it completes, and writes
+ * latency_us[i] = completion - arrival (-1 when rejected).  *wall_us receives the time from
+ * the first arrival to the last completion. */
+int grass_session_replay(grass_session *s, const grass_request *reqs, const double *arrival_us,
+                         uint32_t count, double *latency_us, double *wall_us);
+
 #ifdef __cplusplus
 }
 #endif
