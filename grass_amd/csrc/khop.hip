@@ -370,7 +370,7 @@ static uint32_t next_gen(Workspace& ws, const HashTab& t, hipStream_t s) {
 int khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uint64_t seed,
                grass_khop_output* out, hipStream_t s) {
     std::lock_guard<std::mutex> lock(g->mu);
-    int rc = khop_enqueue(g, reqs, R, seed, out, s, g->ws, nullptr);
+    int rc = khop_enqueue(g, reqs, R, seed, out, s, g->ws);
     if (rc || out->max_hops == 0) return rc;
     hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) return status_of(e);
@@ -379,8 +379,8 @@ int khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uint6
 }
 
 int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uint64_t seed,
-                 grass_khop_output* out, hipStream_t s, Workspace& ws, uint64_t* total_host) {
-    if (total_host) *total_host = 0;
+                 grass_khop_output* out, hipStream_t s, Workspace& ws) {
+    // the edge total goes to ws.result_slot(), taken after reserve_stage may have moved it
     out->total = 0;
     out->max_hops = 0;
     if (R == 0) return GRASS_OK;
@@ -502,7 +502,7 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
     }
     e = hipGetLastError();
     if (e == hipSuccess)
-        e = hipMemcpyAsync(total_host ? total_host : ws.result_slot(), d_hb + H, 8, hipMemcpyDeviceToHost, s);
+        e = hipMemcpyAsync(ws.result_slot(), d_hb + H, 8, hipMemcpyDeviceToHost, s);
     return status_of(e);
 }
 

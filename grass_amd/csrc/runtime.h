@@ -182,9 +182,10 @@ int khop_bound(const grass_graph* g, const grass_khop_request* reqs, uint32_t co
                uint64_t* edges, uint64_t* items, uint64_t* dedup, uint32_t* max_hops);
 int khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t count, uint64_t seed,
                grass_khop_output* out, hipStream_t stream);
-// As rw_batch_enqueue; the batch's edge total is copied to *total_host (pinned host memory)
-// when the stream reaches it.
+// As rw_batch_enqueue; the batch's edge total reaches *ws.result_slot() (pinned host memory)
+// when the stream passes the batch.
+
 int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t count, uint64_t seed,
-                 grass_khop_output* out, hipStream_t stream, Workspace& ws, uint64_t* total_host);
+                 grass_khop_output* out, hipStream_t stream, Workspace& ws);
 
 }  // namespace grass

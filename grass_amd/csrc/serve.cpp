@@ -369,7 +369,7 @@ struct grass_session {
             o.capacity = cap;
             o.offsets_dev = (uint64_t*)(base + kout);
             if (s.ws_kh.host.reserve(Workspace::kResultBytes) != hipSuccess) return GRASS_ENOMEM;
-            int rc = khop_enqueue(g, kh.data(), K, seed, &o, s.stream, s.ws_kh, s.ws_kh.result_slot());
+            int rc = khop_enqueue(g, kh.data(), K, seed, &o, s.stream, s.ws_kh);
             if (rc) return rc;
             const size_t ob = (size_t)std::max<uint32_t>(s.H, 1) * (K + 1) * 8;
             if (s.offs.reserve(ob) != hipSuccess) return GRASS_ENOMEM;
@@ -429,7 +429,9 @@ struct grass_session {
         }
         for (uint32_t x : r->data)
             if (x >= g->n) return GRASS_EINVAL;
-        r->q.starts = r->q.seeds = nullptr;
+        // from here on the request refers to its own copy of the ids
+        if (q->kind == GRASS_REQ_RW) r->q.starts = r->data.empty() ? nullptr : r->data.data();
+        else r->q.seeds = r->data.data();
         const uint64_t need = own_bytes(r->q) + (q->kind == GRASS_REQ_KHOP ? khop_out_bytes(r->q) + 4096 : 0);
         std::lock_guard<std::mutex> lk(mu);
         stats.submitted++;
