@@ -59,7 +59,8 @@ def serving_bench(g, seed, rng, seconds=1.5, burst=6000):
     """Saturation QPS from a burst replay (all requests due at once), then p50/p99 latency of an
     open-loop Poisson replay at 70 % of it."""
     from grass_amd import serve
-    with serve.Session(g, memory_budget=24 << 30, max_batch=512, max_inflight=3, seed=seed) as s:
+    with serve.Session(g, memory_budget=24 << 30, max_batch=512, max_inflight=4, seed=seed,
+                       max_group_edges=4 << 20) as s:
         warm = serving_mix(rng, 500, g.n)
         s.replay(warm, np.zeros(len(warm)))
         reqs = serving_mix(rng, burst, g.n)

@@ -1,14 +1,18 @@
 // Serving session: continuous batching of random-walk and k-hop requests on one resident graph.
 //
-// A scheduler thread owns `max_inflight` launch-group slots, each with its own stream, event,
-// workspaces and pinned staging.  Whenever a slot is free and requests are queued, it takes
-// queued requests in arrival order while their HBM fits the session's arena (memory-based
-// admission: inputs, walk matrices and the group's k-hop output), launches them as one group
-// (one persistent random-walk launch and one k-hop pipeline) and goes back to watching events.
-// Requests that arrive while groups run are therefore admitted as soon as a slot and memory
-// free up, and their work overlaps the running groups on the device.  Results stay in HBM until
-// the client releases them.  Every random choice is keyed by (seed, instance, ...), so a
-// request's output does not depend on its group.
+// A scheduler thread owns `max_inflight` launch-group slots.  Whenever a slot is free and
+// requests are queued, it takes queued requests in arrival order while their HBM fits the
+// session's arena (memory-based admission) and the group's edge bound stays under
+// max_group_edges, and launches them as one group:
+//   * all the group's inputs (walk starts, k-hop seeds) go to HBM in one copy from pinned staging;
+//   * the random walks run as one persistent launch on the slot's walk stream while the k-hop
+//     requests run as one pipeline on its k-hop stream, so the two kinds overlap;
+//   * then the scheduler goes back to watching the slots' events.
+// Requests that arrive while groups run are therefore admitted as soon as a slot and memory free
+// up, and their work overlaps the running groups on the device.  Walk matrices and k-hop edges
+// stay in HBM until the client releases them (the group's input block is freed when the group
+// completes).  Every random choice is keyed by (seed, instance, ...), so a request's output does
+// not depend on its group.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -45,8 +49,9 @@ class Arena {
     ~Arena() {
         if (base_) (void)hipFree(base_);
     }
+    static size_t round(size_t bytes) { return (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255; }
     bool alloc(size_t bytes, size_t* off) {
-        bytes = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+        bytes = round(bytes);
         for (auto it = free_.begin(); it != free_.end(); ++it) {
             if (it->second < bytes) continue;
             *off = it->first;
@@ -60,8 +65,24 @@ class Arena {
         }
         return false;
     }
+    // Whether regions of the given sizes could all be allocated now.
+    bool fits(std::initializer_list<size_t> sizes) {
+        std::vector<std::pair<size_t, size_t>> got;
+        bool ok = true;
+        for (size_t b : sizes) {
+            if (!b) continue;
+            size_t off;
+            if (!alloc(b, &off)) {
+                ok = false;
+                break;
+            }
+            got.emplace_back(off, b);
+        }
+        for (auto& p : got) release(p.first, p.second);
+        return ok;
+    }
     void release(size_t off, size_t bytes) {
-        bytes = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+        bytes = round(bytes);
         in_use -= bytes;
         auto it = free_.emplace(off, bytes).first;
         auto nx = std::next(it);
@@ -101,35 +122,40 @@ struct Req {
     St state = St::Pending;
     int status = GRASS_OK;
     bool cancelled = false, auto_release = false;
-    size_t off = 0, bytes = 0;  // own arena region (inputs + walk matrix)
+    size_t off = 0, bytes = 0;  // own arena region: the walk matrix of a random-walk request
     bool has_mem = false;
     std::shared_ptr<OutBlock> out;
-    uint32_t kidx = 0;  // index among its group's k-hop requests
+    uint32_t kidx = 0;       // index among its group's k-hop requests
+    uint64_t kbound = 0;     // k-hop edge bound
+    size_t in_pos = 0;       // offset of its ids in the group's input block (words)
     grass_result res{};
     double arrival_us = 0;
     uint64_t edges = 0;
 };
 
 struct Slot {
-    hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    hipStream_t s_rw = nullptr, s_kh = nullptr;
+    hipEvent_t in_ready = nullptr, ev_rw = nullptr, ev_kh = nullptr;
+    bool rw_pending = false, kh_pending = false;
     Workspace ws_rw, ws_kh;
-    PinnedBuf stage;   // request inputs on their way to HBM
-    PinnedBuf offs;    // k-hop offsets read back
+    PinnedBuf stage;  // the group's inputs on their way to HBM
+    PinnedBuf offs;   // k-hop offsets read back
     bool busy = false;
-    double launch_us = 0;
     std::vector<Req*> reqs;
     std::vector<Req*> kreqs;
     std::shared_ptr<OutBlock> out;
+    size_t in_off = 0, in_bytes = 0;  // input block
     uint32_t H = 0;
     char* out_base = nullptr;  // k-hop block: src[cap], dst[cap], offsets
     uint64_t cap = 0;
     int launch_status = GRASS_OK;
 };
 
-uint64_t rw_bytes(const grass_request& q) {
-    return (q.starts ? q.walkers * 4 : 0) + q.walkers * ((uint64_t)q.length + 1) * 4;
-}
+uint64_t walk_bytes(const grass_request& q) { return q.walkers * ((uint64_t)q.length + 1) * 4; }
+
+// k-hop block of a group: src[cap], dst[cap] (cap = sum of the requests' edge bounds), then the
+// hop-major offsets (at most GRASS_KHOP_MAX_HOPS x (K + 1)).
+uint64_t block_bytes(uint64_t kcap, uint32_t K) { return kcap * 8 + 8ull * GRASS_KHOP_MAX_HOPS * (K + 1); }
 
 }  // namespace
 }  // namespace grass
@@ -156,20 +182,21 @@ struct grass_session {
         return std::chrono::duration<double, std::micro>(Clock::now() - t0).count();
     }
 
-    // Bytes a request needs of its own (k-hop outputs are counted in the group block).
-    uint64_t own_bytes(const grass_request& q) const {
-        if (q.kind == GRASS_REQ_RW) return rw_bytes(q);
-        return q.num_seeds * 4;
-    }
-    uint64_t khop_out_bytes(const grass_request& q) const {
+    uint64_t khop_edge_bound(const grass_request& q) const {
         grass_khop_request k{};
         k.num_seeds = q.num_seeds;
         k.seeds_dev = (const uint32_t*)16;  // only the sizes matter to the bound
         k.hops = q.hops;
         memcpy(k.fanout, q.fanout, sizeof(k.fanout));
         uint64_t e = 0;
-        if (khop_bound(g, &k, 1, &e, nullptr, nullptr, nullptr)) return ~0ull;
-        return e * 8;
+        if (khop_bound(g, &k, 1, &e, nullptr, nullptr, nullptr)) return ~0ull >> 8;
+        return e;
+    }
+
+    void notify_memory() {
+        stats.memory_in_use = arena.in_use;
+        admissible = true;  // memory came back: queued requests may fit now
+        cv_work.notify_one();
     }
 
     void free_req_mem(Req* r) {
@@ -181,9 +208,7 @@ struct grass_session {
             if (--r->out->refs == 0) arena.release(r->out->off, r->out->bytes);
             r->out.reset();
         }
-        stats.memory_in_use = arena.in_use;
-        admissible = true;  // memory came back: queued requests may fit now
-        cv_work.notify_one();
+        notify_memory();
     }
 
     void finish(Req* r, int status) {
@@ -201,6 +226,12 @@ struct grass_session {
         if (r->cancelled) reqs.erase(r->ticket);
     }
 
+    bool slot_done(Slot& s) {
+        if (s.rw_pending && hipEventQuery(s.ev_rw) == hipErrorNotReady) return false;
+        if (s.kh_pending && hipEventQuery(s.ev_kh) == hipErrorNotReady) return false;
+        return true;
+    }
+
     void finalize(Slot& s) {
         const uint64_t* off = (const uint64_t*)s.offs.ptr;
         const uint32_t K = (uint32_t)s.kreqs.size();
@@ -209,7 +240,8 @@ struct grass_session {
                 r->res.hops = r->q.hops;
                 r->edges = 0;
                 for (uint32_t h = 0; h < r->q.hops && h < s.H; ++h) {
-                    const uint64_t a = off[(uint64_t)h * (K + 1) + r->kidx], b = off[(uint64_t)h * (K + 1) + r->kidx + 1];
+                    const uint64_t a = off[(uint64_t)h * (K + 1) + r->kidx];
+                    const uint64_t b = off[(uint64_t)h * (K + 1) + r->kidx + 1];
                     r->res.hop_src_dev[h] = (const uint32_t*)s.out_base + a;
                     r->res.hop_dst_dev[h] = (const uint32_t*)s.out_base + s.cap + a;
                     r->res.hop_edges[h] = b - a;
@@ -220,66 +252,72 @@ struct grass_session {
             }
             finish(r, s.launch_status);
         }
+        if (s.in_bytes) arena.release(s.in_off, s.in_bytes);
+        s.in_bytes = 0;
+        notify_memory();
         s.reqs.clear();
         s.kreqs.clear();
         s.out.reset();
+        s.rw_pending = s.kh_pending = false;
         s.busy = false;
-    }
-
-    // k-hop block of a group: src[cap], dst[cap] (cap = sum of the requests' edge bounds), then
-    // the hop-major offsets (at most GRASS_KHOP_MAX_HOPS x (K + 1)).
-    static uint64_t block_bytes(uint64_t kout, uint32_t K) {
-        return kout + 8ull * GRASS_KHOP_MAX_HOPS * (K + 1);
     }
 
     // Forms and launches one group on slot s; false when nothing could be admitted.
     bool launch(Slot& s) {
         std::vector<Req*> take;
-        uint64_t kout = 0;
+        uint64_t kcap = 0, in_words = 0, est = 0;
         uint32_t K = 0;
         while (!pending.empty() && take.size() < cfg.max_batch) {
             Req* r = pending.front();
-            const uint64_t own = own_bytes(r->q);
             const bool kreq = r->q.kind == GRASS_REQ_KHOP;
-            const uint64_t extra = kreq ? khop_out_bytes(r->q) : 0;
-            size_t off = 0;
-            if (!arena.alloc(own, &off)) break;
-            // the group's k-hop block is allocated after the loop: check there is room for it
-            if (kreq) {
-                size_t probe = 0;
-                const uint64_t bb = block_bytes(kout + extra, K + 1);
-                if (!arena.alloc(bb, &probe)) {
-                    arena.release(off, own);
-                    break;
-                }
-                arena.release(probe, bb);
+            const uint64_t bound = kreq ? r->kbound : r->q.walkers * r->q.length;
+            if (!take.empty() && est + bound > cfg.max_group_edges) break;  // group is full
+            const uint64_t own = kreq ? 0 : walk_bytes(r->q);
+            const uint64_t words = in_words + r->data.size();
+            const uint64_t kc = kcap + (kreq ? bound : 0);
+            const uint32_t kk = K + kreq;
+            // the request's own region, the input block and the k-hop block must fit together
+            if (!arena.fits({own, words * 4, kk ? block_bytes(kc, kk) : 0})) break;
+            if (own) {
+                size_t off = 0;
+                arena.alloc(own, &off);
+                r->off = off;
+                r->bytes = own;
+                r->has_mem = true;
             }
-            r->off = off;
-            r->bytes = own;
-            r->has_mem = true;
-            kout += extra;
-            K += kreq;
+            r->in_pos = in_words;
+            in_words = words;
+            kcap = kc;
+            K = kk;
+            est += bound;
             take.push_back(r);
             pending.pop_front();
         }
-        // allocate the k-hop block; should the requests' own regions have taken its room,
-        // hand k-hop requests back (newest first) until it fits
+        // group blocks; should fragmentation defeat the check above, hand the newest requests back
         std::shared_ptr<OutBlock> blk;
-        while (K) {
-            blk = std::make_shared<OutBlock>();
-            blk->bytes = block_bytes(kout, K);
-            if (arena.alloc(blk->bytes, &blk->off)) break;
-            blk.reset();
-            for (auto it = take.rbegin(); it != take.rend(); ++it) {
-                if ((*it)->q.kind != GRASS_REQ_KHOP) continue;
-                Req* r = *it;
-                kout -= khop_out_bytes(r->q);
-                --K;
-                free_req_mem(r);
-                pending.push_front(r);
-                take.erase(std::next(it).base());
-                break;
+        size_t in_off = 0;
+        for (;;) {
+            bool ok = true;
+            if (in_words && !arena.alloc(in_words * 4, &in_off)) ok = false;
+            if (ok && K) {
+                blk = std::make_shared<OutBlock>();
+                blk->bytes = block_bytes(kcap, K);
+                if (!arena.alloc(blk->bytes, &blk->off)) {
+                    if (in_words) arena.release(in_off, in_words * 4);
+                    blk.reset();
+                    ok = false;
+                }
             }
+            if (ok || take.empty()) break;
+            Req* r = take.back();
+            take.pop_back();
+            in_words -= r->data.size();
+            if (r->q.kind == GRASS_REQ_KHOP) {
+                kcap -= r->kbound;
+                --K;
+            }
+            free_req_mem(r);
+            pending.push_front(r);
         }
         if (take.empty()) {
             admissible = false;  // nothing fits until memory is released
@@ -287,16 +325,15 @@ struct grass_session {
         }
         s.reqs = take;
         s.kreqs.clear();
-        size_t n_in = 0;
-        for (Req* r : take) {
-            n_in += r->data.size();
+        for (Req* r : take)
             if (r->q.kind == GRASS_REQ_KHOP) {
                 r->kidx = (uint32_t)s.kreqs.size();
                 s.kreqs.push_back(r);
             }
-        }
         s.H = 0;
         for (Req* r : s.kreqs) s.H = std::max(s.H, r->q.hops);
+        s.in_off = in_off;
+        s.in_bytes = in_words * 4;
         s.out.reset();
         if (K) {
             blk->refs = (int)K;
@@ -307,74 +344,81 @@ struct grass_session {
         stats.memory_peak = arena.peak;
         stats.groups++;
         const double t = now_us();
-        s.launch_us = t;
         for (Req* r : take) {
             r->state = St::Running;
             r->res.launch_us = t;
         }
-        s.launch_status = enqueue(s, n_in, kout);
-        if (hipEventRecord(s.done, s.stream) != hipSuccess && s.launch_status == GRASS_OK)
-            s.launch_status = GRASS_EDEVICE;
+        s.launch_status = enqueue(s, kcap);
         s.busy = true;
         return true;
     }
 
-    int enqueue(Slot& s, size_t n_in, uint64_t kout) {
-        // inputs: pinned staging -> each request's region
-        if (n_in && s.stage.reserve(n_in * 4) != hipSuccess) return GRASS_ENOMEM;
-        size_t pos = 0;
+    int enqueue(Slot& s, uint64_t kcap) {
+        const uint64_t seed = cfg.seed;
+        uint32_t* in_dev = (uint32_t*)arena.at(s.in_off);
+        // 1. every input of the group in one copy (on the k-hop stream; the walk stream waits)
+        if (s.in_bytes) {
+            if (s.stage.reserve(s.in_bytes) != hipSuccess) return GRASS_ENOMEM;
+            uint32_t* st = (uint32_t*)s.stage.ptr;
+            for (Req* r : s.reqs)
+                if (!r->data.empty()) memcpy(st + r->in_pos, r->data.data(), r->data.size() * 4);
+            if (hipMemcpyAsync(in_dev, st, s.in_bytes, hipMemcpyHostToDevice, s.s_kh) != hipSuccess ||
+                hipEventRecord(s.in_ready, s.s_kh) != hipSuccess ||
+                hipStreamWaitEvent(s.s_rw, s.in_ready, 0) != hipSuccess)
+                return GRASS_EDEVICE;
+        }
         std::vector<grass_rw_request> rw;
         std::vector<grass_khop_request> kh(s.kreqs.size());
         for (Req* r : s.reqs) {
-            uint32_t* in_dev = (uint32_t*)arena.at(r->off);
-            if (!r->data.empty()) {
-                uint32_t* st = (uint32_t*)s.stage.ptr + pos;
-                memcpy(st, r->data.data(), r->data.size() * 4);
-                if (hipMemcpyAsync(in_dev, st, r->data.size() * 4, hipMemcpyHostToDevice, s.stream) != hipSuccess)
-                    return GRASS_EDEVICE;
-                pos += r->data.size();
-            }
+            const uint32_t* ids = r->data.empty() ? nullptr : in_dev + r->in_pos;
             if (r->q.kind == GRASS_REQ_RW) {
-                uint32_t* walks = in_dev + (r->q.starts ? r->q.walkers : 0);
+                uint32_t* walks = (uint32_t*)arena.at(r->off);
                 r->res.kind = GRASS_REQ_RW;
                 r->res.walks_dev = walks;
                 r->res.walkers = r->q.walkers;
                 r->res.length = r->q.length;
-                rw.push_back(grass_rw_request{r->q.walkers, r->q.length, r->q.instance,
-                                              r->q.starts ? in_dev : nullptr, walks});
+                rw.push_back(grass_rw_request{r->q.walkers, r->q.length, r->q.instance, ids, walks});
             } else {
                 r->res.kind = GRASS_REQ_KHOP;
                 grass_khop_request& k = kh[r->kidx];
-                k.seeds_dev = in_dev;
+                k.seeds_dev = ids;
                 k.num_seeds = r->q.num_seeds;
                 k.instance = r->q.instance;
                 k.hops = r->q.hops;
                 memcpy(k.fanout, r->q.fanout, sizeof(k.fanout));
             }
         }
-        const uint64_t seed = cfg.seed;
+        // 2. walks on the walk stream
         if (!rw.empty()) {
-            int rc = rw_batch_enqueue(g, rw.data(), (uint32_t)rw.size(), seed, s.stream, s.ws_rw, nullptr);
+            int rc = rw_batch_enqueue(g, rw.data(), (uint32_t)rw.size(), seed, s.s_rw, s.ws_rw, nullptr);
+            if (hipEventRecord(s.ev_rw, s.s_rw) != hipSuccess && rc == GRASS_OK) rc = GRASS_EDEVICE;
+            s.rw_pending = true;
             if (rc) return rc;
         }
+        // 3. k-hop pipeline on the k-hop stream
         if (!kh.empty()) {
             const uint32_t K = (uint32_t)kh.size();
             char* base = arena.at(s.out->off);
             s.out_base = base;
-            const uint64_t cap = kout / 8;
-            s.cap = cap;
+            s.cap = kcap;
             grass_khop_output o{};
             o.src_dev = (uint32_t*)base;
-            o.dst_dev = (uint32_t*)base + cap;
-            o.capacity = cap;
-            o.offsets_dev = (uint64_t*)(base + kout);
-            if (s.ws_kh.host.reserve(Workspace::kResultBytes) != hipSuccess) return GRASS_ENOMEM;
-            int rc = khop_enqueue(g, kh.data(), K, seed, &o, s.stream, s.ws_kh);
-            if (rc) return rc;
+            o.dst_dev = (uint32_t*)base + kcap;
+            o.capacity = kcap;
+            o.offsets_dev = (uint64_t*)(base + kcap * 8);
+            int rc = khop_enqueue(g, kh.data(), K, seed, &o, s.s_kh, s.ws_kh);
             const size_t ob = (size_t)std::max<uint32_t>(s.H, 1) * (K + 1) * 8;
-            if (s.offs.reserve(ob) != hipSuccess) return GRASS_ENOMEM;
-            if (hipMemcpyAsync(s.offs.ptr, o.offsets_dev, ob, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
-                return GRASS_EDEVICE;
+            if (rc == GRASS_OK && s.offs.reserve(ob) != hipSuccess) rc = GRASS_ENOMEM;
+            if (rc == GRASS_OK &&
+                hipMemcpyAsync(s.offs.ptr, o.offsets_dev, ob, hipMemcpyDeviceToHost, s.s_kh) != hipSuccess)
+                rc = GRASS_EDEVICE;
+            if (hipEventRecord(s.ev_kh, s.s_kh) != hipSuccess && rc == GRASS_OK) rc = GRASS_EDEVICE;
+            s.kh_pending = true;
+            if (rc) return rc;
+        } else if (s.in_bytes) {
+            // only walks: the input copy sits on the k-hop stream; wait for it too
+            if (hipEventRecord(s.ev_kh, s.s_kh) != hipSuccess) return GRASS_EDEVICE;
+            s.kh_pending = true;
         }
         return GRASS_OK;
     }
@@ -386,7 +430,7 @@ struct grass_session {
             bool progress = false, busy = false;
             for (Slot& s : slots) {
                 if (!s.busy) continue;
-                if (hipEventQuery(s.done) == hipErrorNotReady) {
+                if (!slot_done(s)) {
                     busy = true;
                     continue;
                 }
@@ -432,10 +476,16 @@ struct grass_session {
         // from here on the request refers to its own copy of the ids
         if (q->kind == GRASS_REQ_RW) r->q.starts = r->data.empty() ? nullptr : r->data.data();
         else r->q.seeds = r->data.data();
-        const uint64_t need = own_bytes(r->q) + (q->kind == GRASS_REQ_KHOP ? khop_out_bytes(r->q) + 4096 : 0);
+        uint64_t need = r->data.size() * 4;
+        if (q->kind == GRASS_REQ_RW) {
+            need += walk_bytes(r->q);
+        } else {
+            r->kbound = khop_edge_bound(r->q);
+            need += block_bytes(r->kbound, 1);
+        }
         std::lock_guard<std::mutex> lk(mu);
         stats.submitted++;
-        if (need + 8 * 8 * (cfg.max_batch + 1) > arena.size()) {
+        if (need + 4096 > arena.size()) {
             stats.rejected++;
             return GRASS_ECAPACITY;
         }
@@ -462,8 +512,10 @@ int grass_session_create(grass_graph* g, const grass_session_config* cfg, grass_
     if (cfg) s->cfg = *cfg;
     if (!s->cfg.memory_budget) s->cfg.memory_budget = 4ull << 30;
     if (!s->cfg.max_batch) s->cfg.max_batch = 512;
-    if (!s->cfg.max_inflight) s->cfg.max_inflight = 3;
+    if (!s->cfg.max_inflight) s->cfg.max_inflight = 4;
+    if (!s->cfg.max_group_edges) s->cfg.max_group_edges = 4ull << 20;
     s->cfg.max_batch = std::min<uint32_t>(s->cfg.max_batch, 1u << 16);
+    s->cfg.max_inflight = std::min<uint32_t>(s->cfg.max_inflight, 64);
     int prev = 0;
     (void)hipGetDevice(&prev);
     if (hipSetDevice(g->device) != hipSuccess) {
@@ -474,8 +526,11 @@ int grass_session_create(grass_graph* g, const grass_session_config* cfg, grass_
     s->slots.resize(s->cfg.max_inflight);
     for (Slot& sl : s->slots) {
         if (rc) break;
-        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
+        if (hipStreamCreateWithFlags(&sl.s_rw, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&sl.s_kh, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.in_ready, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.ev_rw, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.ev_kh, hipEventDisableTiming) != hipSuccess)
             rc = GRASS_EDEVICE;
     }
     (void)hipSetDevice(prev);
@@ -636,13 +691,16 @@ int grass_session_destroy(grass_session* s) {
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(s->g->device);
     for (Slot& sl : s->slots) {
-        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.s_rw) (void)hipStreamSynchronize(sl.s_rw);
+        if (sl.s_kh) (void)hipStreamSynchronize(sl.s_kh);
         sl.ws_rw.release();
         sl.ws_kh.release();
         sl.stage.release();
         sl.offs.release();
-        if (sl.done) (void)hipEventDestroy(sl.done);
-        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        for (hipEvent_t e : {sl.in_ready, sl.ev_rw, sl.ev_kh})
+            if (e) (void)hipEventDestroy(e);
+        if (sl.s_rw) (void)hipStreamDestroy(sl.s_rw);
+        if (sl.s_kh) (void)hipStreamDestroy(sl.s_kh);
     }
     (void)hipSetDevice(prev);
     delete s;
@@ -650,7 +708,7 @@ int grass_session_destroy(grass_session* s) {
 }
 
 int grass_session_replay(grass_session* s, const grass_request* reqs, const double* arrival_us,
-                         uint32_t count, double* latency_us, double* wall_us) {
+                         uint32_t count, double* latency_us, double* queue_us, double* wall_us) {
     if (!s || (count && (!reqs || !arrival_us || !latency_us))) return GRASS_EINVAL;
     std::vector<uint64_t> tickets(count, 0);
     const double base = s->now_us();
@@ -661,6 +719,7 @@ int grass_session_replay(grass_session* s, const grass_request* reqs, const doub
         if (rc != GRASS_OK) {
             tickets[i] = 0;
             latency_us[i] = -1.0;
+            if (queue_us) queue_us[i] = -1.0;
         }
     }
     double last = base;
@@ -670,11 +729,13 @@ int grass_session_replay(grass_session* s, const grass_request* reqs, const doub
         auto it = s->reqs.find(tickets[i]);
         if (it == s->reqs.end()) {
             latency_us[i] = -1.0;
+            if (queue_us) queue_us[i] = -1.0;
             continue;
         }
         Req* r = it->second.get();
         s->cv_done.wait(lk, [&] { return r->state == St::Done; });
         latency_us[i] = r->status == GRASS_OK ? r->res.done_us - r->arrival_us : -1.0;
+        if (queue_us) queue_us[i] = r->status == GRASS_OK ? r->res.launch_us - r->arrival_us : -1.0;
         last = std::max(last, r->res.done_us);
         s->free_req_mem(r);
         s->reqs.erase(it);

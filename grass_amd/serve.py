@@ -19,7 +19,8 @@ class Request(C.Structure):
 
 class Config(C.Structure):
     _fields_ = [("memory_budget", C.c_uint64), ("max_batch", C.c_uint32),
-                ("max_inflight", C.c_uint32), ("seed", C.c_uint64)]
+                ("max_inflight", C.c_uint32), ("seed", C.c_uint64),
+                ("max_group_edges", C.c_uint64)]
 
 
 class Result(C.Structure):
@@ -56,7 +57,7 @@ def lib():
             "grass_session_cancel": (I32, [P, U64]),
             "grass_session_stats": (I32, [P, P]),
             "grass_session_destroy": (I32, [P]),
-            "grass_session_replay": (I32, [P, P, P, U32, P, P]),
+            "grass_session_replay": (I32, [P, P, P, U32, P, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -88,9 +89,10 @@ def khop_request(seeds, fanout, instance):
 
 
 class Session:
-    def __init__(self, graph, memory_budget=4 << 30, max_batch=512, max_inflight=3, seed=42):
+    def __init__(self, graph, memory_budget=4 << 30, max_batch=512, max_inflight=4, seed=42,
+                 max_group_edges=4 << 20):
         self.graph = graph
-        cfg = Config(memory_budget, max_batch, max_inflight, seed)
+        cfg = Config(memory_budget, max_batch, max_inflight, seed, max_group_edges)
         h = C.c_void_p()
         _check(lib().grass_session_create(graph.handle, C.byref(cfg), C.byref(h)),
                "grass_session_create")
@@ -161,13 +163,18 @@ class Session:
         _check(lib().grass_session_stats(self._h, C.byref(c)), "stats")
         return {k: getattr(c, k) for k, _ in Counters._fields_}
 
-    def replay(self, requests, arrival_us):
-        """Open-loop replay; returns (latency_us array, wall_us)."""
+    def replay(self, requests, arrival_us, queue=False):
+        """Open-loop replay; returns (latency_us, wall_us), or (latency_us, queue_us, wall_us)
+        with queue=True (queue_us: arrival to the launch of the request's group)."""
         n = len(requests)
         arr = (Request * max(n, 1))(*requests)
         arrival = np.ascontiguousarray(arrival_us, dtype=np.float64)
         lat = np.zeros(max(n, 1), dtype=np.float64)
+        qu = np.zeros(max(n, 1), dtype=np.float64)
         wall = C.c_double()
         _check(lib().grass_session_replay(self._h, arr, arrival.ctypes.data, n, lat.ctypes.data,
-                                          C.byref(wall)), "replay")
+                                          qu.ctypes.data if queue else None, C.byref(wall)),
+               "replay")
+        if queue:
+            return lat[:n], qu[:n], wall.value
         return lat[:n], wall.value
