@@ -11,6 +11,8 @@ App templates; the names below follow that surface:
   Graph.decode_csr()                BCGVertex::get_vertex, all k (LE/src/bcg.cuh:196-261)
   Graph.random_walk(...)            DeepWalkApp (LE/src/apps/randomwalk/randomWalks.cu:62-79)
   Graph.random_walk_batch(...)      many random-walk requests, one persistent launch
+  Graph.khop_batch(...)             KHopApp (LE/src/apps/khop/khop.cu:1-69), batched
+  Graph.mdrw(...)                   MultiRWApp (LE/src/apps/multiRW/multiRW.cu:22-97), batched
 """
 import ctypes as C
 import os
@@ -58,6 +60,21 @@ class KhopOutput(C.Structure):
                 ("offsets_dev", C.c_void_p), ("total", C.c_uint64), ("max_hops", C.c_uint32)]
 
 
+class MdrwRequest(C.Structure):
+    _fields_ = [("instance", C.c_uint32), ("frontier", C.c_uint32)]
+
+
+class MdrwOutput(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("step_off_dev", C.c_void_p), ("step_src_dev", C.c_void_p),
+                ("step_dst_dev", C.c_void_p), ("vert_off_dev", C.c_void_p),
+                ("verts_dev", C.c_void_p), ("ind_off_dev", C.c_void_p),
+                ("ind_src_dev", C.c_void_p), ("ind_dst_dev", C.c_void_p),
+                ("steps", C.c_uint64), ("verts", C.c_uint64), ("induced", C.c_uint64)]
+
+
+MDRW_MAX_FRONTIER = 8192
+MDRW_MAX_TARGET = 16384
+
 _lib = None
 
 
@@ -85,6 +102,10 @@ def lib():
             "grass_random_walk_batch_bytes": (I32, [P, P, U32, U64, P, P]),
             "grass_khop_bound": (I32, [P, P, U32, P]),
             "grass_khop_batch": (I32, [P, P, U32, U64, P, P]),
+            "grass_mdrw": (I32, [P, P, U32, U32, U64, P, P]),
+            "grass_mdrw_output_get": (I32, [P, P]),
+            "grass_mdrw_fetch": (I32, [P, P, P, P, P, P, P, P, P]),
+            "grass_mdrw_free": (None, [P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -297,6 +318,33 @@ class Graph:
         return {"offsets": off.cpu().numpy().view(np.uint64)[: H * (R + 1)].reshape(H, R + 1),
                 "src": to_numpy_u32(src)[:t], "dst": to_numpy_u32(dst)[:t], "total": t,
                 "max_hops": out.max_hops}
+
+    def mdrw(self, frontier_sizes, target, seed, instances=None, stream=None):
+        """Frontier sampling, one instance per frontier size; returns host arrays
+        {"step_off", "step_src", "step_dst", "vert_off", "verts", "ind_off", "ind_src",
+        "ind_dst"} (the oracle's layout)."""
+        I = len(frontier_sizes)
+        arr = (MdrwRequest * max(I, 1))()
+        for i, m in enumerate(frontier_sizes):
+            arr[i] = MdrwRequest(int(instances[i]) if instances is not None else i, int(m))
+        res = C.c_void_p()
+        _check(lib().grass_mdrw(self._h, arr, I, target, seed, C.c_void_p(_stream_ptr(stream)),
+                                C.byref(res)), "grass_mdrw")
+        try:
+            o = MdrwOutput()
+            _check(lib().grass_mdrw_output_get(res, C.byref(o)), "grass_mdrw_output_get")
+            so, vo, io = (np.zeros(I + 1, np.uint64) for _ in range(3))
+            ss, sd = np.zeros(o.steps + 1, np.uint32), np.zeros(o.steps + 1, np.uint32)
+            vv = np.zeros(o.verts + 1, np.uint32)
+            isrc, idst = np.zeros(o.induced + 1, np.uint32), np.zeros(o.induced + 1, np.uint32)
+            ptr = (lambda a: a.ctypes.data)
+            _check(lib().grass_mdrw_fetch(res, ptr(so), ptr(ss), ptr(sd), ptr(vo), ptr(vv), ptr(io),
+                                          ptr(isrc), ptr(idst)), "grass_mdrw_fetch")
+        finally:
+            lib().grass_mdrw_free(res)
+        return {"step_off": so, "vert_off": vo, "ind_off": io, "step_src": ss[: o.steps],
+                "step_dst": sd[: o.steps], "verts": vv[: o.verts], "ind_src": isrc[: o.induced],
+                "ind_dst": idst[: o.induced]}
 
     @staticmethod
     def khop_request_edges(res, r):

@@ -1,0 +1,609 @@
+// Multi-dimensional random walk (frontier sampling) in batches of instances.
+//
+// The reference's multi-RW app (LE/src/apps/multiRW/multiRW.cu:22-45,74-97) keeps 80 roots per
+// sample, moves a uniformly chosen root per step and materialises nothing.  Here an instance
+// keeps a frontier of m vertices, moves an entry chosen proportionally to its degree (the
+// intent stated at multiRW.cu:77) and returns its traversed edges, its sorted vertex set and the
+// induced edge list of that set.
+//
+// 1. mdrw_walk_kernel: persistent, one wave per instance taken from a device queue.  The
+//    frontier ids and degrees sit in LDS.  Lane L owns the contiguous block [L*B, L*B + B) of
+//    entries (B = ceil(m / 64)) and keeps its degree sum in a register, so the degree-
+//    proportional pick is two wave scans: one over the 64 block sums to find the block, one over
+//    that block's degrees to find the entry.  A step is then one header load, one neighbour
+//    decode and one degree load, all wave-uniform.
+// 2. mdrw_verts_kernel: one workgroup per instance sorts frontier + step destinations in LDS
+//    (bitonic) and keeps the first of each run of equal ids.
+// 3. Induced edges: the sampled vertices' adjacency lists are cut into chunks of kIndChunk
+//    entries; one workgroup per instance holds the instance's vertex set in an LDS hash table,
+//    its waves decode the chunks (LE: O(1) random access per lane; GT: block heads plus gaps
+//    turned into ids with a segmented wave sum) and keep the entries found in the table.  A count
+//    pass and a scan give every chunk its output offset, so the write pass emits the edges in
+//    (vertex, position) order without any sorting.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "devutil.h"
+#include "philox.h"
+#include "runtime.h"
+#include "scan.h"
+
+namespace grass {
+
+constexpr int kIndBlock = 1024;
+constexpr int kIndWaves = kIndBlock / kWave;
+constexpr uint32_t kIndChunk = 2048;  // adjacency entries per chunk (32 wave iterations)
+constexpr int kSortBlock = 1024;
+
+struct MdDesc {
+    uint64_t step_base;  // bound layout: instance i's steps start here (target - m slots)
+    uint64_t init_base;  // initial frontier
+    uint64_t vert_base;  // bound layout of the vertex set (m + steps slots)
+    uint32_t inst, m, steps, pad;
+};
+
+// ---------------------------------------------------------------------------- 1. walks
+
+template <int FMT>
+__global__ __launch_bounds__(kWave) void mdrw_walk_kernel(DevGraph g, const MdDesc* __restrict__ desc,
+                                                          uint32_t I, unsigned long long* queue,
+                                                          uint64_t seed, uint32_t* __restrict__ init,
+                                                          uint32_t* __restrict__ ssrc,
+                                                          uint32_t* __restrict__ sdst,
+                                                          uint32_t* __restrict__ scnt,
+                                                          uint32_t mmax, uint32_t* __restrict__ err) {
+    extern __shared__ uint32_t lds[];
+    uint32_t* F = lds;         // frontier ids
+    uint32_t* D = lds + mmax;  // their degrees
+    const int lane = lane_id();
+    for (;;) {
+        unsigned long long gi = 0;
+        if (lane == 0) gi = atomicAdd(queue, 1ull);
+        gi = __shfl(gi, 0, kWave);
+        if (gi >= I) break;  // every wave exits once the queue is drained
+        const MdDesc q = desc[gi];
+        const uint32_t m = q.m;
+        for (uint32_t j = lane; j < m; j += kWave) {
+            const uint32_t v = (uint32_t)uniform(seed, q.inst, j, kTagStart, 0, g.n);
+            F[j] = v;
+            D[j] = g.deg[v];
+            init[q.init_base + j] = v;
+        }
+        __syncthreads();
+        const uint32_t B = (m + kWave - 1) / kWave;
+        uint64_t bs = 0;
+        for (uint32_t i = 0; i < B; ++i) {
+            const uint32_t j = lane * B + i;
+            if (j < m) bs += D[j];
+        }
+        uint64_t W = wave_sum(bs);
+        uint32_t t = 0;
+        bool bad = false;
+        for (; t < q.steps && W > 0; ++t) {
+            const U4 r = philox4x32_10(q.inst, t, kTagMdrwStep, 0, seed);
+            const uint64_t x = mulhi64(((uint64_t)r.y << 32) | r.x, W);
+            // block of the pick: first lane whose inclusive block sum exceeds x (exists: total W > x)
+            const uint64_t incl = wave_inclusive_sum(bs);
+            const unsigned long long mk = __ballot(incl > x);
+            const int L = __ffsll(mk) - 1;
+            const uint64_t xr = x - __shfl(incl - bs, L, kWave);
+            // entry of the pick inside block L
+            uint32_t j = 0;
+            uint64_t acc = 0;
+            for (uint32_t c0 = 0; c0 < B; c0 += kWave) {
+                const uint32_t o = c0 + lane, idx = L * B + o;
+                const uint64_t w = (o < B && idx < m) ? D[idx] : 0;
+                const uint64_t inc2 = wave_inclusive_sum(w) + acc;
+                const unsigned long long mk2 = __ballot(inc2 > xr);
+                if (mk2) {
+                    j = L * B + c0 + (__ffsll(mk2) - 1);
+                    break;
+                }
+                acc = __shfl(inc2, kWave - 1, kWave);
+            }
+            const uint32_t u = F[j], du = D[j];
+            Hdr h;
+            load_hdr<FMT>(g, u, h);
+            const uint32_t k = (uint32_t)mulhi64(((uint64_t)r.w << 32) | r.z, du);
+            const uint32_t v = get_neighbor<FMT>(g, h, u, k, nullptr);
+            if (v >= g.n) {  // the graph decodes to garbage: stop this instance
+                if (lane == 0) atomicCAS(err, kErrNone, kErrBadNeighbor);
+                bad = true;
+                break;
+            }
+            const uint32_t dv = g.deg[v];
+            __syncthreads();  // every lane has read F[j], D[j]
+            if (lane == 0) {
+                ssrc[q.step_base + t] = u;
+                sdst[q.step_base + t] = v;
+                F[j] = v;
+                D[j] = dv;
+            }
+            if (lane == L) bs = bs - du + dv;
+            W = W - du + dv;
+            __syncthreads();
+        }
+        if (lane == 0) scnt[gi] = bad ? 0 : t;
+        __syncthreads();  // F and D are rewritten by the next instance
+    }
+}
+
+// ---------------------------------------------------------------------------- 2. vertex sets
+
+// keys: init frontier then step destinations, padded to `cap` (a power of two) with ~0.
+__global__ __launch_bounds__(kSortBlock) void mdrw_verts_kernel(const MdDesc* __restrict__ desc,
+                                                                const uint32_t* __restrict__ init,
+                                                                const uint32_t* __restrict__ sdst,
+                                                                const uint32_t* __restrict__ scnt,
+                                                                uint32_t cap, uint32_t* __restrict__ vbound,
+                                                                uint32_t* __restrict__ vcnt) {
+    extern __shared__ uint32_t keys[];
+    __shared__ uint32_t part[kSortBlock / kWave];
+    const MdDesc q = desc[blockIdx.x];
+    const uint32_t m = q.m, ns = scnt[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < cap; i += kSortBlock) {
+        uint32_t k = 0xFFFFFFFFu;
+        if (i < m) k = init[q.init_base + i];
+        else if (i < m + ns) k = sdst[q.step_base + (i - m)];
+        keys[i] = k;
+    }
+    __syncthreads();
+    for (uint32_t size = 2; size <= cap; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t i = threadIdx.x; i < cap; i += kSortBlock) {
+                const uint32_t o = i ^ stride;
+                if (o > i) {
+                    const uint32_t a = keys[i], b = keys[o];
+                    const bool up = (i & size) == 0;
+                    if ((a > b) == up) {
+                        keys[i] = b;
+                        keys[o] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // keep the first of each run; thread t owns the consecutive keys [t*per, (t+1)*per)
+    const uint32_t per = cap / kSortBlock ? cap / kSortBlock : 1;
+    const uint32_t lo = threadIdx.x * per;
+    uint32_t c = 0;
+    for (uint32_t i = lo; i < lo + per && i < cap; ++i) {
+        const uint32_t k = keys[i];
+        c += k != 0xFFFFFFFFu && (i == 0 || keys[i - 1] != k);
+    }
+    uint32_t tot;
+    uint32_t pos = block_exclusive_sum<uint32_t, kSortBlock>(c, tot, part);
+    for (uint32_t i = lo; i < lo + per && i < cap; ++i) {
+        const uint32_t k = keys[i];
+        if (k != 0xFFFFFFFFu && (i == 0 || keys[i - 1] != k)) vbound[q.vert_base + pos++] = k;
+    }
+    if (threadIdx.x == 0) vcnt[blockIdx.x] = tot;
+}
+
+struct U32Gen {
+    const uint32_t* a;
+    __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return a[i]; }
+};
+
+struct U64Gen {
+    const uint64_t* a;
+    __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return a[i]; }
+};
+
+// Copies instance blockIdx.x's `cnt` entries from its bound-layout region to the compact one.
+__global__ __launch_bounds__(256) void mdrw_compact_kernel(const uint32_t* __restrict__ src,
+                                                           const MdDesc* __restrict__ desc, int which,
+                                                           const uint32_t* __restrict__ cnt,
+                                                           const uint64_t* __restrict__ off,
+                                                           uint32_t* __restrict__ dst) {
+    const MdDesc q = desc[blockIdx.x];
+    const uint64_t base = which == 0 ? q.step_base : q.vert_base;
+    const uint64_t o = off[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < cnt[blockIdx.x]; i += 256) dst[o + i] = src[base + i];
+}
+
+// ---------------------------------------------------------------------------- 3. induced edges
+
+struct ChunkGen {  // chunks of vertex gv
+    const uint32_t* verts;
+    const uint32_t* deg;
+    __device__ __forceinline__ uint64_t operator()(uint64_t gv) const {
+        return (deg[verts[gv]] + kIndChunk - 1) / kIndChunk;
+    }
+};
+
+// chunk c of vertex gv: cv[c] = gv, ck[c] = chunk index within the vertex
+__global__ __launch_bounds__(256) void mdrw_expand_kernel(const uint32_t* __restrict__ verts,
+                                                          const uint32_t* __restrict__ deg,
+                                                          const uint64_t* __restrict__ coff, uint64_t nv,
+                                                          uint32_t* __restrict__ cv, uint32_t* __restrict__ ck) {
+    const uint64_t gv = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (gv >= nv) return;
+    const uint64_t b = coff[gv], e = coff[gv + 1];
+    for (uint64_t c = b; c < e; ++c) {
+        cv[c] = (uint32_t)gv;
+        ck[c] = (uint32_t)(c - b);
+    }
+}
+
+__device__ __forceinline__ uint32_t ht_slot(uint32_t v, uint32_t mask) { return (v * 0x9E3779B1u) >> 7 & mask; }
+
+template <int FMT, bool WRITE>
+__global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, const uint64_t* __restrict__ voff,
+                                                                 const uint32_t* __restrict__ verts,
+                                                                 const uint64_t* __restrict__ coff,
+                                                                 const uint32_t* __restrict__ cv,
+                                                                 const uint32_t* __restrict__ ck,
+                                                                 uint32_t slots, uint64_t* __restrict__ ccnt,
+                                                                 const uint64_t* __restrict__ cpos,
+                                                                 uint32_t* __restrict__ isrc,
+                                                                 uint32_t* __restrict__ idst) {
+    extern __shared__ uint32_t table[];
+    const uint32_t mask = slots - 1;
+    const uint64_t v0 = voff[blockIdx.x], v1 = voff[blockIdx.x + 1];
+    for (uint32_t i = threadIdx.x; i < slots; i += kIndBlock) table[i] = 0xFFFFFFFFu;
+    __syncthreads();
+    for (uint64_t i = v0 + threadIdx.x; i < v1; i += kIndBlock) {
+        const uint32_t v = verts[i];
+        uint32_t s = ht_slot(v, mask);
+        while (atomicCAS(&table[s], 0xFFFFFFFFu, v) != 0xFFFFFFFFu) s = (s + 1) & mask;  // ids unique
+    }
+    __syncthreads();
+    const int lane = lane_id(), wid = threadIdx.x / kWave;
+    const uint64_t c0 = coff[v0], c1 = coff[v1];
+    for (uint64_t c = c0 + wid; c < c1; c += kIndWaves) {
+        const uint32_t x = verts[cv[c]];
+        Hdr h;
+        load_hdr<FMT>(g, x, h);
+        const uint32_t kb = ck[c] * kIndChunk;
+        const uint32_t ke = min(h.d, kb + kIndChunk);
+        uint32_t start = kb;
+        if (FMT == kGT && kb) {  // back up to the head of the block holding kb
+            uint32_t blk, i;
+            gt_locate(h, kb, blk, i);
+            start = kb - i;
+        }
+        uint64_t out = WRITE ? cpos[c] : 0;
+        uint32_t found = 0, carry = 0;
+        for (uint32_t base = start; base < ke; base += kWave) {
+            const uint32_t k = base + lane;
+            const bool act = k < ke;
+            const uint32_t kk = act ? k : ke - 1;
+            uint32_t y;
+            if (FMT == kLE) {
+                y = le_get(g.w, h, x, kk);
+            } else {
+                uint32_t blk, i;
+                gt_locate(h, kk, blk, i);
+                const uint64_t bo = gt_block_bit(h, blk);
+                int head;
+                if (i == 0) {
+                    y = gt_head(g.w, h, x, blk, bo);
+                    head = 1;
+                } else {
+                    y = read_bits(g.w, bo + h.f1 + (uint64_t)(i - 1) * h.f2, h.f2) + 1;
+                    head = 0;
+                }
+                if (lane == 0 && !head) {
+                    y += carry;
+                    head = 1;
+                }
+                y = wave_segmented_sum(y, head);
+                carry = __shfl(y, kWave - 1, kWave);
+            }
+            bool in = false;
+            if (act && k >= kb) {
+                uint32_t s = ht_slot(y, mask);
+                for (;;) {
+                    const uint32_t t = table[s];
+                    if (t == y) {
+                        in = true;
+                        break;
+                    }
+                    if (t == 0xFFFFFFFFu) break;
+                    s = (s + 1) & mask;
+                }
+            }
+            const unsigned long long mk = __ballot(in);
+            if (WRITE && in) {
+                const uint64_t p = out + __popcll(mk & ((1ull << lane) - 1));
+                isrc[p] = x;
+                idst[p] = y;
+            }
+            out += __popcll(mk);
+            found += __popcll(mk);
+        }
+        if (!WRITE && lane == 0) ccnt[c] = found;
+    }
+}
+
+// ind_off[i] = cpos[coff[voff[i]]]
+__global__ void mdrw_ind_off_kernel(const uint64_t* __restrict__ voff, const uint64_t* __restrict__ coff,
+                                    const uint64_t* __restrict__ cpos, uint32_t I, uint64_t* __restrict__ ioff) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= I) ioff[i] = cpos[coff[voff[i]]];
+}
+
+// ---------------------------------------------------------------------------- host
+
+struct DevMem {  // owned device allocation
+    void* p = nullptr;
+    ~DevMem() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, std::max<size_t>(bytes, 256)); }
+    template <typename T>
+    T* as() const { return (T*)p; }
+};
+
+}  // namespace grass
+
+struct grass_mdrw_result {
+    grass_mdrw_output out{};
+    grass::DevMem steps, verts, ind, offs;
+};
+
+namespace grass {
+
+static uint32_t next_pow2(uint32_t x) {
+    uint32_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// Kernels whose dynamic LDS may exceed the default per-launch limit.
+static void allow_lds() {
+    static bool done = false;
+    if (done) return;
+    done = true;
+    const int big = 160 * 1024;
+    (void)hipFuncSetAttribute((const void*)mdrw_verts_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    (void)hipFuncSetAttribute((const void*)mdrw_walk_kernel<kLE>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    (void)hipFuncSetAttribute((const void*)mdrw_walk_kernel<kGT>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    (void)hipFuncSetAttribute((const void*)mdrw_induced_kernel<kLE, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    (void)hipFuncSetAttribute((const void*)mdrw_induced_kernel<kLE, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    (void)hipFuncSetAttribute((const void*)mdrw_induced_kernel<kGT, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    (void)hipFuncSetAttribute((const void*)mdrw_induced_kernel<kGT, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, big);
+}
+
+static int sync_read(hipStream_t s, void* host, const void* dev, size_t bytes) {
+    hipError_t e = hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return status_of(e);
+}
+
+int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t target, uint64_t seed,
+         hipStream_t s, grass_mdrw_result** out) {
+    if (!out || (I && !reqs) || target > GRASS_MDRW_MAX_TARGET) return GRASS_EINVAL;
+    *out = nullptr;
+    std::vector<MdDesc> desc(I);
+    uint64_t tot_steps = 0, tot_init = 0, tot_vb = 0;
+    uint32_t mmax = 1, vmax = 1;
+    for (uint32_t i = 0; i < I; ++i) {
+        const uint32_t m = reqs[i].frontier;
+        if (m == 0 || m > GRASS_MDRW_MAX_FRONTIER) return GRASS_EINVAL;
+        const uint32_t st = target > m ? target - m : 0;
+        desc[i] = MdDesc{tot_steps, tot_init, tot_vb, reqs[i].instance, m, st, 0};
+        tot_steps += st;
+        tot_init += m;
+        tot_vb += (uint64_t)m + st;
+        mmax = std::max(mmax, m);
+        vmax = std::max(vmax, m + st);
+    }
+    std::unique_ptr<grass_mdrw_result> res(new (std::nothrow) grass_mdrw_result());
+    if (!res) return GRASS_ENOMEM;
+    res->out.count = I;
+    if (res->offs.alloc(3 * (uint64_t)(I + 1) * 8) != hipSuccess) return GRASS_ENOMEM;
+    uint64_t* step_off = res->offs.as<uint64_t>();
+    uint64_t* vert_off = step_off + (I + 1);
+    uint64_t* ind_off = vert_off + (I + 1);
+    res->out.step_off_dev = step_off;
+    res->out.vert_off_dev = vert_off;
+    res->out.ind_off_dev = ind_off;
+    if (I == 0) {
+        uint64_t z[3] = {0, 0, 0};
+        for (int k = 0; k < 3; ++k)
+            if (hipMemcpy(step_off + k, &z[k], 8, hipMemcpyHostToDevice) != hipSuccess) return GRASS_EDEVICE;
+        *out = res.release();
+        return GRASS_OK;
+    }
+    const DevGraph dg = g->dev();
+    const bool le = g->fmt == kLE;
+    allow_lds();
+
+    // phase 1: walks and vertex sets in bound layout
+    const uint64_t nbI = scan_blocks(I);
+    DevMem tmp;
+    Carve need(nullptr);
+    auto plan = [&](Carve& c) {
+        c.take<MdDesc>(I);
+        c.take<unsigned long long>(1);
+        c.take<uint32_t>(tot_init);
+        c.take<uint32_t>(tot_steps);
+        c.take<uint32_t>(tot_steps);
+        c.take<uint32_t>(I);
+        c.take<uint32_t>(tot_vb);
+        c.take<uint32_t>(I);
+        c.take<uint64_t>(nbI + 2);
+        c.take<uint64_t>(1);
+    };
+    plan(need);
+    if (tmp.alloc(need.off + 256) != hipSuccess) return GRASS_ENOMEM;
+    Carve c(tmp.p);
+    MdDesc* d_desc = c.take<MdDesc>(I);
+    unsigned long long* d_queue = c.take<unsigned long long>(1);
+    uint32_t* d_init = c.take<uint32_t>(tot_init);
+    uint32_t* d_ssrc = c.take<uint32_t>(tot_steps);
+    uint32_t* d_sdst = c.take<uint32_t>(tot_steps);
+    uint32_t* d_scnt = c.take<uint32_t>(I);
+    uint32_t* d_vb = c.take<uint32_t>(tot_vb);
+    uint32_t* d_vcnt = c.take<uint32_t>(I);
+    uint64_t* d_part = c.take<uint64_t>(nbI + 2);
+    uint64_t* d_tot = c.take<uint64_t>(1);
+    hipError_t e = hipMemcpyAsync(d_desc, desc.data(), I * sizeof(MdDesc), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return status_of(e);
+    hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, s, d_queue, (uint64_t)1, 0ull);
+    {
+        const size_t lds = 2ull * mmax * 4;
+        const void* kfn = le ? (const void*)mdrw_walk_kernel<kLE> : (const void*)mdrw_walk_kernel<kGT>;
+        int dev = 0, cus = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kfn, kWave, lds);
+        const unsigned grid = (unsigned)std::min<uint64_t>(I, (uint64_t)std::max(cus, 1) * std::max(per, 1));
+        if (le)
+            hipLaunchKernelGGL(mdrw_walk_kernel<kLE>, dim3(grid), dim3(kWave), lds, s, dg, d_desc, I, d_queue,
+                               seed, d_init, d_ssrc, d_sdst, d_scnt, mmax, g->d_err);
+        else
+            hipLaunchKernelGGL(mdrw_walk_kernel<kGT>, dim3(grid), dim3(kWave), lds, s, dg, d_desc, I, d_queue,
+                               seed, d_init, d_ssrc, d_sdst, d_scnt, mmax, g->d_err);
+    }
+    const uint32_t cap = next_pow2(vmax);
+    hipLaunchKernelGGL(mdrw_verts_kernel, dim3(I), dim3(kSortBlock), (size_t)cap * 4, s, d_desc, d_init,
+                       d_sdst, d_scnt, cap, d_vb, d_vcnt);
+    e = exclusive_scan(U32Gen{d_scnt}, Count{I, nullptr}, I, step_off, d_part, d_tot, s);
+    if (e == hipSuccess) e = exclusive_scan(U32Gen{d_vcnt}, Count{I, nullptr}, I, vert_off, d_part, d_tot, s);
+    if (e != hipSuccess) return status_of(e);
+    uint64_t totals[2];
+    if (hipMemcpyAsync(&totals[0], step_off + I, 8, hipMemcpyDeviceToHost, s) != hipSuccess) return GRASS_EDEVICE;
+    int rc = sync_read(s, &totals[1], vert_off + I, 8);
+    if (rc) return rc;
+    uint32_t err[2] = {0, 0};
+    if (hipMemcpy(err, g->d_err, sizeof(err), hipMemcpyDeviceToHost) != hipSuccess) return GRASS_EDEVICE;
+    if (err[0] == kErrBadNeighbor) {
+        (void)hipMemset(g->d_err, 0, sizeof(err));
+        return GRASS_EFORMAT;
+    }
+    const uint64_t NS = totals[0], NV = totals[1];
+    res->out.steps = NS;
+    res->out.verts = NV;
+    if (res->steps.alloc(NS * 8) != hipSuccess || res->verts.alloc(NV * 4) != hipSuccess) return GRASS_ENOMEM;
+    uint32_t* o_ssrc = res->steps.as<uint32_t>();
+    uint32_t* o_sdst = o_ssrc + NS;
+    uint32_t* o_verts = res->verts.as<uint32_t>();
+    res->out.step_src_dev = o_ssrc;
+    res->out.step_dst_dev = o_sdst;
+    res->out.verts_dev = o_verts;
+    hipLaunchKernelGGL(mdrw_compact_kernel, dim3(I), dim3(256), 0, s, d_ssrc, d_desc, 0, d_scnt, step_off, o_ssrc);
+    hipLaunchKernelGGL(mdrw_compact_kernel, dim3(I), dim3(256), 0, s, d_sdst, d_desc, 0, d_scnt, step_off, o_sdst);
+    hipLaunchKernelGGL(mdrw_compact_kernel, dim3(I), dim3(256), 0, s, d_vb, d_desc, 1, d_vcnt, vert_off, o_verts);
+
+    // phase 2: chunks of the sampled vertices' lists
+    const uint64_t nbV = scan_blocks(NV);
+    DevMem tmp2;
+    if (tmp2.alloc((NV + 1) * 8 + (nbV + 2) * 8 + 512) != hipSuccess) return GRASS_ENOMEM;
+    Carve c2(tmp2.p);
+    uint64_t* d_coff = c2.take<uint64_t>(NV + 1);
+    uint64_t* d_part2 = c2.take<uint64_t>(nbV + 2);
+    e = exclusive_scan(ChunkGen{o_verts, dg.deg}, Count{NV, nullptr}, NV, d_coff, d_part2, d_part2 + nbV + 1, s);
+    if (e != hipSuccess) return status_of(e);
+    uint64_t NC = 0;
+    rc = sync_read(s, &NC, d_coff + NV, 8);
+    if (rc) return rc;
+    const uint64_t nbC = scan_blocks(NC);
+    DevMem tmp3;
+    if (tmp3.alloc(NC * 8 + (NC + 1) * 16 + (nbC + 2) * 8 + 1024) != hipSuccess) return GRASS_ENOMEM;
+    Carve c3(tmp3.p);
+    uint32_t* d_cv = c3.take<uint32_t>(NC);
+    uint32_t* d_ck = c3.take<uint32_t>(NC);
+    uint64_t* d_ccnt = c3.take<uint64_t>(NC + 1);
+    uint64_t* d_cpos = c3.take<uint64_t>(NC + 1);
+    uint64_t* d_part3 = c3.take<uint64_t>(nbC + 2);
+    if (NV)
+        hipLaunchKernelGGL(mdrw_expand_kernel, dim3((unsigned)((NV + 255) / 256)), dim3(256), 0, s, o_verts,
+                           dg.deg, d_coff, NV, d_cv, d_ck);
+    const uint32_t slots = 2 * next_pow2(vmax);
+    const size_t tl = (size_t)slots * 4;
+    if (le)
+        hipLaunchKernelGGL((mdrw_induced_kernel<kLE, false>), dim3(I), dim3(kIndBlock), tl, s, dg, vert_off,
+                           o_verts, d_coff, d_cv, d_ck, slots, d_ccnt, nullptr, nullptr, nullptr);
+    else
+        hipLaunchKernelGGL((mdrw_induced_kernel<kGT, false>), dim3(I), dim3(kIndBlock), tl, s, dg, vert_off,
+                           o_verts, d_coff, d_cv, d_ck, slots, d_ccnt, nullptr, nullptr, nullptr);
+    e = exclusive_scan(U64Gen{d_ccnt}, Count{NC, nullptr}, NC, d_cpos, d_part3, d_part3 + nbC + 1, s);
+    if (e != hipSuccess) return status_of(e);
+    uint64_t NI = 0;
+    rc = sync_read(s, &NI, d_cpos + NC, 8);
+    if (rc) return rc;
+    res->out.induced = NI;
+    if (res->ind.alloc(NI * 8) != hipSuccess) return GRASS_ENOMEM;
+    uint32_t* o_isrc = res->ind.as<uint32_t>();
+    uint32_t* o_idst = o_isrc + NI;
+    res->out.ind_src_dev = o_isrc;
+    res->out.ind_dst_dev = o_idst;
+    if (le)
+        hipLaunchKernelGGL((mdrw_induced_kernel<kLE, true>), dim3(I), dim3(kIndBlock), tl, s, dg, vert_off,
+                           o_verts, d_coff, d_cv, d_ck, slots, nullptr, d_cpos, o_isrc, o_idst);
+    else
+        hipLaunchKernelGGL((mdrw_induced_kernel<kGT, true>), dim3(I), dim3(kIndBlock), tl, s, dg, vert_off,
+                           o_verts, d_coff, d_cv, d_ck, slots, nullptr, d_cpos, o_isrc, o_idst);
+    hipLaunchKernelGGL(mdrw_ind_off_kernel, dim3((I + 256) / 256), dim3(256), 0, s, vert_off, d_coff, d_cpos, I,
+                       ind_off);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // the temporaries are freed on return
+    if (e != hipSuccess) return status_of(e);
+    *out = res.release();
+    return GRASS_OK;
+}
+
+}  // namespace grass
+
+extern "C" {
+
+int grass_mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t count, uint32_t target,
+               uint64_t seed, void* stream, grass_mdrw_result** out) {
+    if (!g) return GRASS_EINVAL;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    if (hipSetDevice(g->device) != hipSuccess) return GRASS_EDEVICE;
+    hipStream_t s = stream ? (hipStream_t)stream : g->stream;
+    int rc = grass::mdrw(g, reqs, count, target, seed, s, out);
+    (void)hipSetDevice(prev);
+    return rc;
+}
+
+int grass_mdrw_output_get(const grass_mdrw_result* res, grass_mdrw_output* out) {
+    if (!res || !out) return GRASS_EINVAL;
+    *out = res->out;
+    return GRASS_OK;
+}
+
+int grass_mdrw_fetch(const grass_mdrw_result* res, uint64_t* step_off, uint32_t* step_src,
+                     uint32_t* step_dst, uint64_t* vert_off, uint32_t* verts, uint64_t* ind_off,
+                     uint32_t* ind_src, uint32_t* ind_dst) {
+    if (!res) return GRASS_EINVAL;
+    const grass_mdrw_output& o = res->out;
+    const uint64_t no = (uint64_t)o.count + 1;
+    struct Part {
+        void* host;
+        const void* dev;
+        uint64_t bytes;
+    } parts[] = {{step_off, o.step_off_dev, no * 8}, {step_src, o.step_src_dev, o.steps * 4},
+                 {step_dst, o.step_dst_dev, o.steps * 4}, {vert_off, o.vert_off_dev, no * 8},
+                 {verts, o.verts_dev, o.verts * 4},      {ind_off, o.ind_off_dev, no * 8},
+                 {ind_src, o.ind_src_dev, o.induced * 4}, {ind_dst, o.ind_dst_dev, o.induced * 4}};
+    for (const Part& p : parts) {
+        if (!p.host || !p.bytes) continue;
+        if (hipMemcpy(p.host, p.dev, p.bytes, hipMemcpyDeviceToHost) != hipSuccess) return GRASS_EDEVICE;
+    }
+    return GRASS_OK;
+}
+
+void grass_mdrw_free(grass_mdrw_result* res) {
+    if (!res) return;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    delete res;
+    (void)hipSetDevice(prev);
+}
+
+}  // extern "C"

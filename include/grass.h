@@ -160,6 +160,58 @@ int grass_khop_bound(const grass_graph *g, const grass_khop_request *reqs, uint3
 int grass_khop_batch(grass_graph *g, const grass_khop_request *reqs, uint32_t count, uint64_t seed,
                      grass_khop_output *out, void *stream);
 
+/* ------------------------------------------------------------------ multi-dimensional RW */
+
+#define GRASS_MDRW_MAX_FRONTIER 8192
+#define GRASS_MDRW_MAX_TARGET 16384
+
+/* One frontier-sampling instance (Ribeiro & Towsley, IMC 2010).  Entry j of the frontier starts
+ * at floor((r1:r0) * n / 2^64) of Philox(seed, {instance, j, 0xFFFFFFFF, 0}) (the walk-start draw);
+ * then max(0, target - frontier) steps run, step t drawing Philox(seed, {instance, t,
+ * 0xFFFFFFFE, 0}) = (r0, r1, r2, r3): x = floor((r1:r0) * W / 2^64) with W the frontier's
+ * degree sum picks the first entry whose running degree sum exceeds x (degree-proportional
+ * choice); k = floor((r3:r2) * d / 2^64) picks its neighbour k, which replaces the entry.  An
+ * instance whose frontier has no edge left stops early.  1 <= frontier <= GRASS_MDRW_MAX_FRONTIER. */
+typedef struct {
+    uint32_t instance;
+    uint32_t frontier;
+} grass_mdrw_request;
+
+/* Output of a batch (device memory owned by the result).  Instance i's part of each array is
+ * [x_off_dev[i], x_off_dev[i+1]):
+ *   steps   — the traversed edges (step_src, step_dst) in step order;
+ *   verts   — the sampled vertex set: initial frontier and every step's destination, ascending;
+ *   induced — every adjacency entry (x, y) with x and y both in verts, ordered by x and then by
+ *             y's position in x's list. */
+typedef struct {
+    uint32_t count;
+    const uint64_t *step_off_dev;
+    const uint32_t *step_src_dev, *step_dst_dev;
+    const uint64_t *vert_off_dev;
+    const uint32_t *verts_dev;
+    const uint64_t *ind_off_dev;
+    const uint32_t *ind_src_dev, *ind_dst_dev;
+    uint64_t steps, verts, induced;  /* totals */
+} grass_mdrw_output;
+
+typedef struct grass_mdrw_result grass_mdrw_result;
+
+/* Runs every instance of the batch in one persistent launch (one wave per instance, taken from a
+ * device queue), then sorts each sample and extracts its induced edges.  Replaces MultiRWApp
+ * (LE/src/apps/multiRW/multiRW.cu:22-45 next, :74-82 stepTransits, :85-96 initializeSample;
+ * GT/src/apps/multiRW/multiRW.cu likewise: 80 roots, 100 steps, uniform root choice, no output)
+ * driven by doTransitParallelSampling (LE/src/grass.cu:1299-1900).  target <=
+ * GRASS_MDRW_MAX_TARGET.  Synchronous on `stream`; *out is freed with grass_mdrw_free. */
+int grass_mdrw(grass_graph *g, const grass_mdrw_request *reqs, uint32_t count, uint32_t target,
+               uint64_t seed, void *stream, grass_mdrw_result **out);
+int grass_mdrw_output_get(const grass_mdrw_result *res, grass_mdrw_output *out);
+/* Copies the result to host memory: offsets arrays hold count+1 entries, the others the totals
+ * of grass_mdrw_output.  NULL pointers are skipped.  Synchronises. */
+int grass_mdrw_fetch(const grass_mdrw_result *res, uint64_t *step_off, uint32_t *step_src,
+                     uint32_t *step_dst, uint64_t *vert_off, uint32_t *verts, uint64_t *ind_off,
+                     uint32_t *ind_src, uint32_t *ind_dst);
+void grass_mdrw_free(grass_mdrw_result *res);
+
 /* ------------------------------------------------------------------ serving
  * No reference counterpart (the reference runs one compile-time app over every vertex,
  * LE/src/grass.cu:1924-1992).  A session keeps the graph resident and runs a scheduler thread
@@ -188,8 +240,9 @@ typedef struct {
 typedef struct {
     uint64_t memory_budget;    /* HBM arena for request inputs and outputs (0 = 4 GiB) */
     uint32_t max_batch;        /* requests per launch group (0 = 512) */
-    uint32_t max_inflight;     /* launch groups on the device at once (0 = 3) */
+    uint32_t max_inflight;     /* launch groups on the device at once (0 = 4) */
     uint64_t seed;
+    uint64_t max_group_edges;  /* edge bound that closes a group early (0 = 4M) */
 } grass_session_config;
 
 typedef struct {
@@ -239,10 +292,11 @@ int grass_session_destroy(grass_session *s);
 
 /* Open-loop trace replay for latency measurement: submits reqs[i] at arrival_us[i]
  * (microseconds after the call), releases each result as it completes, and writes
- * latency_us[i] = completion - arrival (-1 when rejected).  *wall_us receives the time from
- * the first arrival to the last completion. */
+ * latency_us[i] = completion - arrival (-1 when rejected) and, when queue_us is not NULL,
+ * queue_us[i] = launch of its group - arrival.  *wall_us receives the time from the first
+ * arrival to the last completion. */
 int grass_session_replay(grass_session *s, const grass_request *reqs, const double *arrival_us,
-                         uint32_t count, double *latency_us, double *wall_us);
+                         uint32_t count, double *latency_us, double *queue_us, double *wall_us);
 
 #ifdef __cplusplus
 }
