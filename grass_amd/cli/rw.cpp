@@ -1,0 +1,4 @@
+// rw: random walks (LE/src/apps/randomwalk/main.cu) on the MI355X.
+#include "app_main.hpp"
+
+int main(int argc, char* argv[]) { return grass_app::appMain<grass_app::RandomWalkApp>(argc, argv); }

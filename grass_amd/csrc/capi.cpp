@@ -179,6 +179,35 @@ int grass_device_arrays(const grass_graph* g, const uint32_t** words, const uint
     return GRASS_OK;
 }
 
+int grass_device_alloc(grass_graph* g, uint64_t bytes, void** ptr_dev) {
+    if (!g || !ptr_dev) return GRASS_EINVAL;
+    *ptr_dev = nullptr;
+    DeviceGuard guard(g->device);
+    if (!guard.ok) return GRASS_EDEVICE;
+    return hipMalloc(ptr_dev, bytes ? bytes : 1) == hipSuccess ? GRASS_OK : GRASS_ENOMEM;
+}
+
+int grass_device_free(grass_graph* g, void* ptr_dev) {
+    if (!g) return GRASS_EINVAL;
+    if (!ptr_dev) return GRASS_OK;
+    DeviceGuard guard(g->device);
+    return hipFree(ptr_dev) == hipSuccess ? GRASS_OK : GRASS_EDEVICE;
+}
+
+int grass_copy_to_host(grass_graph* g, void* host, const void* src_dev, uint64_t bytes) {
+    if (!g || (bytes && (!host || !src_dev))) return GRASS_EINVAL;
+    if (!bytes) return GRASS_OK;
+    DeviceGuard guard(g->device);
+    return hipMemcpy(host, src_dev, bytes, hipMemcpyDeviceToHost) == hipSuccess ? GRASS_OK : GRASS_EDEVICE;
+}
+
+int grass_copy_to_device(grass_graph* g, void* dst_dev, const void* host, uint64_t bytes) {
+    if (!g || (bytes && (!host || !dst_dev))) return GRASS_EINVAL;
+    if (!bytes) return GRASS_OK;
+    DeviceGuard guard(g->device);
+    return hipMemcpy(dst_dev, host, bytes, hipMemcpyHostToDevice) == hipSuccess ? GRASS_OK : GRASS_EDEVICE;
+}
+
 int grass_sync(grass_graph* g, void* stream) {
     if (!g) return GRASS_EINVAL;
     DeviceGuard guard(g->device);

@@ -75,6 +75,15 @@ int grass_info(const grass_graph *g, grass_graph_info *info);
 int grass_device_arrays(const grass_graph *g, const uint32_t **words_dev,
                         const uint64_t **starts_dev, const uint32_t **degree_dev);
 
+/* Device memory and copies for callers without a HIP runtime of their own (FFI bindings, the
+ * rw/ks/mrw command-line tools).  The reference allocates everything itself inside nextdoor
+ * (allocNextDoorDataOnGPU, LE/src/grass.cu:1129-1270) and never returns samples.  The copies
+ * are synchronous.  grass_device_free(g, NULL) is a no-op. */
+int grass_device_alloc(grass_graph *g, uint64_t bytes, void **ptr_dev);
+int grass_device_free(grass_graph *g, void *ptr_dev);
+int grass_copy_to_host(grass_graph *g, void *host, const void *src_dev, uint64_t bytes);
+int grass_copy_to_device(grass_graph *g, void *dst_dev, const void *host, uint64_t bytes);
+
 /* Waits for `stream` and returns the first error a kernel of this graph recorded since the
  * last call (a decoded neighbour id >= n, a start vertex >= n), then clears it. */
 int grass_sync(grass_graph *g, void *stream);
