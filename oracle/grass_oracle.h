@@ -22,10 +22,12 @@
  *   or_parse_offsets : fixture record lengths summed == graph bits (< 8 bits padding), the
  *       n-overcount fixture.
  *   or_philox4x32_10 : Random123 known-answer vectors and the rocRAND host implementation.
- *   or_rw / or_khop / or_mdrw : invariants (each step is an edge of the golden CSR, subsets
- *       are distinct and of size min(fanout, d), induced edges are exactly the CSR restricted
- *       to the sample), chi-square uniformity, and a brute-force Python restatement on tiny
- *       graphs.  Bit-exactness of these against the reference itself is "parity unpinned":
+ *   or_rw / or_khop / or_mdrw : invariants against the golden CSR (each walk step is an edge,
+ *       dead ends pad with n; k-hop subsets are distinct, ascending, of size min(fanout, d),
+ *       frontiers are first-appearance dedups; MDRW steps are edges and the induced list is
+ *       exactly the golden CSR restricted to the sample), chi-square tests of uniform neighbour
+ *       choice, uniform subsets and degree-proportional frontier choice, and determinism per
+ *       (seed, instance).  Bit-exactness against the reference itself is "parity unpinned":
  *       the reference's outputs are never materialised and its RNG is closed cuRAND.
  */
 #ifndef GRASS_ORACLE_H

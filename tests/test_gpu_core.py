@@ -129,3 +129,20 @@ def test_rmat16_decode_and_walks(ga, oracle_mod, fmt):
     exp, _ = og.random_walk(4096, 80, seed=42, instance=0)
     np.testing.assert_array_equal(out, exp)
     g.close()
+
+
+@pytest.mark.parametrize("fmt", FMTS)
+@pytest.mark.parametrize("case", ["degrees", "rmat12", "hub"])
+def test_walk_bytes_match_oracle(ga, oracle_mod, case, fmt):
+    """The algorithmic byte count the roofline uses is the oracle's count of the same walks."""
+    g, c = _open(ga, case, fmt)
+    og = oracle_mod.Graph(c[f"{fmt}_graph"].tobytes(), c[f"{fmt}_offset"].tobytes(), fmt)
+    _, exp = og.random_walk(3000, 25, seed=5, instance=1)
+    assert g.random_walk_bytes(3000, 25, seed=5, instance=1) == exp
+    starts = np.random.default_rng(1).integers(0, g.n, size=500).astype(np.uint32)
+    _, exp_s = og.random_walk(500, 9, seed=5, instance=2, starts=starts)
+    assert g.random_walk_bytes(500, 9, seed=5, instance=2, starts=starts) == exp_s
+    reqs = [{"walkers": 3000, "length": 25, "instance": 1},
+            {"walkers": 500, "length": 9, "instance": 2, "starts": starts}]
+    assert g.random_walk_batch_bytes(reqs, seed=5) == exp + exp_s
+    g.close()
