@@ -106,6 +106,10 @@ def lib():
             "grass_mdrw_output_get": (I32, [P, P]),
             "grass_mdrw_fetch": (I32, [P, P, P, P, P, P, P, P, P]),
             "grass_mdrw_free": (None, [P]),
+            "grass_plan_create": (I32, [P, P, U32, P, U32, P, U64, P]),
+            "grass_plan_launch": (I32, [P, P]),
+            "grass_plan_khop_total": (I32, [P, P]),
+            "grass_plan_destroy": (I32, [P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -346,6 +350,10 @@ class Graph:
                 "step_dst": sd[: o.steps], "verts": vv[: o.verts], "ind_src": isrc[: o.induced],
                 "ind_dst": idst[: o.induced]}
 
+    def plan(self, rw_requests, khop_seeds, khop_fanouts, seed, khop_instances=None):
+        """Captures a fixed batch (see Plan); the buffers live as long as the plan."""
+        return Plan(self, rw_requests, khop_seeds, khop_fanouts, seed, khop_instances)
+
     @staticmethod
     def khop_request_edges(res, r):
         """Request r's edges, hop by hop concatenated (request-major view of a hop-major batch)."""
@@ -388,3 +396,63 @@ def header_functions(path=HEADER):
     text = open(path).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(grass_[a-z_0-9]+)\s*\(", text)))
+
+
+class Plan:
+    """A batch of random-walk and k-hop requests captured into a HIP graph (grass_plan_*).
+    `walks` are the walk matrices, `khop_seeds` the device seed buffers (rewrite their contents
+    between launches for new mini-batches), `khop_src/dst/offsets` the k-hop output."""
+
+    def __init__(self, graph, rw_requests, khop_seeds, khop_fanouts, seed, khop_instances=None):
+        torch = _torch()
+        self.graph = graph
+        dev = graph._cuda()
+        self.walks, self._keep = [], []
+        self._rw = (RwRequest * max(len(rw_requests), 1))()
+        for i, r in enumerate(rw_requests):
+            o = torch.empty((r["walkers"], r["length"] + 1), dtype=torch.int32, device=dev)
+            st = r.get("starts")
+            if st is not None and not hasattr(st, "data_ptr"):
+                st = torch.from_numpy(np.ascontiguousarray(st, dtype=np.uint32).view(np.int32)).to(dev)
+            self._keep.append(st)
+            self.walks.append(o)
+            self._rw[i] = RwRequest(r["walkers"], r["length"], r.get("instance", i),
+                                    st.data_ptr() if st is not None else None, o.data_ptr())
+        self._kh, keep = graph.khop_request_array(khop_seeds, khop_fanouts, khop_instances)
+        self.khop_seeds = keep
+        R = len(khop_seeds)
+        cap = graph.khop_bound(self._kh, R) if R else 0
+        H = max([len(f) for f in khop_fanouts] + [0])
+        self.R, self.H = R, H
+        self.khop_src = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+        self.khop_dst = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+        self.khop_offsets = torch.zeros(max(H * (R + 1), 1), dtype=torch.int64, device=dev)
+        self._kout = KhopOutput(self.khop_src.data_ptr(), self.khop_dst.data_ptr(), cap,
+                                self.khop_offsets.data_ptr(), 0, 0)
+        h = C.c_void_p()
+        _check(lib().grass_plan_create(graph.handle, self._rw, len(rw_requests), self._kh, R,
+                                       C.byref(self._kout), seed, C.byref(h)), "grass_plan_create")
+        self._h = h
+
+    def launch(self, stream=None):
+        _check(lib().grass_plan_launch(self._h, C.c_void_p(_stream_ptr(stream))), "grass_plan_launch")
+
+    def khop_result(self):
+        """Host view of the last completed launch's k-hop output (khop_batch's layout)."""
+        t = C.c_uint64()
+        _check(lib().grass_plan_khop_total(self._h, C.byref(t)), "grass_plan_khop_total")
+        n = t.value
+        off = self.khop_offsets.cpu().numpy().view(np.uint64)[: self.H * (self.R + 1)]
+        return {"offsets": off.reshape(self.H, self.R + 1), "src": to_numpy_u32(self.khop_src)[:n],
+                "dst": to_numpy_u32(self.khop_dst)[:n], "total": n}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().grass_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -221,6 +221,24 @@ int grass_mdrw_fetch(const grass_mdrw_result *res, uint64_t *step_off, uint32_t 
                      uint32_t *ind_src, uint32_t *ind_dst);
 void grass_mdrw_free(grass_mdrw_result *res);
 
+/* ------------------------------------------------------------------ captured batches
+ * No reference counterpart (the reference launches and synchronises every step from the host,
+ * LE/src/grass.cu:1299-1900).  A plan records a fixed batch -- random-walk requests and k-hop
+ * requests with their device buffers -- into a HIP graph, the walks and the k-hop pipeline on
+ * two forked branches, after one eager run that sizes every workspace.  grass_plan_launch
+ * replays it as one submission.  Between launches the contents of the input buffers (walk starts,
+ * k-hop seeds) may change; their addresses, the request shapes and the seed may not.  Launches of
+ * one plan must not overlap (they share its workspaces); each replay gives the result an eager
+ * grass_random_walk_batch + grass_khop_batch of the same batch gives. */
+typedef struct grass_plan grass_plan;
+int grass_plan_create(grass_graph *g, const grass_rw_request *rw, uint32_t rw_count,
+                      const grass_khop_request *khop, uint32_t khop_count,
+                      const grass_khop_output *khop_out, uint64_t seed, grass_plan **out);
+int grass_plan_launch(grass_plan *p, void *stream);
+/* Edge total of the last completed launch's k-hop part (read after the stream passed it). */
+int grass_plan_khop_total(const grass_plan *p, uint64_t *total);
+int grass_plan_destroy(grass_plan *p);
+
 /* ------------------------------------------------------------------ serving
  * No reference counterpart (the reference runs one compile-time app over every vertex,
  * LE/src/grass.cu:1924-1992).  A session keeps the graph resident and runs a scheduler thread
