@@ -13,10 +13,14 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <ctype.h>
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <memory>
 #include <new>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -33,6 +37,10 @@ struct grass_csr {
 
 struct grass_blob {
     std::vector<uint8_t> bytes;
+};
+
+struct grass_edges {
+    std::vector<uint32_t> src, dst;
 };
 
 namespace grass {
@@ -249,6 +257,34 @@ void words_to_bytes(const std::vector<uint64_t>& w, uint64_t nbytes, std::vector
     parallel_chunks(nt, nbytes, [&](int, uint64_t lo, uint64_t hi) {
         for (uint64_t j = lo; j < hi; ++j) out[j] = (uint8_t)(w[j >> 3] >> (56 - 8 * (j & 7)));
     });
+}
+
+// Whitespace-separated unsigned integers of buf[lo, hi) (lo and hi on token boundaries).
+// Parsing stops at the first token that is not a number below 2^32 - 1; *stop then holds its
+// position.  Tokens are what the reference's `fin >> u >> v` (graph_sort.cpp:45) and
+// fscanf("%ld %ld") (compressor.hpp:101) read.
+void parse_tokens(const char* buf, uint64_t lo, uint64_t hi, std::vector<uint32_t>& out, uint64_t* stop) {
+    *stop = UINT64_MAX;
+    uint64_t i = lo;
+    for (;;) {
+        while (i < hi && isspace((unsigned char)buf[i])) ++i;
+        if (i >= hi) return;
+        const uint64_t t0 = i;
+        uint64_t v = 0;
+        bool ok = buf[i] >= '0' && buf[i] <= '9';
+        while (i < hi && !isspace((unsigned char)buf[i])) {
+            const char ch = buf[i];
+            if (ch < '0' || ch > '9') ok = false;
+            else if (ok) v = v * 10 + (uint64_t)(ch - '0');
+            if (v >= 0xFFFFFFFFull) ok = false;
+            ++i;
+        }
+        if (!ok) {
+            *stop = t0;
+            return;
+        }
+        out.push_back((uint32_t)v);
+    }
 }
 
 }  // namespace
@@ -530,6 +566,97 @@ int grass_write_files(const char* prefix, const grass_blob* graph, const grass_b
         if (!ok) return GRASS_EIO;
     }
     return GRASS_OK;
+}
+
+int grass_read_edge_list(const char* path, uint64_t skip_lines, int nthreads, grass_edges** out) {
+    if (!path || !out) return GRASS_EINVAL;
+    *out = nullptr;
+    FILE* f = fopen(path, "rb");
+    if (!f) return GRASS_EIO;
+    std::vector<char> buf;
+    char tmp[1 << 16];
+    size_t got;
+    while ((got = fread(tmp, 1, sizeof(tmp), f)) > 0) buf.insert(buf.end(), tmp, tmp + got);
+    const bool err = ferror(f) != 0;
+    fclose(f);
+    if (err) return GRASS_EIO;
+    uint64_t pos = 0;
+    const uint64_t n = buf.size();
+    for (uint64_t l = 0; l < skip_lines && pos < n; ++l) {
+        while (pos < n && buf[pos] != '\n') ++pos;
+        if (pos < n) ++pos;
+    }
+    // chunks cut at whitespace so that no token straddles two chunks
+    const int nt = threads_of(nthreads);
+    std::vector<uint64_t> cut(nt + 1, n);
+    cut[0] = pos;
+    for (int t = 1; t < nt; ++t) {
+        uint64_t c = std::max(cut[t - 1], pos + (n - pos) * t / nt);
+        while (c < n && !isspace((unsigned char)buf[c])) ++c;
+        cut[t] = c;
+    }
+    std::vector<std::vector<uint32_t>> tok(nt);
+    std::vector<uint64_t> stop(nt, UINT64_MAX);
+    {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&, t] { parse_tokens(buf.data(), cut[t], cut[t + 1], tok[t], &stop[t]); });
+        for (auto& x : th) x.join();
+    }
+    std::unique_ptr<grass_edges> e(new (std::nothrow) grass_edges());
+    if (!e) return GRASS_ENOMEM;
+    uint64_t total = 0;
+    int last = nt - 1;
+    for (int t = 0; t < nt; ++t) {
+        total += tok[t].size();
+        if (stop[t] != UINT64_MAX) {
+            last = t;
+            break;
+        }
+    }
+    const uint64_t m = total / 2;  // an unpaired last token is ignored, as the reference does
+    e->src.resize(m);
+    e->dst.resize(m);
+    uint64_t k = 0;
+    for (int t = 0; t <= last; ++t)
+        for (uint32_t v : tok[t]) {
+            if (k >= 2 * m) break;
+            (k & 1 ? e->dst : e->src)[k >> 1] = v;
+            ++k;
+        }
+    *out = e.release();
+    return GRASS_OK;
+}
+
+int grass_edges_data(const grass_edges* e, const uint32_t** src, const uint32_t** dst, uint64_t* m) {
+    if (!e) return GRASS_EINVAL;
+    if (src) *src = e->src.data();
+    if (dst) *dst = e->dst.data();
+    if (m) *m = e->src.size();
+    return GRASS_OK;
+}
+
+void grass_edges_free(grass_edges* e) { delete e; }
+
+int grass_write_edge_list(const char* path, const grass_csr* csr, int nthreads) {
+    if (!path || !csr) return GRASS_EINVAL;
+    const int nt = threads_of(nthreads);
+    std::vector<std::string> part(nt);
+    parallel_chunks(nt, csr->n, [&](int t, uint64_t lo, uint64_t hi) {
+        std::string& o = part[t];
+        char line[32];
+        for (uint64_t u = lo; u < hi; ++u)
+            for (uint64_t k = csr->rowptr[u]; k < csr->rowptr[u + 1]; ++k) {
+                int len = snprintf(line, sizeof(line), "%llu %u\n", (unsigned long long)u, csr->col[k]);
+                o.append(line, (size_t)len);
+            }
+    });
+    FILE* f = fopen(path, "wb");
+    if (!f) return GRASS_EIO;
+    bool ok = true;
+    for (const std::string& p : part) ok = ok && fwrite(p.data(), 1, p.size(), f) == p.size();
+    ok = (fclose(f) == 0) && ok;
+    return ok ? GRASS_OK : GRASS_EIO;
 }
 
 }  // extern "C"

@@ -48,8 +48,21 @@ int grass_compress(const grass_csr *csr, int format, int nthreads, grass_blob **
 int grass_blob_data(const grass_blob *blob, const uint8_t **data, uint64_t *size);
 void grass_blob_free(grass_blob *blob);
 
-/* Writes <prefix>.graph and <prefix>.offset. */
+/* Writes <prefix>.graph and <prefix>.offset (Compressor::write_cgr,
+ * LE/chunk_compress/compressor.hpp:130-187). */
 int grass_write_files(const char *prefix, const grass_blob *graph, const grass_blob *offset);
+
+/* Text edge lists, as the reference tools read and write them.  Reading skips `skip_lines`
+ * lines, then takes whitespace-separated unsigned integers in pairs (u, v) until the first token
+ * that is not a number below 2^32 - 1 (graph_sort.cpp:41-46 `fin >> u >> v`, compressor.hpp:
+ * 93-128 fscanf("%ld %ld")); an unpaired last number is dropped.  Parsing is split over
+ * nthreads at whitespace.  Writing prints the CSR's entries as "u v" lines in order
+ * (graph_sort.cpp:74-77). */
+typedef struct grass_edges grass_edges;
+int grass_read_edge_list(const char *path, uint64_t skip_lines, int nthreads, grass_edges **out);
+int grass_edges_data(const grass_edges *e, const uint32_t **src, const uint32_t **dst, uint64_t *m);
+void grass_edges_free(grass_edges *e);
+int grass_write_edge_list(const char *path, const grass_csr *csr, int nthreads);
 
 #ifdef __cplusplus
 }

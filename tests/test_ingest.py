@@ -88,3 +88,64 @@ def test_against_reference_tools_rmat14(tmp_path):
         gb, ob = csr.compress(fmt, threads=4)
         assert gb == (tmp_path / f"{fmt}.graph").read_bytes()
         assert ob == (tmp_path / f"{fmt}.offset").read_bytes()
+
+
+BIN = os.path.join(REPO, "grass_amd", "bin")
+
+
+def _edge_text(path, rng, header=0):
+    """Directed edges over sparse large ids, with self loops, duplicates and a header."""
+    ids = rng.choice(10**9, size=400, replace=False)
+    u, v = ids[rng.integers(0, 400, 6000)], ids[rng.integers(0, 400, 6000)]
+    u[:50] = v[:50]  # self loops
+    lines = [f"# header line {i}" for i in range(header)] + [f"{a} {b}" for a, b in zip(u, v)]
+    lines += [f"{u[7]} {v[7]}"] * 5  # duplicates
+    path.write_text("\n".join(lines) + "\n")
+
+
+@pytest.mark.parametrize("mode", ["u", "d"])
+def test_graph_sort_tool(tmp_path, mode):
+    raw, out = tmp_path / "raw.txt", tmp_path / "out.txt"
+    _edge_text(raw, np.random.default_rng(1), header=2)
+    p = subprocess.run([os.path.join(BIN, "graph_sort"), str(raw), str(out), mode, "2"],
+                       capture_output=True, text=True, timeout=60)
+    assert p.returncode == 0, p.stderr
+    assert "Skip 2 Lines" in p.stdout and "Graph has" in p.stdout
+    if os.path.exists(os.path.join(REF_BIN, "graph_sort")):
+        ref = tmp_path / "ref.txt"
+        r = subprocess.run([os.path.join(REF_BIN, "graph_sort"), str(raw), str(ref), mode, "2"],
+                           capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0
+        assert out.read_bytes() == ref.read_bytes()
+        assert p.stdout.splitlines()[-1] == r.stdout.splitlines()[-1]  # "Graph has ..."
+
+
+@pytest.mark.parametrize("fmt", ["le", "gt"])
+def test_compressor_tool(tmp_path, fmt):
+    raw, srt = tmp_path / "raw.txt", tmp_path / "sorted.txt"
+    _edge_text(raw, np.random.default_rng(2))
+    assert subprocess.run([os.path.join(BIN, "graph_sort"), str(raw), str(srt), "u"],
+                          capture_output=True, timeout=60).returncode == 0
+    prefix = tmp_path / "g"
+    p = subprocess.run([os.path.join(BIN, "compressor"), "--format", fmt, str(srt), str(prefix)],
+                       capture_output=True, text=True, timeout=60)
+    assert p.returncode == 0, p.stderr
+    assert "graph loaded." in p.stdout and "CGR generation completed." in p.stdout
+    if os.path.exists(os.path.join(REF_BIN, f"{fmt}_compressor")):
+        rp = tmp_path / "r"
+        assert subprocess.run([os.path.join(REF_BIN, f"{fmt}_compressor"), str(srt), str(rp)],
+                              capture_output=True, timeout=60).returncode == 0
+        for ext in (".graph", ".offset"):
+            assert (tmp_path / f"g{ext}").read_bytes() == (tmp_path / f"r{ext}").read_bytes()
+
+
+def test_ingest_tools_reject_bad_input(tmp_path):
+    assert subprocess.run([os.path.join(BIN, "graph_sort"), "a", "b"], capture_output=True).returncode == 1
+    unsorted = tmp_path / "uns.txt"
+    unsorted.write_text("3 1\n1 2\n")
+    p = subprocess.run([os.path.join(BIN, "compressor"), str(unsorted), str(tmp_path / "x")],
+                       capture_output=True, text=True)
+    assert p.returncode == 1 and "sorted" in p.stderr
+    p = subprocess.run([os.path.join(BIN, "compressor"), str(unsorted), str(tmp_path / "x"), "head_bit"],
+                       capture_output=True, text=True)
+    assert p.returncode == 1
