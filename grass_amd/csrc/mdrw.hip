@@ -356,23 +356,13 @@ static uint32_t next_pow2(uint32_t x) {
     return p;
 }
 
-// Kernels whose dynamic LDS may exceed the default per-launch limit.
-static void allow_lds() {
-    static bool done = false;
-    if (done) return;
-    done = true;
-    const int big = 160 * 1024;
-    (void)hipFuncSetAttribute((const void*)mdrw_verts_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    (void)hipFuncSetAttribute((const void*)mdrw_walk_kernel<kLE>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    (void)hipFuncSetAttribute((const void*)mdrw_walk_kernel<kGT>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    (void)hipFuncSetAttribute((const void*)mdrw_induced_kernel<kLE, false>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    (void)hipFuncSetAttribute((const void*)mdrw_induced_kernel<kLE, true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    (void)hipFuncSetAttribute((const void*)mdrw_induced_kernel<kGT, false>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    (void)hipFuncSetAttribute((const void*)mdrw_induced_kernel<kGT, true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, big);
+// Raises a kernel's dynamic-LDS limit when a launch needs more than 64 KB.  hipFuncSetAttribute
+// records a failure as the thread's last error, which later hipGetLastError checks would pick up,
+// so the error state is cleared here and the launch itself reports any real problem.
+static void allow_lds(const void* kernel, size_t bytes) {
+    if (bytes <= 65536) return;
+    (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipGetLastError();
 }
 
 static int sync_read(hipStream_t s, void* host, const void* dev, size_t bytes) {
@@ -418,7 +408,6 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
     }
     const DevGraph dg = g->dev();
     const bool le = g->fmt == kLE;
-    allow_lds();
 
     // phase 1: walks and vertex sets in bound layout
     const uint64_t nbI = scan_blocks(I);
@@ -523,6 +512,8 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
                            dg.deg, d_coff, NV, d_cv, d_ck);
     const uint32_t slots = 2 * next_pow2(vmax);
     const size_t tl = (size_t)slots * 4;
+    allow_lds(le ? (const void*)mdrw_induced_kernel<kLE, false> : (const void*)mdrw_induced_kernel<kGT, false>, tl);
+    allow_lds(le ? (const void*)mdrw_induced_kernel<kLE, true> : (const void*)mdrw_induced_kernel<kGT, true>, tl);
     if (le)
         hipLaunchKernelGGL((mdrw_induced_kernel<kLE, false>), dim3(I), dim3(kIndBlock), tl, s, dg, vert_off,
                            o_verts, d_coff, d_cv, d_ck, slots, d_ccnt, nullptr, nullptr, nullptr);

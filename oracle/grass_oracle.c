@@ -268,7 +268,10 @@ static int64_t gt_get(const or_graph *g, const hdr_t *h, uint64_t k, uint64_t *f
     b += hb;
     int64_t val = blk >= h->fp ? (int64_t)h->v + head + 1 : (int64_t)h->v - head - 1;
     for (uint64_t t = 0; t < i; ++t, b += bb) val += (int64_t)readb(g->w, b, (unsigned)bb) + 1;
-    if (fbits) *fbits = hb + i * bb;
+    /* Bits the device reads: it keeps the absolute id at every 32nd in-block position (a 32-bit
+     * checkpoint built at load), so element i costs that checkpoint plus i mod 32 gaps once
+     * i >= 32, and the head plus i gaps before. */
+    if (fbits) *fbits = i >= 32 ? 32 + (i % 32) * bb : hb + i * bb;
     return val;
 }
 
