@@ -5,11 +5,14 @@ compressed R-MAT graph (a/b/c/d = .57/.19/.19/.05, edge factor 16, seed 42, grap
 compressor), inputs resident in HBM before the timed region:
   * random-walk requests: walkers U[1, 4096], length U[10, 100], Philox starts;
   * 2-hop requests (fanout [25, 10], without replacement, per-hop dedup), seed counts
-    lognormal(6, 1.2) clipped to [1, 8192]   [when --khop is on; see DESIGN.md for status].
-`value` is sampled edges/s of the batched path (all requests in persistent launches);
-`plain_*` is the same batch issued one launch per request.  `roofline` is the dominant
-kernel's algorithmic bytes per launch over its HIP-event-timed duration, `cpu_baseline` the
-CPU oracle on a bounded sample of the same requests.
+    lognormal(6, 1.2) clipped to [1, 8192].
+`value` is sampled edges/s of the batched path (each kind's requests in one batched call);
+`plain_*` is the same batch issued one launch per request; `graph_*` the same batch replayed
+from a captured HIP graph.  `roofline` is the dominant kernel's algorithmic bytes per launch over
+its HIP-event-timed duration (`traffic` from the committed rocprofv3 summary of the same
+workload when there is one), `cpu_baseline` the CPU oracle on a bounded sample of the same
+requests.  Also reported: serving saturation QPS and p50/p99 at 70 % of it, the full decode to
+CSR, and a batch of 512 multi-dimensional random walks.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 24] [--fmt gt]
 """
@@ -59,8 +62,7 @@ def serving_bench(g, seed, rng, seconds=1.5, burst=6000):
     """Saturation QPS from a burst replay (all requests due at once), then p50/p99 latency of an
     open-loop Poisson replay at 70 % of it."""
     from grass_amd import serve
-    with serve.Session(g, memory_budget=24 << 30, max_batch=512, max_inflight=4, seed=seed,
-                       max_group_edges=4 << 20) as s:
+    with serve.Session(g, memory_budget=24 << 30, max_batch=512, seed=seed) as s:
         warm = serving_mix(rng, 500, g.n)
         s.replay(warm, np.zeros(len(warm)))
         reqs = serving_mix(rng, burst, g.n)
@@ -86,6 +88,45 @@ def serving_bench(g, seed, rng, seconds=1.5, burst=6000):
     return out
 
 
+def profile_traffic(scale, fmt):
+    """HBM bytes per rw_batch_kernel launch (rocprofv3 FETCH_SIZE + WRITE_SIZE) from the committed
+    summary of this exact workload, or None when there is none."""
+    import re
+    path = os.path.join(REPO, "profiles", f"rw_batch_kernel_rmat{scale}_{fmt}.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", open(path).read(), re.M)
+    return int(m.group(1)) if m else None
+
+
+def mdrw_bench(g, seed, rng, instances, sptr, target=16384):
+    """Batched multi-dimensional random walks: `instances` frontiers of U[500, 4000] vertices,
+    16384-vertex samples, degree-proportional frontier moves, induced edge lists (one
+    grass_mdrw call: walks, vertex sets, induced edges; outputs stay in HBM)."""
+    import grass_amd
+    C = grass_amd.C
+    L = grass_amd.lib()
+    sizes = rng.integers(500, 4001, size=instances)
+    arr = (grass_amd.MdrwRequest * instances)(*[grass_amd.MdrwRequest(i, int(m)) for i, m in enumerate(sizes)])
+
+    def run():
+        res = C.c_void_p()
+        t = time.perf_counter()
+        grass_amd._check(L.grass_mdrw(g.handle, arr, instances, target, seed, C.c_void_p(sptr),
+                                      C.byref(res)), "grass_mdrw")
+        dt = time.perf_counter() - t
+        o = grass_amd.MdrwOutput()
+        L.grass_mdrw_output_get(res, C.byref(o))
+        L.grass_mdrw_free(res)
+        return dt, o
+
+    run()  # warm-up
+    dt, o = run()
+    return {"instances": instances, "frontier": "U[500,4000]", "target": target, "ms": round(dt * 1e3, 2),
+            "steps": o.steps, "sampled_vertices": o.verts, "induced_edges": o.induced,
+            "steps_per_s": o.steps / dt, "induced_edges_per_s": o.induced / dt}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,7 +140,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--decode", action="store_true", help="also time the full decode to CSR")
+    ap.add_argument("--no-decode", action="store_true", help="skip the full decode to CSR")
+    ap.add_argument("--no-graph", action="store_true", help="skip the captured-graph replay")
+    ap.add_argument("--no-mdrw", action="store_true", help="skip the multi-dimensional random walks")
+    ap.add_argument("--mdrw-instances", type=int, default=512)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (p50/p99) run")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 256 MiB device copy (a known-size kernel that scales rocprof "
@@ -107,8 +151,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "design is described in DESIGN.md and not measured here.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): requests shard "
+              "across GPUs that each hold a replica of the graph, which is not measured here.",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -145,7 +190,10 @@ def main():
     outs = [torch.empty((r["walkers"], r["length"] + 1), dtype=torch.int32, device=dev) for r in reqs]
     arr, keep = g.rw_request_array(reqs, outs)
     rw_edges = sum(r["walkers"] * r["length"] for r in reqs)
-    stream = torch.cuda.current_stream()
+    # one explicit stream for every launch and every timing event (a null handle would select
+    # the graph's own stream inside the library)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
     sptr = stream.cuda_stream
     L = grass_amd.lib()
     C = grass_amd.C
@@ -253,7 +301,24 @@ def main():
             "traffic": None, "bytes_per_launch": bytes_launch,
             "bytes_per_edge": round(bytes_launch / rw_edges, 3), "launch_ms": round(kern_ms, 4)}
 
+    traffic = profile_traffic(args.scale, args.fmt)
+    if traffic is not None:
+        roof["traffic"] = traffic
+        roof["traffic_per_edge"] = round(traffic / rw_edges, 3)
+
     extra = {}
+    if not args.no_graph:
+        plan = g.plan(reqs, kseeds, kfans, seed, khop_instances=list(range(10000, 10000 + R2)))
+        _, ev_g = timed(lambda: plan.launch(stream=stream), args.steps, args.warmup, events=True)
+        pres = plan.khop_result()
+        assert pres["total"] == khop_edges and torch.equal(plan.walks[0], ref0), \
+            "graph replay differs from the eager batch"
+        gms = float(np.mean(ev_g))
+        extra["graph"] = {"ms_per_step": round(gms, 4), "value": edges_step / (gms * 1e-3),
+                          "eager_ms_per_step": round(float(np.mean(ev_b)), 4)}
+        log(f"graph replay: {gms:.3f} ms/step ({edges_step / (gms * 1e-3) / 1e9:.3f} G edges/s)")
+        plan.close()
+        del plan
     p99 = None
     if not args.no_serve:
         del outs, ksrc, kdst  # the session arena takes their HBM
@@ -261,7 +326,10 @@ def main():
         extra["serving"] = serving_bench(g, seed, rng)
         p99 = extra["serving"]["p99_ms"]
         log(f"serving: {extra['serving']}")
-    if args.decode:
+    if not args.no_mdrw:
+        extra["mdrw"] = mdrw_bench(g, seed, rng, args.mdrw_instances, sptr)
+        log(f"mdrw: {extra['mdrw']}")
+    if not args.no_decode:
         rowptr = torch.empty(g.n + 1, dtype=torch.int64, device=dev)
         col = torch.empty(g.m, dtype=torch.int32, device=dev)
 
@@ -276,34 +344,46 @@ def main():
                            "algorithmic_GBps": round(dbytes / (dms * 1e-3) / 1e9, 2),
                            "frac": round(dbytes / (dms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         log(f"decode CSR: {dms:.3f} ms  {extra['decode']}")
+        del rowptr, col
+        torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:
         import oracle
         og = oracle.Graph(gb, ob, args.fmt)
         cores = min(16, os.cpu_count() or 1)
-        # bounded sample: a prefix of the same requests, sized to ~cpu-seconds of work
-        probe = reqs[:4]
+        # bounded sample of the same step: a prefix of the walk requests and a prefix of the 2-hop
+        # requests, each sized to half of ~cpu-seconds of oracle work
         t = time.perf_counter()
         pe = 0
-        for r in probe:
+        for r in reqs[:4]:
             og.random_walk(r["walkers"], r["length"], seed, instance=r["instance"], nthreads=cores)
             pe += r["walkers"] * r["length"]
-        rate = pe / (time.perf_counter() - t)
-        budget = rate * args.cpu_seconds
-        sample, se = [], 0
-        for r in reqs:
-            if se >= budget:
-                break
-            sample.append(r)
-            se += r["walkers"] * r["length"]
+        rw_rate = pe / (time.perf_counter() - t)
         t = time.perf_counter()
-        for r in sample:
+        pk = og.khop(kseeds[:4], kfans[:4], seed=seed, instances=list(range(10000, 10004)), nthreads=cores)
+        kh_rate = max(len(pk["dst"]), 1) / (time.perf_counter() - t)
+        half = args.cpu_seconds / 2
+        rs, re_ = [], 0
+        for r in reqs:
+            if re_ >= rw_rate * half:
+                break
+            rs.append(r)
+            re_ += r["walkers"] * r["length"]
+        nk = 1
+        while nk < R2 and sum(len(x) for x in kseeds[:nk]) * 260 < kh_rate * half:
+            nk += 1
+        t = time.perf_counter()
+        for r in rs:
             og.random_walk(r["walkers"], r["length"], seed, instance=r["instance"], nthreads=cores)
+        kres = og.khop(kseeds[:nk], kfans[:nk], seed=seed, instances=list(range(10000, 10000 + nk)),
+                       nthreads=cores) if R2 else {"dst": []}
         dt = time.perf_counter() - t
-        cpu = {"value": se / dt, "unit": "edges/s", "cores": cores, "kind": "port",
-               "sample": f"first {len(sample)} of the {len(reqs)} random-walk requests "
-                         f"({se} sampled edges), oracle C restatement, OpenMP over walkers"}
+        ke = len(kres["dst"])
+        cpu = {"value": (re_ + ke) / dt, "unit": "edges/s", "cores": cores, "kind": "port",
+               "sample": f"first {len(rs)} of the {len(reqs)} random-walk requests ({re_} edges) and "
+                         f"first {nk} of the {R2} 2-hop requests ({ke} edges), the oracle's C "
+                         f"restatement with OpenMP over walkers / requests, {dt:.1f} s"}
         log(f"cpu baseline: {cpu}")
 
     line = {

@@ -239,13 +239,17 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
                                                                  const uint64_t* __restrict__ coff,
                                                                  const uint32_t* __restrict__ cv,
                                                                  const uint32_t* __restrict__ ck,
-                                                                 uint32_t slots, uint64_t* __restrict__ ccnt,
+                                                                 uint32_t slots, uint32_t per_inst,
+                                                                 uint64_t* __restrict__ ccnt,
                                                                  const uint64_t* __restrict__ cpos,
                                                                  uint32_t* __restrict__ isrc,
                                                                  uint32_t* __restrict__ idst) {
     extern __shared__ uint32_t table[];
     const uint32_t mask = slots - 1;
-    const uint64_t v0 = voff[blockIdx.x], v1 = voff[blockIdx.x + 1];
+    // per_inst workgroups share an instance: each holds the instance's whole vertex set and
+    // takes every per_inst-th group of kIndWaves chunks
+    const uint32_t inst = blockIdx.x / per_inst, part = blockIdx.x % per_inst;
+    const uint64_t v0 = voff[inst], v1 = voff[inst + 1];
     for (uint32_t i = threadIdx.x; i < slots; i += kIndBlock) table[i] = 0xFFFFFFFFu;
     __syncthreads();
     for (uint64_t i = v0 + threadIdx.x; i < v1; i += kIndBlock) {
@@ -256,7 +260,7 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
     __syncthreads();
     const int lane = lane_id(), wid = threadIdx.x / kWave;
     const uint64_t c0 = coff[v0], c1 = coff[v1];
-    for (uint64_t c = c0 + wid; c < c1; c += kIndWaves) {
+    for (uint64_t c = c0 + (uint64_t)part * kIndWaves + wid; c < c1; c += (uint64_t)per_inst * kIndWaves) {
         const uint32_t x = verts[cv[c]];
         Hdr h;
         load_hdr<FMT>(g, x, h);
@@ -512,14 +516,19 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
                            dg.deg, d_coff, NV, d_cv, d_ck);
     const uint32_t slots = 2 * next_pow2(vmax);
     const size_t tl = (size_t)slots * 4;
+    // enough workgroups to cover the device twice even for a handful of instances
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const uint32_t per_inst = (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, (2ull * std::max(cus, 1) + I - 1) / I));
     allow_lds(le ? (const void*)mdrw_induced_kernel<kLE, false> : (const void*)mdrw_induced_kernel<kGT, false>, tl);
     allow_lds(le ? (const void*)mdrw_induced_kernel<kLE, true> : (const void*)mdrw_induced_kernel<kGT, true>, tl);
     if (le)
-        hipLaunchKernelGGL((mdrw_induced_kernel<kLE, false>), dim3(I), dim3(kIndBlock), tl, s, dg, vert_off,
-                           o_verts, d_coff, d_cv, d_ck, slots, d_ccnt, nullptr, nullptr, nullptr);
+        hipLaunchKernelGGL((mdrw_induced_kernel<kLE, false>), dim3(I * per_inst), dim3(kIndBlock), tl, s, dg, vert_off,
+                           o_verts, d_coff, d_cv, d_ck, slots, per_inst, d_ccnt, nullptr, nullptr, nullptr);
     else
-        hipLaunchKernelGGL((mdrw_induced_kernel<kGT, false>), dim3(I), dim3(kIndBlock), tl, s, dg, vert_off,
-                           o_verts, d_coff, d_cv, d_ck, slots, d_ccnt, nullptr, nullptr, nullptr);
+        hipLaunchKernelGGL((mdrw_induced_kernel<kGT, false>), dim3(I * per_inst), dim3(kIndBlock), tl, s, dg, vert_off,
+                           o_verts, d_coff, d_cv, d_ck, slots, per_inst, d_ccnt, nullptr, nullptr, nullptr);
     e = exclusive_scan(U64Gen{d_ccnt}, Count{NC, nullptr}, NC, d_cpos, d_part3, d_part3 + nbC + 1, s);
     if (e != hipSuccess) return status_of(e);
     uint64_t NI = 0;
@@ -532,11 +541,11 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
     res->out.ind_src_dev = o_isrc;
     res->out.ind_dst_dev = o_idst;
     if (le)
-        hipLaunchKernelGGL((mdrw_induced_kernel<kLE, true>), dim3(I), dim3(kIndBlock), tl, s, dg, vert_off,
-                           o_verts, d_coff, d_cv, d_ck, slots, nullptr, d_cpos, o_isrc, o_idst);
+        hipLaunchKernelGGL((mdrw_induced_kernel<kLE, true>), dim3(I * per_inst), dim3(kIndBlock), tl, s, dg, vert_off,
+                           o_verts, d_coff, d_cv, d_ck, slots, per_inst, nullptr, d_cpos, o_isrc, o_idst);
     else
-        hipLaunchKernelGGL((mdrw_induced_kernel<kGT, true>), dim3(I), dim3(kIndBlock), tl, s, dg, vert_off,
-                           o_verts, d_coff, d_cv, d_ck, slots, nullptr, d_cpos, o_isrc, o_idst);
+        hipLaunchKernelGGL((mdrw_induced_kernel<kGT, true>), dim3(I * per_inst), dim3(kIndBlock), tl, s, dg, vert_off,
+                           o_verts, d_coff, d_cv, d_ck, slots, per_inst, nullptr, d_cpos, o_isrc, o_idst);
     hipLaunchKernelGGL(mdrw_ind_off_kernel, dim3((I + 256) / 256), dim3(256), 0, s, vert_off, d_coff, d_cpos, I,
                        ind_off);
     e = hipGetLastError();

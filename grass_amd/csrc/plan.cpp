@@ -5,7 +5,7 @@
 // into the same device buffers) pays the ~30 kernel launches of a 2-hop pipeline plus the walk
 // launch every step.  A plan runs the batch once eagerly (which sizes every workspace, so that
 // nothing allocates while capturing), then captures it with the walks and the k-hop pipeline on
-// two forked streams, and instantiates the graph.  Each replay starts with a clear of the k-hop
+// two branches forked from the launch stream and joined back into it, and instantiates the graph.  Each replay starts with a clear of the k-hop
 // dedup table (the captured pipeline uses the table's first generations), so replays are
 // independent of each other and of any eager call.
 #include <hip/hip_runtime.h>
@@ -43,7 +43,8 @@ struct grass_plan {
         ws_kh.release();
     }
 
-    // The batch on s_main (walks) and s_side (k-hop), joined back into s_main.
+    // The batch: walks on s_main, the k-hop pipeline on s_side forked from and joined back into
+    // s_main, so that in the graph the two run side by side.
     int enqueue() {
         if (!rw.empty()) {
             int rc = rw_batch_enqueue(g, rw.data(), (uint32_t)rw.size(), seed, s_main, ws_rw, nullptr);

@@ -141,6 +141,7 @@ struct Slot {
     PinnedBuf stage;  // the group's inputs on their way to HBM
     PinnedBuf offs;   // k-hop offsets read back
     bool busy = false;
+    bool rw_finished = false;  // the group's walks have been completed
     std::vector<Req*> reqs;
     std::vector<Req*> kreqs;
     std::shared_ptr<OutBlock> out;
@@ -226,17 +227,18 @@ struct grass_session {
         if (r->cancelled) reqs.erase(r->ticket);
     }
 
-    bool slot_done(Slot& s) {
-        if (s.rw_pending && hipEventQuery(s.ev_rw) == hipErrorNotReady) return false;
-        if (s.kh_pending && hipEventQuery(s.ev_kh) == hipErrorNotReady) return false;
-        return true;
-    }
+    static bool passed(bool pending, hipEvent_t ev) { return !pending || hipEventQuery(ev) != hipErrorNotReady; }
 
-    void finalize(Slot& s) {
+    // Completes the requests of one kind in slot s (the walks finish without waiting for the
+    // group's k-hop pipeline).
+    void finish_kind(Slot& s, int kind) {
         const uint64_t* off = (const uint64_t*)s.offs.ptr;
         const uint32_t K = (uint32_t)s.kreqs.size();
-        for (Req* r : s.reqs) {
-            if (r->q.kind == GRASS_REQ_KHOP && s.launch_status == GRASS_OK) {
+        for (Req*& slot_req : s.reqs) {
+            Req* r = slot_req;
+            if (!r || r->q.kind != kind) continue;
+            slot_req = nullptr;  // finish() may delete a cancelled request
+            if (kind == GRASS_REQ_KHOP && s.launch_status == GRASS_OK) {
                 r->res.hops = r->q.hops;
                 r->edges = 0;
                 for (uint32_t h = 0; h < r->q.hops && h < s.H; ++h) {
@@ -247,19 +249,35 @@ struct grass_session {
                     r->res.hop_edges[h] = b - a;
                     r->edges += b - a;
                 }
-            } else if (r->q.kind == GRASS_REQ_RW) {
+            } else if (kind == GRASS_REQ_RW) {
                 r->edges = r->q.walkers * r->q.length;
             }
             finish(r, s.launch_status);
         }
-        if (s.in_bytes) arena.release(s.in_off, s.in_bytes);
-        s.in_bytes = 0;
-        notify_memory();
-        s.reqs.clear();
-        s.kreqs.clear();
-        s.out.reset();
-        s.rw_pending = s.kh_pending = false;
-        s.busy = false;
+    }
+
+    // Advances slot s; true when some request completed.
+    bool progress_slot(Slot& s) {
+        bool any = false;
+        if (!s.rw_finished && (s.launch_status != GRASS_OK || passed(s.rw_pending, s.ev_rw))) {
+            finish_kind(s, GRASS_REQ_RW);
+            s.rw_finished = true;
+            any = true;
+        }
+        if (s.rw_finished && (s.launch_status != GRASS_OK || passed(s.kh_pending, s.ev_kh)) &&
+            passed(s.rw_pending, s.ev_rw)) {
+            finish_kind(s, GRASS_REQ_KHOP);
+            if (s.in_bytes) arena.release(s.in_off, s.in_bytes);
+            s.in_bytes = 0;
+            notify_memory();
+            s.reqs.clear();
+            s.kreqs.clear();
+            s.out.reset();
+            s.rw_pending = s.kh_pending = false;
+            s.busy = false;
+            any = true;
+        }
+        return any;
     }
 
     // Forms and launches one group on slot s; false when nothing could be admitted.
@@ -349,6 +367,7 @@ struct grass_session {
             r->res.launch_us = t;
         }
         s.launch_status = enqueue(s, kcap);
+        s.rw_finished = false;
         s.busy = true;
         return true;
     }
@@ -430,12 +449,8 @@ struct grass_session {
             bool progress = false, busy = false;
             for (Slot& s : slots) {
                 if (!s.busy) continue;
-                if (!slot_done(s)) {
-                    busy = true;
-                    continue;
-                }
-                finalize(s);
-                progress = true;
+                if (progress_slot(s)) progress = true;
+                if (s.busy) busy = true;
             }
             if (progress) cv_done.notify_all();
             for (Slot& s : slots) {
@@ -512,8 +527,8 @@ int grass_session_create(grass_graph* g, const grass_session_config* cfg, grass_
     if (cfg) s->cfg = *cfg;
     if (!s->cfg.memory_budget) s->cfg.memory_budget = 4ull << 30;
     if (!s->cfg.max_batch) s->cfg.max_batch = 512;
-    if (!s->cfg.max_inflight) s->cfg.max_inflight = 4;
-    if (!s->cfg.max_group_edges) s->cfg.max_group_edges = 4ull << 20;
+    if (!s->cfg.max_inflight) s->cfg.max_inflight = 2;
+    if (!s->cfg.max_group_edges) s->cfg.max_group_edges = 8ull << 20;
     s->cfg.max_batch = std::min<uint32_t>(s->cfg.max_batch, 1u << 16);
     s->cfg.max_inflight = std::min<uint32_t>(s->cfg.max_inflight, 64);
     int prev = 0;

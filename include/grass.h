@@ -225,7 +225,8 @@ void grass_mdrw_free(grass_mdrw_result *res);
  * No reference counterpart (the reference launches and synchronises every step from the host,
  * LE/src/grass.cu:1299-1900).  A plan records a fixed batch -- random-walk requests and k-hop
  * requests with their device buffers -- into a HIP graph, the walks and the k-hop pipeline on
- * two forked branches, after one eager run that sizes every workspace.  grass_plan_launch
+ * two branches, after one eager run that sizes every workspace.  Pass an explicit stream to
+ * grass_plan_launch to order later work on that stream after the replay.  grass_plan_launch
  * replays it as one submission.  Between launches the contents of the input buffers (walk starts,
  * k-hop seeds) may change; their addresses, the request shapes and the seed may not.  Launches of
  * one plan must not overlap (they share its workspaces); each replay gives the result an eager
@@ -267,9 +268,9 @@ typedef struct {
 typedef struct {
     uint64_t memory_budget;    /* HBM arena for request inputs and outputs (0 = 4 GiB) */
     uint32_t max_batch;        /* requests per launch group (0 = 512) */
-    uint32_t max_inflight;     /* launch groups on the device at once (0 = 4) */
+    uint32_t max_inflight;     /* launch groups on the device at once (0 = 2) */
     uint64_t seed;
-    uint64_t max_group_edges;  /* edge bound that closes a group early (0 = 4M) */
+    uint64_t max_group_edges;  /* edge bound that closes a group early (0 = 8M) */
 } grass_session_config;
 
 typedef struct {

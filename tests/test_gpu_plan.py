@@ -1,5 +1,6 @@
-"""Captured batches (grass_plan_*): every replay of the HIP graph equals the oracle -- also with
-eager work on the same graph between replays, and after the seed buffers are rewritten in place."""
+"""Captured batches (grass_plan_*): every replay of the HIP graph equals the oracle -- read back on
+the launch stream with no device-wide synchronisation, with eager work on the same graph between
+replays, and after the seed buffers are rewritten in place."""
 import numpy as np
 import pytest
 
@@ -51,11 +52,13 @@ def test_plan_replays_equal_oracle(ga, oracle_mod, fmt):
     seeds = [rng.integers(0, g.n, size=int(k)).astype(np.uint32) for k in (1, 30, 250, 900, 7)]
     fans = [[25, 10], [15, 10], [10, 10, 5], [25, 10], [3]]
     inst = [90, 91, 92, 93, 94]
+    stream = torch.cuda.Stream()  # explicit: launches and the torch reads share it
+    torch.cuda.set_stream(stream)
     plan = g.plan(rw, seeds, fans, seed=31, khop_instances=inst)
     for rep in range(3):
-        plan.launch()
-        g.sync()
+        plan.launch()  # on torch's current stream; the reads below are ordered after it
         _check(ga, plan, og, rw, seeds, fans, inst, 31)
+        g.sync()
         # eager work on the same graph between replays
         g.khop_batch([rng.integers(0, g.n, size=50).astype(np.uint32)], [[5, 5]], seed=3)
         g.random_walk(100, 10, seed=3, instance=5)
@@ -66,7 +69,8 @@ def test_plan_replays_equal_oracle(ga, oracle_mod, fmt):
         t.copy_(torch.from_numpy(s.view(np.int32)).to(t.device))
     torch.cuda.synchronize()
     plan.launch()
-    g.sync()
     _check(ga, plan, og, rw, seeds2, fans, inst, 31)
+    g.sync()
     plan.close()
     g.close()
+    torch.cuda.set_stream(torch.cuda.default_stream())
