@@ -39,7 +39,8 @@ class _Info(C.Structure):
     _fields_ = [("n", C.c_uint64), ("m", C.c_uint64), ("max_degree", C.c_uint64),
                 ("graph_bytes", C.c_uint64), ("offset_bytes", C.c_uint64),
                 ("total_bits", C.c_uint64), ("ubl", C.c_uint32), ("format", C.c_int32),
-                ("device", C.c_int32), ("device_bytes", C.c_uint64)]
+                ("device", C.c_int32), ("device_bytes", C.c_uint64),
+                ("gt_checkpoint", C.c_uint32)]
 
 
 class RwRequest(C.Structure):
@@ -166,6 +167,13 @@ class Graph:
         self.ubl = info.ubl
         self.fmt = info.format
         self.device_bytes = info.device_bytes
+        self.gt_checkpoint = info.gt_checkpoint
+
+    def info(self):
+        """grass_graph_info as a dict."""
+        i = _Info()
+        _check(lib().grass_info(self._h, C.byref(i)), "grass_info")
+        return {k: getattr(i, k) for k, _ in _Info._fields_}
 
     @classmethod
     def open(cls, prefix, fmt="le", device=0):

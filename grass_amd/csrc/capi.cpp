@@ -1,5 +1,6 @@
 // C ABI entry points of libgrass_amd.so (declared in include/grass.h).
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -91,6 +92,12 @@ int open_sources(ByteSource& gs, ByteSource& os, int format, int device, grass_g
     if (!g) return GRASS_ENOMEM;
     g->device = device;
     g->fmt = format;
+    // GT checkpoint interval: GRASS_GT_CHECKPOINT = a power of two in [2, 1024] overrides the default
+    if (const char* e = getenv("GRASS_GT_CHECKPOINT")) {
+        const unsigned long p = strtoul(e, nullptr, 10);
+        for (uint32_t sh = 1; sh <= 10; ++sh)
+            if (p == (1ul << sh)) g->cp_shift = sh;
+    }
     int rc = GRASS_OK;
     if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) rc = GRASS_EDEVICE;
     if (!rc && hipMalloc(&g->d_err, 64) != hipSuccess) rc = GRASS_ENOMEM;
@@ -167,6 +174,7 @@ int grass_info(const grass_graph* g, grass_graph_info* info) {
     info->format = g->fmt;
     info->device = g->device;
     info->device_bytes = g->device_bytes;
+    info->gt_checkpoint = g->fmt == GRASS_FMT_GT ? (1u << g->cp_shift) : 0;
     return GRASS_OK;
 }
 

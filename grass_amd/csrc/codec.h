@@ -30,9 +30,10 @@ struct DevGraph {
     const uint64_t* __restrict__ start;  // bit start of record v, n + 1 entries
     const uint32_t* __restrict__ deg;    // degree of v, decoded and validated at load
     const uint4* __restrict__ hdr;       // decoded header of v (see pack_hdr), built at load
-    const uint32_t* __restrict__ cp;     // GT block checkpoints (see kGtCheck), built at load
+    const uint32_t* __restrict__ cp;     // GT block checkpoints, built at load
     uint32_t n;
     int fmt;
+    uint32_t cp_shift;  // GT checkpoint interval = 1 << cp_shift in-block positions
 };
 
 // Decoded record header.  f1/f2 = rock_bit/sea_bit (LE) or head_bit/body_bit (GT).
@@ -45,11 +46,11 @@ struct Hdr {
     uint64_t cpb;      // GT: index of this vertex's first checkpoint
 };
 
-// GT checkpoints: for every block, the absolute id at in-block positions 32, 64, ... (K =
-// floor(q / 32) slots per block, K * C per vertex).  A GT access at in-block position i then
-// reads one checkpoint and at most 31 gaps instead of i gaps (the reference's O(sqrt d) walk,
-// GT/src/bcg.cuh:154-168).  Only vertices with q >= 32 (d >= 1024) have any.
-constexpr uint32_t kGtCheck = 32;
+// GT checkpoints: for every block, the absolute id at in-block positions P, 2P, ... with
+// P = 1 << cp_shift (K = floor(q / P) slots per block, K * C per vertex).  A GT access at in-block
+// position i then reads one checkpoint and at most P - 1 gaps instead of i gaps (the reference's
+// O(sqrt d) walk, GT/src/bcg.cuh:154-168).  Only vertices with q >= P have any.
+constexpr uint32_t kGtCheckShiftDefault = 2;
 
 // Decoded-header cache: one 16-byte entry per vertex, so that a sampling step reads one aligned
 // dwordx4 instead of walking four dependent gamma codes through the record.
@@ -327,21 +328,22 @@ __device__ __forceinline__ uint32_t gt_get(const uint32_t* __restrict__ w, const
 
 
 // GT random access through the block checkpoints: neighbour k of v is the checkpoint at or
-// before its in-block position (or the block head) plus at most kGtCheck - 1 gaps.  fbits
-// receives the bits read: 32 for a checkpoint (or head_bit for a head) plus the gap fields.
+// before its in-block position (or the block head) plus at most P - 1 gaps.  fbits receives the
+// bits read: 32 for a checkpoint (or head_bit for a head) plus the gap fields.
 __device__ __forceinline__ uint32_t gt_get_cp(const DevGraph& g, const Hdr& h, uint32_t v, uint32_t k,
                                               uint32_t* fbits = nullptr) {
     uint32_t blk, i;
     gt_locate(h, k, blk, i);
     const uint64_t bo = gt_block_bit(h, blk);
-    const uint32_t j = i / kGtCheck;  // 0 when the vertex has no checkpoints (q < kGtCheck)
+    const uint32_t sh = g.cp_shift;
+    const uint32_t j = i >> sh;  // 0 when the vertex has no checkpoints (q < P)
     uint32_t val, rem;
     uint64_t gaps;
     if (j) {
-        const uint32_t K = h.q / kGtCheck;
+        const uint32_t K = h.q >> sh;
         val = g.cp[h.cpb + (uint64_t)blk * K + (j - 1)];
-        rem = i - j * kGtCheck;
-        gaps = bo + h.f1 + (uint64_t)(j * kGtCheck) * h.f2;
+        rem = i - (j << sh);
+        gaps = bo + h.f1 + (uint64_t)(j << sh) * h.f2;
     } else {
         val = gt_head(g.w, h, v, blk, bo);
         rem = i;

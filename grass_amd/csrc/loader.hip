@@ -111,15 +111,16 @@ __global__ __launch_bounds__(kValBlock) void validate_kernel(const uint32_t* __r
 
 // ------------------------------------------------------------------ GT block checkpoints
 
-// Checkpoint slots of a GT vertex: K = floor(q / kGtCheck) per block, C blocks.
+// Checkpoint slots of a GT vertex: K = floor(q / P) per block, C blocks (P = 1 << shift).
 struct CpCountGen {
     const uint4* hdr;
+    uint32_t shift;
     __device__ __forceinline__ uint64_t operator()(uint64_t v) const {
         const uint4 e = hdr[2 * v];
         const uint32_t d = e.z, C = e.w & 0xFFFFu;
         if (d == 0 || C == 0) return 0;
         const uint32_t q = d / C;
-        return (uint64_t)C * (q / kGtCheck);
+        return (uint64_t)C * (q >> shift);
     }
 };
 
@@ -133,15 +134,16 @@ __global__ void cp_base_kernel(uint4* __restrict__ hdr, const uint64_t* __restri
 
 // One wave per vertex with checkpoints: the wave decodes the whole list 64 elements at a time
 // (block heads absolute, bodies as gaps, a segmented prefix sum with the running id carried
-// between passes) and stores the id at every in-block position that is a multiple of kGtCheck.
+// between passes) and stores the id at every in-block position that is a multiple of P.
 __global__ __launch_bounds__(256) void gt_checkpoint_kernel(DevGraph g, uint32_t* __restrict__ cp) {
     const int lane = lane_id();
     const uint64_t nw = (uint64_t)gridDim.x * 256 / kWave;
     for (uint64_t v = ((uint64_t)blockIdx.x * 256 + threadIdx.x) / kWave; v < g.n; v += nw) {
         Hdr h;
         load_hdr<kGT>(g, (uint32_t)v, h);  // same vertex in every lane: uniform branches
-        if (h.d == 0 || h.q < kGtCheck) continue;
-        const uint32_t K = h.q / kGtCheck;
+        const uint32_t sh = g.cp_shift, P = 1u << sh;
+        if (h.d == 0 || h.q < P) continue;
+        const uint32_t K = h.q >> sh;
         uint32_t carry = 0;
         for (uint32_t base = 0; base < h.d; base += kWave) {
             const uint32_t k = base + lane;
@@ -164,8 +166,7 @@ __global__ __launch_bounds__(256) void gt_checkpoint_kernel(DevGraph g, uint32_t
             }
             x = wave_segmented_sum(x, head);
             carry = __shfl(x, kWave - 1, kWave);
-            if (act && i >= kGtCheck && i % kGtCheck == 0)
-                cp[h.cpb + (uint64_t)blk * K + i / kGtCheck - 1] = x;
+            if (act && i >= P && (i & (P - 1)) == 0) cp[h.cpb + (uint64_t)blk * K + (i >> sh) - 1] = x;
         }
     }
 }
@@ -337,7 +338,7 @@ int build_checkpoints(grass_graph* g, uint64_t* count) {
     const uint64_t n = g->n, nb = scan_blocks(n);
     uint64_t* d_base = nullptr;  // n + 1 bases, partials, total
     if (hipMalloc(&d_base, (n + 1 + nb + 2) * 8) != hipSuccess) return GRASS_ENOMEM;
-    hipError_t e = exclusive_scan(CpCountGen{g->d_hdr}, Count{n, nullptr}, n, d_base, d_base + n + 1,
+    hipError_t e = exclusive_scan(CpCountGen{g->d_hdr, g->cp_shift}, Count{n, nullptr}, n, d_base, d_base + n + 1,
                                   d_base + n + 1 + nb, s);
     const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 16384);
     if (e == hipSuccess) {

@@ -119,6 +119,7 @@ struct grass_graph {
     uint32_t* d_deg = nullptr;
     uint4* d_hdr = nullptr;  // decoded-header cache, 16 B (LE) or 32 B (GT) per vertex
     uint32_t* d_cp = nullptr;  // GT block checkpoints
+    uint32_t cp_shift = grass::kGtCheckShiftDefault;  // GT checkpoint interval 1 << cp_shift
     uint32_t* d_err = nullptr;  // [0] first error kind, [1] detail
     hipStream_t stream = nullptr;
     std::mutex mu;
@@ -134,6 +135,7 @@ struct grass_graph {
         d.cp = d_cp;
         d.n = (uint32_t)n;
         d.fmt = fmt;
+        d.cp_shift = cp_shift;
         return d;
     }
 };

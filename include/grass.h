@@ -48,6 +48,9 @@ typedef struct {
     int32_t format;        /* grass_format */
     int32_t device;
     uint64_t device_bytes; /* HBM held by the resident graph */
+    uint32_t gt_checkpoint; /* GT: in-block interval of the checkpoint index built at load (the
+                               environment variable GRASS_GT_CHECKPOINT, a power of two in
+                               [2, 1024], overrides the default 4); 0 for LE */
 } grass_graph_info;
 
 /* Text of a status code. */

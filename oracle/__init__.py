@@ -30,6 +30,7 @@ def lib():
             "or_parse_offsets": (I32, [P, U64, P, P, P]),
             "or_open": (I32, [P, U64, P, U64, I32, P]),
             "or_close": (None, [P]),
+            "or_set_gt_checkpoint": (None, [P, U32]),
             "or_num_vertices": (U64, [P]),
             "or_total_bits": (U64, [P]),
             "or_record_start": (U64, [P, U64]),
@@ -113,6 +114,10 @@ class Graph:
         with open(prefix + ".offset", "rb") as f:
             ob = f.read()
         return cls(gb, ob, fmt)
+
+    def set_gt_checkpoint(self, interval):
+        """Byte model only: count GT reads through a checkpoint every `interval` positions."""
+        lib().or_set_gt_checkpoint(self._h, interval)
 
     def close(self):
         if self._h:

@@ -28,6 +28,7 @@ struct or_graph {
     uint64_t *start;    /* n + 1 record bit starts */
     uint64_t n;
     int fmt;
+    uint32_t cp_interval; /* GT byte model: the device's checkpoint interval (0 = none) */
 };
 
 /* ---------------------------------------------------------------- bit stream primitives */
@@ -126,6 +127,7 @@ int or_open(const uint8_t *graph, uint64_t graph_bytes, const uint8_t *offset,
     if (!g) { free(lens); return OR_ENOMEM; }
     g->n = n;
     g->fmt = fmt;
+    g->cp_interval = 4;
     g->nbits = graph_bytes * 8;
     g->nwords = (graph_bytes + 3) / 4;
     g->w = (uint32_t *)malloc((g->nwords + 4) * sizeof(uint32_t));
@@ -144,6 +146,10 @@ int or_open(const uint8_t *graph, uint64_t graph_bytes, const uint8_t *offset,
     if (acc > g->nbits || g->nbits - acc >= 8) { or_close(g); return OR_EFORMAT; }
     *out = g;
     return OR_OK;
+}
+
+void or_set_gt_checkpoint(or_graph *g, uint32_t interval) {
+    if (g) g->cp_interval = interval;
 }
 
 void or_close(or_graph *g) {
@@ -268,10 +274,11 @@ static int64_t gt_get(const or_graph *g, const hdr_t *h, uint64_t k, uint64_t *f
     b += hb;
     int64_t val = blk >= h->fp ? (int64_t)h->v + head + 1 : (int64_t)h->v - head - 1;
     for (uint64_t t = 0; t < i; ++t, b += bb) val += (int64_t)readb(g->w, b, (unsigned)bb) + 1;
-    /* Bits the device reads: it keeps the absolute id at every 32nd in-block position (a 32-bit
-     * checkpoint built at load), so element i costs that checkpoint plus i mod 32 gaps once
-     * i >= 32, and the head plus i gaps before. */
-    if (fbits) *fbits = i >= 32 ? 32 + (i % 32) * bb : hb + i * bb;
+    /* Bits the device reads: it keeps the absolute id at every P-th in-block position (a 32-bit
+     * checkpoint built at load, P = cp_interval), so element i costs that checkpoint plus i mod P
+     * gaps once i >= P, and the head plus i gaps before. */
+    const uint64_t P = g->cp_interval;
+    if (fbits) *fbits = (P && i >= P) ? 32 + (i % P) * bb : hb + i * bb;
     return val;
 }
 

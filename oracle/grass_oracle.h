@@ -66,6 +66,10 @@ int or_parse_offsets(const uint8_t *off, uint64_t nbytes, uint32_t *ubl, uint64_
 int or_open(const uint8_t *graph, uint64_t graph_bytes, const uint8_t *offset,
             uint64_t offset_bytes, int fmt, or_graph **out);
 void or_close(or_graph *g);
+/* Byte model only (decoded values never change): GT neighbour reads are counted as the device
+ * performs them, through a checkpoint every `interval` in-block positions (default 4; 0 counts
+ * the head plus every gap). */
+void or_set_gt_checkpoint(or_graph *g, uint32_t interval);
 uint64_t or_num_vertices(const or_graph *g);
 uint64_t or_total_bits(const or_graph *g);
 uint64_t or_record_start(const or_graph *g, uint64_t v);

@@ -137,6 +137,7 @@ def test_walk_bytes_match_oracle(ga, oracle_mod, case, fmt):
     """The algorithmic byte count the roofline uses is the oracle's count of the same walks."""
     g, c = _open(ga, case, fmt)
     og = oracle_mod.Graph(c[f"{fmt}_graph"].tobytes(), c[f"{fmt}_offset"].tobytes(), fmt)
+    og.set_gt_checkpoint(g.info()["gt_checkpoint"])
     _, exp = og.random_walk(3000, 25, seed=5, instance=1)
     assert g.random_walk_bytes(3000, 25, seed=5, instance=1) == exp
     starts = np.random.default_rng(1).integers(0, g.n, size=500).astype(np.uint32)
