@@ -328,17 +328,28 @@ int khop_bound(const grass_graph* g, const grass_khop_request* reqs, uint32_t co
     return GRASS_OK;
 }
 
+// Persistent grid of one sample-kernel instantiation (computed once per instantiation).
+template <int FMT, int G>
+static unsigned sample_grid() {
+    static unsigned grid = 0;
+    if (!grid) grid = persistent_grid((const void*)sample_kernel<FMT, G>, kKhBlock);
+    return grid;
+}
+
+// Hop h with G lanes per frontier vertex: the smallest of 8, 16, 32, 64 that covers the hop's
+// largest fanout, so that a fanout-10 hop does not leave two thirds of a 32-lane group idle.
 template <int FMT>
-static void launch_sample(grass_graph* g, bool wide, unsigned grid, hipStream_t s, const uint32_t* fv,
-                          const uint32_t* fr, const uint64_t* ic, const KhReq* reqs, uint32_t h,
-                          uint64_t seed, const uint64_t* eoff, const uint64_t* hb, uint32_t* osrc,
-                          uint32_t* odst, uint64_t cap) {
-    if (wide)
-        hipLaunchKernelGGL((sample_kernel<FMT, 64>), dim3(grid), dim3(kKhBlock), 0, s, g->dev(), fv, fr,
-                           ic, reqs, h, seed, eoff, hb, osrc, odst, cap, g->d_err);
-    else
-        hipLaunchKernelGGL((sample_kernel<FMT, 32>), dim3(grid), dim3(kKhBlock), 0, s, g->dev(), fv, fr,
-                           ic, reqs, h, seed, eoff, hb, osrc, odst, cap, g->d_err);
+static void launch_sample(grass_graph* g, uint32_t G, hipStream_t s, const uint32_t* fv, const uint32_t* fr,
+                          const uint64_t* ic, const KhReq* reqs, uint32_t h, uint64_t seed, const uint64_t* eoff,
+                          const uint64_t* hb, uint32_t* osrc, uint32_t* odst, uint64_t cap) {
+#define GRASS_SAMPLE(GG)                                                                                   \
+    hipLaunchKernelGGL((sample_kernel<FMT, GG>), dim3(sample_grid<FMT, GG>()), dim3(kKhBlock), 0, s, g->dev(), \
+                       fv, fr, ic, reqs, h, seed, eoff, hb, osrc, odst, cap, g->d_err)
+    if (G <= 8) GRASS_SAMPLE(8);
+    else if (G <= 16) GRASS_SAMPLE(16);
+    else if (G <= 32) GRASS_SAMPLE(32);
+    else GRASS_SAMPLE(64);
+#undef GRASS_SAMPLE
 }
 
 static hipError_t ensure_table(Workspace& ws, uint64_t entries, HashTab& t) {
@@ -466,12 +477,15 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
         hipLaunchKernelGGL(front_offsets_kernel, dim3((R + 256) / 256), dim3(256), 0, s, d_sbeg, R, d_fpos,
                            d_foff[0], d_icount[0]);
     }
-    bool wide = false;
-    for (uint32_t r = 0; r < R; ++r)
-        for (uint32_t h = 0; h < reqs[r].hops; ++h) wide = wide || reqs[r].fanout[h] > 32;
-    const void* skern = g->fmt == kLE ? (wide ? (const void*)sample_kernel<kLE, 64> : (const void*)sample_kernel<kLE, 32>)
-                                      : (wide ? (const void*)sample_kernel<kGT, 64> : (const void*)sample_kernel<kGT, 32>);
-    const unsigned gridS = persistent_grid(skern, kKhBlock);
+    uint32_t lanes[GRASS_KHOP_MAX_HOPS];
+    for (uint32_t h = 0; h < H; ++h) {
+        uint32_t f = 0;
+        for (uint32_t r = 0; r < R; ++r)
+            if (h < reqs[r].hops) f = std::max(f, reqs[r].fanout[h]);
+        uint32_t G = 8;
+        while (G < f) G <<= 1;
+        lanes[h] = G;
+    }
     for (uint32_t h = 0; h < H; ++h) {
         const int cur = h & 1, nxt = cur ^ 1;
         CountGen cg{d_fv[cur], d_fr[cur], d_req, g->d_deg, h};
@@ -480,10 +494,10 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
         hipLaunchKernelGGL(hop_offsets_kernel, dim3((R + 256) / 256), dim3(256), 0, s, d_foff[cur], R, d_eoff,
                            d_eoff_r, d_hb, h, out->offsets_dev);
         if (g->fmt == kLE)
-            launch_sample<kLE>(g, wide, gridS, s, d_fv[cur], d_fr[cur], d_icount[cur], d_req, h, seed, d_eoff,
+            launch_sample<kLE>(g, lanes[h], s, d_fv[cur], d_fr[cur], d_icount[cur], d_req, h, seed, d_eoff,
                                d_hb, out->src_dev, out->dst_dev, out->capacity);
         else
-            launch_sample<kGT>(g, wide, gridS, s, d_fv[cur], d_fr[cur], d_icount[cur], d_req, h, seed, d_eoff,
+            launch_sample<kGT>(g, lanes[h], s, d_fv[cur], d_fr[cur], d_icount[cur], d_req, h, seed, d_eoff,
                                d_hb, out->src_dev, out->dst_dev, out->capacity);
         if (h + 1 == H) break;
         DedupIn in{out->dst_dev, d_hb + h, out->capacity};
