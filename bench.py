@@ -144,6 +144,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="skip the captured-graph replay")
     ap.add_argument("--no-mdrw", action="store_true", help="skip the multi-dimensional random walks")
     ap.add_argument("--mdrw-instances", type=int, default=512)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (p50/p99) run")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 256 MiB device copy (a known-size kernel that scales rocprof "
@@ -184,6 +185,12 @@ def main():
     g = grass_amd.Graph.from_bytes(gb, ob, args.fmt)
     t_load = time.time() - t0
     log(f"load to HBM: {t_load:.2f} s ({(len(gb) + len(ob)) / t_load / 1e9:.2f} GB/s incl. validation)")
+
+    tuning = None
+    if not args.no_tune:
+        t0 = time.time()
+        tuning, trep = g.autotune()
+        log(f"autotune ({time.time() - t0:.2f} s): {tuning}  {trep}")
 
     rng = np.random.default_rng(seed)
     reqs = rw_requests(rng, args.rw_requests)
@@ -402,7 +409,8 @@ def main():
         "plain_value": plain, "batched_over_plain": (value / plain) if plain else None,
         "p99_ms": p99,
         "roofline": roof, "cpu_baseline": cpu,
-        "ingest_s": round(t_ingest, 2), "load_s": round(t_load, 3),
+        "ingest_s": round(t_ingest, 2), "load_s": round(t_load, 3), "tuning": tuning,
+        "gt_checkpoint": g.gt_checkpoint,
     }
     line.update(extra)
     print(json.dumps(line), flush=True)

@@ -73,6 +73,21 @@ class MdrwOutput(C.Structure):
                 ("steps", C.c_uint64), ("verts", C.c_uint64), ("induced", C.c_uint64)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("rw_stage", C.c_uint32), ("rw_claim", C.c_uint32), ("khop_lane_scale", C.c_uint32),
+                ("decode_edges_per_wave", C.c_uint32)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("rw_count", C.c_uint32), ("rw_stage", C.c_uint32 * 9), ("rw_claim", C.c_uint32 * 9),
+                ("rw_ms", C.c_double * 9), ("khop_count", C.c_uint32),
+                ("khop_lane_scale", C.c_uint32 * 2), ("khop_ms", C.c_double * 2),
+                ("decode_count", C.c_uint32), ("decode_edges_per_wave", C.c_uint32 * 5),
+                ("decode_ms", C.c_double * 5)]
+
+
+TUNE_RW, TUNE_KHOP, TUNE_DECODE = 1, 2, 4
+
 MDRW_MAX_FRONTIER = 8192
 MDRW_MAX_TARGET = 16384
 
@@ -111,6 +126,9 @@ def lib():
             "grass_plan_launch": (I32, [P, P]),
             "grass_plan_khop_total": (I32, [P, P]),
             "grass_plan_destroy": (I32, [P]),
+            "grass_get_tuning": (I32, [P, P]),
+            "grass_set_tuning": (I32, [P, P]),
+            "grass_autotune": (I32, [P, U32, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -357,6 +375,27 @@ class Graph:
         return {"step_off": so, "vert_off": vo, "ind_off": io, "step_src": ss[: o.steps],
                 "step_dst": sd[: o.steps], "verts": vv[: o.verts], "ind_src": isrc[: o.induced],
                 "ind_dst": idst[: o.induced]}
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib().grass_get_tuning(self._h, C.byref(t)), "grass_get_tuning")
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+    def set_tuning(self, **kw):
+        cur = self.tuning()
+        cur.update(kw)
+        t = Tuning(**cur)
+        _check(lib().grass_set_tuning(self._h, C.byref(t)), "grass_set_tuning")
+
+    def autotune(self, what=0):
+        """Times every candidate of the selected knobs (TUNE_* mask, 0 = all) and keeps the fastest;
+        returns (chosen tuning, per-candidate timings)."""
+        t, r = Tuning(), TuneReport()
+        _check(lib().grass_autotune(self._h, what, C.byref(t), C.byref(r)), "grass_autotune")
+        rep = {"rw": [(r.rw_stage[i], r.rw_claim[i], r.rw_ms[i]) for i in range(r.rw_count)],
+               "khop": [(r.khop_lane_scale[i], r.khop_ms[i]) for i in range(r.khop_count)],
+               "decode": [(r.decode_edges_per_wave[i], r.decode_ms[i]) for i in range(r.decode_count)]}
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}, rep
 
     def plan(self, rw_requests, khop_seeds, khop_fanouts, seed, khop_instances=None):
         """Captures a fixed batch (see Plan); the buffers live as long as the plan."""

@@ -1,8 +1,9 @@
 // Full decode of the resident compressed graph to CSR (rowptr u64[n+1], col u32[m]).
 //
-// Edge-parallel: each wave64 owns a contiguous range of kEdgesPerWave output slots, so writes
-// are coalesced and work is balanced whatever the degree skew.  A lane finds the vertex of its
-// slot by binary search of rowptr inside the wave's vertex range, parses that vertex's header
+// Edge-parallel: each wave64 owns a contiguous range of output slots (a tuning choice, 2048 by
+// default), so writes are coalesced and work is balanced whatever the degree skew.  A lane
+// finds the vertex of its slot by binary search of rowptr inside the wave's vertex range,
+// parses that vertex's header
 // (lanes of one vertex read the same words) and decodes its element:
 //   LE: O(1) random access (LE/src/bcg.cuh:196-261);
 //   GT: the head of each block is absolute and every body is a gap, so the wave turns
@@ -21,7 +22,6 @@
 namespace grass {
 
 constexpr int kDecBlock = 256;
-constexpr uint64_t kEdgesPerWave = 2048;
 
 struct DegGen {
     const uint32_t* deg;
@@ -43,12 +43,12 @@ template <int FMT>
 __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
                                                                const uint64_t* __restrict__ rowptr,
                                                                uint32_t* __restrict__ col,
-                                                               uint64_t m) {
+                                                               uint64_t m, uint32_t per_wave) {
     const uint64_t wave = ((uint64_t)blockIdx.x * kDecBlock + threadIdx.x) / kWave;
     const int lane = lane_id();
-    const uint64_t e0 = wave * kEdgesPerWave;
+    const uint64_t e0 = wave * per_wave;
     if (e0 >= m) return;  // uniform per wave
-    const uint64_t e1 = e0 + kEdgesPerWave < m ? e0 + kEdgesPerWave : m;
+    const uint64_t e1 = e0 + per_wave < m ? e0 + per_wave : m;
     const uint32_t vlo = owner_of(rowptr, 0, g.n - 1, e0);
     const uint32_t vhi = owner_of(rowptr, vlo, g.n - 1, e1 - 1);
     uint64_t eb = e0;
@@ -110,15 +110,16 @@ int decode_csr(grass_graph* g, uint64_t* rowptr, uint32_t* col, hipStream_t s) {
         if (e != hipSuccess) return status_of(e);
     }
     if (m == 0 || !col) return GRASS_OK;
-    const uint64_t waves = (m + kEdgesPerWave - 1) / kEdgesPerWave;
+    const uint32_t per_wave = g->tune.decode_edges_per_wave.load(std::memory_order_relaxed);
+    const uint64_t waves = (m + per_wave - 1) / per_wave;
     const uint64_t blocks = (waves + kDecBlock / kWave - 1) / (kDecBlock / kWave);
     if (blocks > 0x7fffffffull) return GRASS_EINVAL;
     if (g->fmt == kLE)
         hipLaunchKernelGGL(decode_csr_kernel<kLE>, dim3((unsigned)blocks), dim3(kDecBlock), 0, s,
-                           g->dev(), (const uint64_t*)rowptr, col, m);
+                           g->dev(), (const uint64_t*)rowptr, col, m, per_wave);
     else
         hipLaunchKernelGGL(decode_csr_kernel<kGT>, dim3((unsigned)blocks), dim3(kDecBlock), 0, s,
-                           g->dev(), (const uint64_t*)rowptr, col, m);
+                           g->dev(), (const uint64_t*)rowptr, col, m, per_wave);
     return status_of(hipGetLastError());
 }
 

@@ -484,7 +484,7 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
             if (h < reqs[r].hops) f = std::max(f, reqs[r].fanout[h]);
         uint32_t G = 8;
         while (G < f) G <<= 1;
-        lanes[h] = G;
+        lanes[h] = std::min<uint32_t>(64, G * g->tune.khop_lane_scale.load(std::memory_order_relaxed));
     }
     for (uint32_t h = 0; h < H; ++h) {
         const int cur = h & 1, nxt = cur ^ 1;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 
@@ -83,6 +84,14 @@ struct Workspace {
     }
 };
 
+// Launch-shape choices that never change a result (grass_tuning; read at every launch).
+struct TuneState {
+    std::atomic<uint32_t> rw_stage{32};              // walk columns staged per LDS flush: 8, 16, 32
+    std::atomic<uint32_t> rw_claim{1};               // 64-walker granules per queue access: 1, 2, 4
+    std::atomic<uint32_t> khop_lane_scale{1};        // k-hop lanes per vertex = scale x fanout cover
+    std::atomic<uint32_t> decode_edges_per_wave{2048};  // full-decode slots per wave
+};
+
 // Bump allocator over a Scratch region (256-byte aligned slices).
 struct Carve {
     char* base;
@@ -125,6 +134,7 @@ struct grass_graph {
     std::mutex mu;
     grass::Workspace ws;     // working memory of the synchronous entry points (under mu)
     grass::Scratch ws_bytes; // device counter of the byte-counting entry points (under mu)
+    grass::TuneState tune;
 
     grass::DevGraph dev() const {
         grass::DevGraph d;

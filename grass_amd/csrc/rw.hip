@@ -9,15 +9,16 @@
 //
 // The 64 walkers of a wave belong to one request and so have one length: the step loop is
 // wave-uniform, which allows staged output: each lane puts its walk values in an LDS tile of
-// kStage columns and the wave writes whole 64-byte row segments (the walk matrix is row-major,
-// so one column written straight from 64 lanes would touch 64 lines).  GT accesses go through
-// the block checkpoints built at load, so a step reads at most 31 gaps.
+// STAGE columns (8, 16 or 32: a tuning choice) and the wave writes whole row segments of STAGE
+// ids (the walk matrix is row-major, so one column written straight from 64 lanes would touch
+// 64 lines).  GT accesses go through the block checkpoints built at load, so a step reads fewer
+// gaps than the checkpoint interval.
 //
 // plain:   one launch, one request (grass_random_walk);
 // batched: many requests of different walker counts and lengths in one persistent launch;
-//          waves pull 64-walker granules of one request from a device queue, longest walks
-//          first.  Outputs are addressed by (request, walker), so the schedule never changes
-//          a result.
+//          waves pull 64-walker granules of one request from a device queue (`claim` granules
+//          per queue access, a tuning choice), longest walks first.  Outputs are addressed by
+//          (request, walker), so neither the schedule nor the tuning ever changes a result.
 // COUNT variants write nothing and instead sum the algorithmic bytes of the same walks:
 //   per walker 4 (start written) [+ 4 start read], per visited vertex 8 (record start) +
 //   ceil(header bits / 8), per step ceil(payload bits read / 8) + 4 (id written).
@@ -35,8 +36,9 @@ namespace grass {
 
 constexpr int kRwBlock = 256;
 constexpr int kRwWaves = kRwBlock / kWave;
-constexpr uint32_t kStage = 16;             // walk columns staged per flush (64 bytes a row)
-constexpr uint32_t kPitch = kStage + 1;     // LDS row pitch (odd: spreads lanes over banks)
+// LDS row pitch of a STAGE-column tile (odd: spreads the lanes' rows over the banks)
+template <uint32_t STAGE>
+constexpr uint32_t pitch() { return STAGE + 1; }
 
 __device__ __forceinline__ uint64_t bytes_of_bits(uint32_t bits) { return (bits + 7) >> 3; }
 
@@ -46,20 +48,20 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Writes staged columns [c0, c0 + ncols) of the wave's rows: 16 lanes per row, 4 rows per store.
+// Writes staged columns [c0, c0 + ncols) of the wave's rows: STAGE lanes per row, 64 / STAGE rows
+// per store.
+template <uint32_t STAGE>
 __device__ __forceinline__ void flush_tile(const uint32_t* __restrict__ tile, uint32_t* __restrict__ row0,
                                            uint64_t stride, uint32_t nrows, uint32_t c0, uint32_t ncols) {
     wave_sync();
     const int lane = lane_id();
-    const uint32_t col = lane & 15, r4 = lane >> 4;
-    for (uint32_t r = r4; r < (uint32_t)kWave; r += 4)
-        if (r < nrows && col < ncols) row0[r * stride + c0 + col] = tile[r * kPitch + col];
+    const uint32_t col = lane % STAGE, rs = lane / STAGE;
+    for (uint32_t r = rs; r < (uint32_t)kWave; r += kWave / STAGE)
+        if (r < nrows && col < ncols) row0[r * stride + c0 + col] = tile[r * pitch<STAGE>() + col];
     wave_sync();
 }
 
-// Walks the 64 walkers of one wave (lane = walker); all lanes call it with the same L.
-// row0 is the output row of lane 0's walker; rows are consecutive with stride L + 1.
-template <int FMT, bool COUNT>
+template <int FMT, bool COUNT, uint32_t STAGE>
 __device__ __forceinline__ uint64_t walk_wave(const DevGraph& g, bool active, uint32_t v, uint32_t L,
                                               uint64_t seed, uint32_t inst, uint32_t w,
                                               uint32_t* __restrict__ row0, uint32_t nrows,
@@ -73,8 +75,8 @@ __device__ __forceinline__ uint64_t walk_wave(const DevGraph& g, bool active, ui
         live = false;
     }
     if (!COUNT) {
-        tile[lane * kPitch] = live ? v : g.n;
-        if (L == 0) flush_tile(tile, row0, stride, nrows, 0, 1);
+        tile[lane * pitch<STAGE>()] = live ? v : g.n;
+        if (L == 0) flush_tile<STAGE>(tile, row0, stride, nrows, 0, 1);
     }
     for (uint32_t s = 0; s < L; ++s) {
         uint32_t nv = 0;
@@ -101,21 +103,21 @@ __device__ __forceinline__ uint64_t walk_wave(const DevGraph& g, bool active, ui
             }
         }
         if (!COUNT) {
-            const uint32_t col = s + 1, cc = col % kStage;
-            tile[lane * kPitch + cc] = live ? v : g.n;
-            if (cc == kStage - 1 || col == L) flush_tile(tile, row0, stride, nrows, col - cc, cc + 1);
+            const uint32_t col = s + 1, cc = col % STAGE;
+            tile[lane * pitch<STAGE>() + cc] = live ? v : g.n;
+            if (cc == STAGE - 1 || col == L) flush_tile<STAGE>(tile, row0, stride, nrows, col - cc, cc + 1);
         }
     }
     return acc;
 }
 
-template <int FMT, bool COUNT>
+template <int FMT, bool COUNT, uint32_t STAGE>
 __global__ __launch_bounds__(kRwBlock) void rw_kernel(DevGraph g, const uint32_t* __restrict__ starts,
                                                       uint64_t W, uint32_t L, uint64_t seed,
                                                       uint32_t inst, uint32_t* __restrict__ out,
                                                       uint32_t* __restrict__ err,
                                                       unsigned long long* bytes) {
-    __shared__ uint32_t tiles[kRwWaves][kWave * kPitch];
+    __shared__ uint32_t tiles[kRwWaves][kWave * pitch<STAGE>()];
     const uint64_t w = (uint64_t)blockIdx.x * kRwBlock + threadIdx.x;
     const uint64_t w0 = w & ~(uint64_t)(kWave - 1);
     if (w0 >= W) return;  // whole wave past the end
@@ -123,7 +125,7 @@ __global__ __launch_bounds__(kRwBlock) void rw_kernel(DevGraph g, const uint32_t
     uint32_t v = 0;
     if (active) v = starts ? starts[w] : (uint32_t)uniform(seed, inst, (uint32_t)w, kTagStart, 0, g.n);
     const uint32_t nrows = (uint32_t)std::min<uint64_t>(kWave, W - w0);
-    uint64_t acc = walk_wave<FMT, COUNT>(g, active, v, L, seed, inst, (uint32_t)w,
+    uint64_t acc = walk_wave<FMT, COUNT, STAGE>(g, active, v, L, seed, inst, (uint32_t)w,
                                          COUNT ? nullptr : out + w0 * ((uint64_t)L + 1), nrows,
                                          tiles[threadIdx.x / kWave], err);
     if (COUNT) {
@@ -142,39 +144,42 @@ struct RwDesc {
     uint32_t instance;
 };
 
-template <int FMT, bool COUNT>
+template <int FMT, bool COUNT, uint32_t STAGE>
 __global__ __launch_bounds__(kRwBlock) void rw_batch_kernel(DevGraph g, const RwDesc* __restrict__ reqs,
-                                                            uint32_t R, uint64_t ngran,
+                                                            uint32_t R, uint64_t ngran, uint32_t claim,
                                                             unsigned long long* queue, uint64_t seed,
                                                             uint32_t* __restrict__ err,
                                                             unsigned long long* bytes) {
-    __shared__ uint32_t tiles[kRwWaves][kWave * kPitch];
+    __shared__ uint32_t tiles[kRwWaves][kWave * pitch<STAGE>()];
     uint32_t* tile = tiles[threadIdx.x / kWave];
     const int lane = lane_id();
     uint64_t acc = 0;
     for (;;) {
-        unsigned long long gi = 0;
-        if (lane == 0) gi = atomicAdd(queue, 1ull);
-        gi = __shfl(gi, 0, kWave);
-        if (gi >= ngran) break;  // every wave exits once the queue is drained
-        uint32_t lo = 0, hi = R - 1;  // last request with gran_begin <= gi
-        while (lo < hi) {
-            uint32_t mid = lo + (hi - lo + 1) / 2;
-            if (reqs[mid].gran_begin <= gi) lo = mid;
-            else hi = mid - 1;
+        unsigned long long g0 = 0;
+        if (lane == 0) g0 = atomicAdd(queue, (unsigned long long)claim);
+        g0 = __shfl(g0, 0, kWave);
+        if (g0 >= ngran) break;  // every wave exits once the queue is drained
+        const uint64_t g1 = g0 + claim < ngran ? g0 + claim : ngran;
+        for (uint64_t gi = g0; gi < g1; ++gi) {
+            uint32_t lo = 0, hi = R - 1;  // last request with gran_begin <= gi
+            while (lo < hi) {
+                uint32_t mid = lo + (hi - lo + 1) / 2;
+                if (reqs[mid].gran_begin <= gi) lo = mid;
+                else hi = mid - 1;
+            }
+            const RwDesc d = reqs[lo];
+            const uint64_t w0 = (gi - d.gran_begin) * kWave;
+            const uint64_t w = w0 + lane;
+            const bool active = w < d.walkers;
+            uint32_t v = 0;
+            if (active)
+                v = d.starts ? d.starts[w] : (uint32_t)uniform(seed, d.instance, (uint32_t)w, kTagStart, 0, g.n);
+            const uint32_t nrows = (uint32_t)std::min<uint64_t>(kWave, d.walkers - w0);
+            acc += walk_wave<FMT, COUNT, STAGE>(g, active, v, d.length, seed, d.instance, (uint32_t)w,
+                                                COUNT ? nullptr : d.out + w0 * ((uint64_t)d.length + 1), nrows,
+                                                tile, err);
+            if (COUNT && active && d.starts) acc += 4;
         }
-        const RwDesc d = reqs[lo];
-        const uint64_t w0 = (gi - d.gran_begin) * kWave;
-        const uint64_t w = w0 + lane;
-        const bool active = w < d.walkers;
-        uint32_t v = 0;
-        if (active)
-            v = d.starts ? d.starts[w] : (uint32_t)uniform(seed, d.instance, (uint32_t)w, kTagStart, 0, g.n);
-        const uint32_t nrows = (uint32_t)std::min<uint64_t>(kWave, d.walkers - w0);
-        acc += walk_wave<FMT, COUNT>(g, active, v, d.length, seed, d.instance, (uint32_t)w,
-                                     COUNT ? nullptr : d.out + w0 * ((uint64_t)d.length + 1), nrows,
-                                     tile, err);
-        if (COUNT && active && d.starts) acc += 4;
     }
     if (COUNT) {
         acc = wave_sum(acc);
@@ -182,16 +187,30 @@ __global__ __launch_bounds__(kRwBlock) void rw_batch_kernel(DevGraph g, const Rw
     }
 }
 
+template <int FMT, bool COUNT>
+static void launch_rw_fmt(uint32_t stage, dim3 grid, hipStream_t s, const DevGraph& dg, const uint32_t* starts,
+                          uint64_t W, uint32_t L, uint64_t seed, uint32_t inst, uint32_t* out, uint32_t* err,
+                          unsigned long long* bytes) {
+    if (stage <= 8)
+        hipLaunchKernelGGL((rw_kernel<FMT, COUNT, 8>), grid, dim3(kRwBlock), 0, s, dg, starts, W, L, seed, inst,
+                           out, err, bytes);
+    else if (stage <= 16)
+        hipLaunchKernelGGL((rw_kernel<FMT, COUNT, 16>), grid, dim3(kRwBlock), 0, s, dg, starts, W, L, seed, inst,
+                           out, err, bytes);
+    else
+        hipLaunchKernelGGL((rw_kernel<FMT, COUNT, 32>), grid, dim3(kRwBlock), 0, s, dg, starts, W, L, seed, inst,
+                           out, err, bytes);
+}
+
 template <bool COUNT>
 static void launch_rw(grass_graph* g, dim3 grid, hipStream_t s, const uint32_t* starts, uint64_t W,
                       uint32_t L, uint64_t seed, uint32_t inst, uint32_t* out,
                       unsigned long long* bytes) {
+    const uint32_t stage = g->tune.rw_stage.load(std::memory_order_relaxed);
     if (g->fmt == kLE)
-        hipLaunchKernelGGL((rw_kernel<kLE, COUNT>), grid, dim3(kRwBlock), 0, s, g->dev(), starts, W,
-                           L, seed, inst, out, g->d_err, bytes);
+        launch_rw_fmt<kLE, COUNT>(stage, grid, s, g->dev(), starts, W, L, seed, inst, out, g->d_err, bytes);
     else
-        hipLaunchKernelGGL((rw_kernel<kGT, COUNT>), grid, dim3(kRwBlock), 0, s, g->dev(), starts, W,
-                           L, seed, inst, out, g->d_err, bytes);
+        launch_rw_fmt<kGT, COUNT>(stage, grid, s, g->dev(), starts, W, L, seed, inst, out, g->d_err, bytes);
 }
 
 int random_walk(grass_graph* g, const uint32_t* starts, uint64_t W, uint32_t L, uint64_t seed,
@@ -217,13 +236,24 @@ int random_walk(grass_graph* g, const uint32_t* starts, uint64_t W, uint32_t L, 
     return status_of(e);
 }
 
+template <int FMT, bool COUNT, uint32_t STAGE>
+static void launch_rw_batch_stage(grass_graph* g, hipStream_t s, const RwDesc* d, uint32_t R, uint64_t gran,
+                                  uint32_t claim, unsigned long long* queue, uint64_t seed,
+                                  unsigned long long* bytes) {
+    static unsigned grid = 0;
+    if (!grid) grid = persistent_grid((const void*)rw_batch_kernel<FMT, COUNT, STAGE>, kRwBlock);
+    hipLaunchKernelGGL((rw_batch_kernel<FMT, COUNT, STAGE>), dim3(grid), dim3(kRwBlock), 0, s, g->dev(), d, R,
+                       gran, claim, queue, seed, g->d_err, bytes);
+}
+
 template <int FMT, bool COUNT>
 static void launch_rw_batch(grass_graph* g, hipStream_t s, const RwDesc* d, uint32_t R, uint64_t gran,
                             unsigned long long* queue, uint64_t seed, unsigned long long* bytes) {
-    static unsigned grid = 0;
-    if (!grid) grid = persistent_grid((const void*)rw_batch_kernel<FMT, COUNT>, kRwBlock);
-    hipLaunchKernelGGL((rw_batch_kernel<FMT, COUNT>), dim3(grid), dim3(kRwBlock), 0, s, g->dev(), d,
-                       R, gran, queue, seed, g->d_err, bytes);
+    const uint32_t stage = g->tune.rw_stage.load(std::memory_order_relaxed);
+    const uint32_t claim = g->tune.rw_claim.load(std::memory_order_relaxed);
+    if (stage <= 8) launch_rw_batch_stage<FMT, COUNT, 8>(g, s, d, R, gran, claim, queue, seed, bytes);
+    else if (stage <= 16) launch_rw_batch_stage<FMT, COUNT, 16>(g, s, d, R, gran, claim, queue, seed, bytes);
+    else launch_rw_batch_stage<FMT, COUNT, 32>(g, s, d, R, gran, claim, queue, seed, bytes);
 }
 
 int rw_batch_enqueue(grass_graph* g, const grass_rw_request* reqs, uint32_t count, uint64_t seed,

@@ -1,0 +1,224 @@
+// Launch-shape tuning of a resident graph: get/set, and an autotuner that times the candidates
+// of each knob on synthetic requests drawn from the graph and keeps the fastest.
+//
+// Knobs (none changes a result; every output is keyed by request, walker and Philox counters):
+//   rw_stage   LDS tile width of the walk kernels (row-segment size of the walk-matrix stores);
+//   rw_claim   granules a wave takes from the batch queue per atomic (queue granularity);
+//   khop_lane_scale  lanes per frontier vertex of the k-hop sampling kernel (decode width);
+//   decode_edges_per_wave  output slots per wave of the full decode (work per wave).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "philox.h"
+#include "runtime.h"
+
+using namespace grass;
+
+namespace {
+
+bool valid(const grass_tuning& t) {
+    auto in = [](uint32_t x, std::initializer_list<uint32_t> set) {
+        return std::find(set.begin(), set.end(), x) != set.end();
+    };
+    return in(t.rw_stage, {8, 16, 32}) && in(t.rw_claim, {1, 2, 4}) && in(t.khop_lane_scale, {1, 2}) &&
+           in(t.decode_edges_per_wave, {512, 1024, 2048, 4096, 8192});
+}
+
+void apply(grass_graph* g, const grass_tuning& t) {
+    g->tune.rw_stage.store(t.rw_stage);
+    g->tune.rw_claim.store(t.rw_claim);
+    g->tune.khop_lane_scale.store(t.khop_lane_scale);
+    g->tune.decode_edges_per_wave.store(t.decode_edges_per_wave);
+}
+
+grass_tuning current(const grass_graph* g) {
+    grass_tuning t;
+    t.rw_stage = g->tune.rw_stage.load();
+    t.rw_claim = g->tune.rw_claim.load();
+    t.khop_lane_scale = g->tune.khop_lane_scale.load();
+    t.decode_edges_per_wave = g->tune.decode_edges_per_wave.load();
+    return t;
+}
+
+struct DevMem {
+    void* p = nullptr;
+    ~DevMem() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// Best of `reps` timed runs of fn (after one warm-up run), in milliseconds; < 0 on an error.
+template <class F>
+double time_best(hipStream_t s, int reps, F fn) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1;
+    double best = -1;
+    if (fn() == GRASS_OK) {
+        for (int r = 0; r < reps; ++r) {
+            (void)hipEventRecord(a, s);
+            if (fn() != GRASS_OK) {
+                best = -1;
+                break;
+            }
+            (void)hipEventRecord(b, s);
+            (void)hipEventSynchronize(b);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, a, b);
+            if (best < 0 || ms < best) best = ms;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return best;
+}
+
+constexpr uint64_t kTuneSeed = 0x7475u;  // synthetic requests of the autotuner
+
+}  // namespace
+
+extern "C" {
+
+int grass_get_tuning(const grass_graph* g, grass_tuning* out) {
+    if (!g || !out) return GRASS_EINVAL;
+    *out = current(g);
+    return GRASS_OK;
+}
+
+int grass_set_tuning(grass_graph* g, const grass_tuning* t) {
+    if (!g || !t || !valid(*t)) return GRASS_EINVAL;
+    apply(g, *t);
+    return GRASS_OK;
+}
+
+int grass_autotune(grass_graph* g, uint32_t what, grass_tuning* chosen, grass_tune_report* rep) {
+    if (!g) return GRASS_EINVAL;
+    if (what == 0) what = GRASS_TUNE_RW | GRASS_TUNE_KHOP | GRASS_TUNE_DECODE;
+    grass_tune_report local;
+    grass_tune_report& r = rep ? *rep : local;
+    memset(&r, 0, sizeof(r));
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    if (hipSetDevice(g->device) != hipSuccess) return GRASS_EDEVICE;
+    hipStream_t s = g->stream;
+    grass_tuning best = current(g);
+    int rc = GRASS_OK;
+    const uint32_t n = (uint32_t)g->n;
+
+    if (what & GRASS_TUNE_RW) {
+        // 128 requests of 2048 walkers, lengths 20..80, Philox starts
+        std::vector<grass_rw_request> reqs(128);
+        uint64_t words = 0;
+        for (uint32_t i = 0; i < 128; ++i) words += 2048ull * (21 + (i * 37) % 61);
+        DevMem out;
+        if (hipMalloc(&out.p, words * 4) != hipSuccess) rc = GRASS_ENOMEM;
+        uint64_t off = 0;
+        for (uint32_t i = 0; i < 128 && !rc; ++i) {
+            const uint32_t L = 20 + (i * 37) % 61;
+            reqs[i] = grass_rw_request{2048, L, i, nullptr, (uint32_t*)out.p + off};
+            off += 2048ull * (L + 1);
+        }
+        double bms = -1;
+        for (uint32_t st : {8u, 16u, 32u})
+            for (uint32_t cl : {1u, 2u, 4u}) {
+                if (rc) break;
+                grass_tuning t = best;
+                t.rw_stage = st;
+                t.rw_claim = cl;
+                apply(g, t);
+                const double ms = time_best(s, 3, [&] { return random_walk_batch(g, reqs.data(), 128, kTuneSeed, s, nullptr); });
+                if (ms < 0) rc = GRASS_EDEVICE;
+                r.rw_stage[r.rw_count] = st;
+                r.rw_claim[r.rw_count] = cl;
+                r.rw_ms[r.rw_count++] = ms;
+                if (ms >= 0 && (bms < 0 || ms < bms)) {
+                    bms = ms;
+                    best.rw_stage = st;
+                    best.rw_claim = cl;
+                }
+            }
+        apply(g, best);
+    }
+
+    if (!rc && (what & GRASS_TUNE_KHOP)) {
+        // 128 2-hop [25,10] requests of 400 uniform seeds
+        const uint32_t R = 128, S = 400;
+        std::vector<uint32_t> seeds((size_t)R * S);
+        for (uint32_t i = 0; i < R * S; ++i) seeds[i] = (uint32_t)uniform(kTuneSeed, i / S, i % S, kTagStart, 1, n);
+        DevMem d_seeds, d_out;
+        std::vector<grass_khop_request> reqs(R);
+        if (hipMalloc(&d_seeds.p, seeds.size() * 4) != hipSuccess ||
+            hipMemcpy(d_seeds.p, seeds.data(), seeds.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+            rc = GRASS_ENOMEM;
+        for (uint32_t i = 0; i < R && !rc; ++i) {
+            grass_khop_request& q = reqs[i];
+            memset(&q, 0, sizeof(q));
+            q.seeds_dev = (const uint32_t*)d_seeds.p + (size_t)i * S;
+            q.num_seeds = S;
+            q.instance = i;
+            q.hops = 2;
+            q.fanout[0] = 25;
+            q.fanout[1] = 10;
+        }
+        uint64_t cap = 0;
+        if (!rc) rc = khop_bound(g, reqs.data(), R, &cap, nullptr, nullptr, nullptr);
+        if (!rc && hipMalloc(&d_out.p, cap * 8 + 8ull * 2 * (R + 1)) != hipSuccess) rc = GRASS_ENOMEM;
+        double bms = -1;
+        for (uint32_t sc : {1u, 2u}) {
+            if (rc) break;
+            grass_tuning t = best;
+            t.khop_lane_scale = sc;
+            apply(g, t);
+            grass_khop_output o{};
+            o.src_dev = (uint32_t*)d_out.p;
+            o.dst_dev = (uint32_t*)d_out.p + cap;
+            o.capacity = cap;
+            o.offsets_dev = (uint64_t*)((uint32_t*)d_out.p + 2 * cap);
+            const double ms = time_best(s, 3, [&] { return khop_batch(g, reqs.data(), R, kTuneSeed, &o, s); });
+            if (ms < 0) rc = GRASS_EDEVICE;
+            r.khop_lane_scale[r.khop_count] = sc;
+            r.khop_ms[r.khop_count++] = ms;
+            if (ms >= 0 && (bms < 0 || ms < bms)) {
+                bms = ms;
+                best.khop_lane_scale = sc;
+            }
+        }
+        apply(g, best);
+    }
+
+    if (!rc && (what & GRASS_TUNE_DECODE)) {
+        size_t fr = 0, tot = 0;
+        (void)hipMemGetInfo(&fr, &tot);
+        const uint64_t need = (g->n + 1) * 8 + g->m * 4;
+        DevMem rp, col;
+        if (need < fr / 2 && hipMalloc(&rp.p, (g->n + 1) * 8) == hipSuccess &&
+            hipMalloc(&col.p, std::max<uint64_t>(g->m, 1) * 4) == hipSuccess) {
+            double bms = -1;
+            for (uint32_t e : {512u, 1024u, 2048u, 4096u, 8192u}) {
+                if (rc) break;
+                grass_tuning t = best;
+                t.decode_edges_per_wave = e;
+                apply(g, t);
+                const double ms =
+                    time_best(s, 3, [&] { return decode_csr(g, (uint64_t*)rp.p, (uint32_t*)col.p, s); });
+                if (ms < 0) rc = GRASS_EDEVICE;
+                r.decode_edges_per_wave[r.decode_count] = e;
+                r.decode_ms[r.decode_count++] = ms;
+                if (ms >= 0 && (bms < 0 || ms < bms)) {
+                    bms = ms;
+                    best.decode_edges_per_wave = e;
+                }
+            }
+            apply(g, best);
+        }
+        (void)hipGetLastError();  // a failed allocation above only skips this knob
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = GRASS_EDEVICE;
+    if (chosen) *chosen = current(g);
+    (void)hipSetDevice(prev);
+    return rc;
+}
+
+}  // extern "C"

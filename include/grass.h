@@ -224,6 +224,46 @@ int grass_mdrw_fetch(const grass_mdrw_result *res, uint64_t *step_off, uint32_t 
                      uint32_t *ind_src, uint32_t *ind_dst);
 void grass_mdrw_free(grass_mdrw_result *res);
 
+/* ------------------------------------------------------------------ tuning
+ * No reference counterpart (the reference fixes its launch shapes at compile time,
+ * LE/src/grass.cu:49-52,1716-1767, LE/src/utils.hpp:449-461).  Launch-shape choices of a resident
+ * graph; no choice ever changes a result. */
+typedef struct {
+    uint32_t rw_stage;              /* walk columns a wave stages in LDS per flush: 8, 16, 32 */
+    uint32_t rw_claim;              /* 64-walker granules a wave takes per queue access: 1, 2, 4 */
+    uint32_t khop_lane_scale;       /* k-hop lanes per frontier vertex = scale x the smallest of
+                                       8/16/32/64 covering the hop's fanout (capped at 64): 1, 2 */
+    uint32_t decode_edges_per_wave; /* full-decode output slots per wave: 512 ... 8192, power of 2 */
+} grass_tuning;
+
+int grass_get_tuning(const grass_graph *g, grass_tuning *out);
+/* GRASS_EINVAL (and no change) for a value outside the lists above. */
+int grass_set_tuning(grass_graph *g, const grass_tuning *t);
+
+#define GRASS_TUNE_RW 1u
+#define GRASS_TUNE_KHOP 2u
+#define GRASS_TUNE_DECODE 4u
+
+/* Timings of every candidate the autotuner tried (milliseconds, best of 3 runs). */
+typedef struct {
+    uint32_t rw_count;
+    uint32_t rw_stage[9], rw_claim[9];
+    double rw_ms[9];
+    uint32_t khop_count;
+    uint32_t khop_lane_scale[2];
+    double khop_ms[2];
+    uint32_t decode_count;
+    uint32_t decode_edges_per_wave[5];
+    double decode_ms[5];
+} grass_tune_report;
+
+/* Times each candidate of the selected knobs (what = a mask of GRASS_TUNE_*, 0 = all) on
+ * synthetic requests drawn from the graph with Philox -- 128 random-walk requests of 2048
+ * walkers and lengths 20-80; 128 2-hop [25,10] requests of 400 seeds; one full decode (skipped
+ * when its CSR does not fit in free HBM) -- keeps the fastest in the graph and returns it.
+ * Synchronises; uses the graph's own stream.  report may be NULL. */
+int grass_autotune(grass_graph *g, uint32_t what, grass_tuning *chosen, grass_tune_report *report);
+
 /* ------------------------------------------------------------------ captured batches
  * No reference counterpart (the reference launches and synchronises every step from the host,
  * LE/src/grass.cu:1299-1900).  A plan records a fixed batch -- random-walk requests and k-hop
