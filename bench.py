@@ -334,6 +334,34 @@ def main():
         log(f"graph replay: {gms:.3f} ms/step ({edges_step / (gms * 1e-3) / 1e9:.3f} G edges/s)")
         plan.close()
         del plan
+        # a small steady-state step (16 walk requests of <= 256 walkers, 16 2-hop requests of
+        # <= 64 seeds), where launch overhead matters: eager calls against the captured replay
+        sreqs = [dict(r, walkers=min(r["walkers"], 256)) for r in reqs[:16]]
+        sseeds = [k[:64] for k in kseeds[:16]]
+        souts = [torch.empty((r["walkers"], r["length"] + 1), dtype=torch.int32, device=dev) for r in sreqs]
+        sarr, skeep = g.rw_request_array(sreqs, souts)
+        skarr, skkeep = g.khop_request_array(sseeds, kfans[:16], instances=list(range(10000, 10016)))
+        skcap = g.khop_bound(skarr, 16)
+        sks = torch.empty(max(skcap, 1), dtype=torch.int32, device=dev)
+        skd = torch.empty(max(skcap, 1), dtype=torch.int32, device=dev)
+        sko = torch.zeros(2 * 17, dtype=torch.int64, device=dev)
+        skout = grass_amd.KhopOutput(sks.data_ptr(), skd.data_ptr(), skcap, sko.data_ptr(), 0, 0)
+
+        def small_eager():
+            grass_amd._check(L.grass_random_walk_batch(g.handle, sarr, 16, seed, C.c_void_p(sptr)), "small rw")
+            grass_amd._check(L.grass_khop_batch(g.handle, skarr, 16, seed, C.byref(skout), C.c_void_p(sptr)),
+                             "small khop")
+        _, ev_se = timed(small_eager, 20, 5, events=True)
+        splan = g.plan(sreqs, sseeds, kfans[:16], seed, khop_instances=list(range(10000, 10016)))
+        _, ev_sg = timed(lambda: splan.launch(stream=stream), 20, 5, events=True)
+        assert splan.khop_result()["total"] == skout.total
+        splan.close()
+        del splan
+        extra["graph"]["small_step"] = {
+            "rw_requests": 16, "khop_requests": 16,
+            "edges": int(sum(r["walkers"] * r["length"] for r in sreqs) + skout.total),
+            "eager_ms": round(float(np.median(ev_se)), 4), "graph_ms": round(float(np.median(ev_sg)), 4)}
+        log(f"small step: eager {np.median(ev_se):.3f} ms, graph replay {np.median(ev_sg):.3f} ms")
     p99 = None
     if not args.no_serve:
         del outs, ksrc, kdst  # the session arena takes their HBM
