@@ -177,7 +177,12 @@ def main():
         log("calibration: fill 256 MiB, sum over 256 MiB, copy 256 MiB")
 
     t0 = time.time()
-    gb, ob, csr = ingest.rmat_graph(args.scale, 16, fmt=args.fmt, seed=seed, threads=args.threads)
+    src, dst = ingest.rmat_edges(args.scale, 16, seed=seed, threads=args.threads)
+    log(f"R-MAT s{args.scale}: {len(src)} edges in {time.time() - t0:.1f} s")
+    csr = ingest.graph_sort(src, dst, threads=args.threads)
+    del src, dst
+    log(f"graph_sort: n={csr.n} m={csr.m} at {time.time() - t0:.1f} s")
+    gb, ob = csr.compress(args.fmt, args.threads)
     t_ingest = time.time() - t0
     log(f"ingest s{args.scale} {args.fmt}: n={csr.n} m={csr.m} graph={len(gb)/1e6:.1f} MB "
         f"offset={len(ob)/1e6:.1f} MB in {t_ingest:.1f} s")

@@ -87,9 +87,14 @@ class Csr:
 
 
 def _blob_bytes(h):
+    """The blob as bytes (copied through numpy: ctypes.string_at takes a C int size and cannot
+    copy 2 GiB or more)."""
     p, n = C.c_void_p(), C.c_uint64()
     _check(lib().grass_blob_data(h, C.byref(p), C.byref(n)), "grass_blob_data")
-    return C.string_at(p, n.value) if n.value else b""
+    if not n.value:
+        return b""
+    view = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)), (n.value,))
+    return view.tobytes()
 
 
 def graph_sort(src, dst, directed=False, threads=0):
