@@ -88,15 +88,17 @@ def serving_bench(g, seed, rng, seconds=1.5, burst=6000):
     return out
 
 
-def profile_traffic(scale, fmt):
-    """HBM bytes per rw_batch_kernel launch (rocprofv3 FETCH_SIZE + WRITE_SIZE) from the committed
-    summary of this exact workload, or None when there is none."""
+def profile_figures(scale, fmt):
+    """(HBM bytes per rw_batch_kernel launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, TCC hit rate)
+    from the committed summary of this exact workload; (None, None) when there is none."""
     import re
     path = os.path.join(REPO, "profiles", f"rw_batch_kernel_rmat{scale}_{fmt}.md")
     if not os.path.exists(path):
-        return None
-    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", open(path).read(), re.M)
-    return int(m.group(1)) if m else None
+        return None, None
+    text = open(path).read()
+    t = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", text, re.M)
+    h = re.search(r"^tcc_hit_rate:\s*([0-9.]+)", text, re.M)
+    return (int(t.group(1)) if t else None), (float(h.group(1)) if h else None)
 
 
 def mdrw_bench(g, seed, rng, instances, sptr, target=16384):
@@ -320,7 +322,9 @@ def main():
             "traffic": None, "bytes_per_launch": bytes_launch,
             "bytes_per_edge": round(bytes_launch / rw_edges, 3), "launch_ms": round(kern_ms, 4)}
 
-    traffic = profile_traffic(args.scale, args.fmt)
+    traffic, tcc_hit = profile_figures(args.scale, args.fmt)
+    if tcc_hit is not None:
+        roof["tcc_hit_rate"] = tcc_hit
     if traffic is not None:
         roof["traffic"] = traffic
         roof["traffic_per_edge"] = round(traffic / rw_edges, 3)
