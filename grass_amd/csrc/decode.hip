@@ -60,12 +60,36 @@ __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
         eb = e0 - i;
     }
     uint32_t carry = 0;
+    uint32_t cur = vlo;  // owner of the iteration's first slot (a lower bound for every lane's)
     for (uint64_t base = eb; base < e1; base += kWave) {
         const uint64_t e = base + lane;
         const bool act = e < e1;
         const uint64_t ee = act ? e : e1 - 1;
-        const uint32_t v = owner_of(rowptr, vlo, vhi, ee);
-        const uint32_t k = (uint32_t)(ee - rowptr[v]);
+        // Owner of slot ee: one coalesced load of the 64 row ends after `cur`, then a 6-step
+        // search over them in registers; slots more than 64 vertices past `cur` (only possible
+        // across vertices without edges) fall back to a binary search of rowptr.
+        const uint64_t rc = rowptr[cur];
+        const uint64_t bj = (uint64_t)cur + 1 + lane <= g.n ? rowptr[cur + 1 + lane] : ~0ull;
+        uint32_t pos = 0;
+#pragma unroll
+        for (uint32_t st = 32; st >= 1; st >>= 1) {
+            const uint64_t t = __shfl(bj, (int)(pos + st - 1), kWave);
+            if (t <= ee) pos += st;
+        }
+        const uint64_t last = __shfl(bj, kWave - 1, kWave);
+        if (pos == kWave - 1 && last <= ee) pos = kWave;
+        const uint64_t prev = __shfl(bj, (int)(pos > 0 ? pos - 1 : 0), kWave);
+        uint32_t v;
+        uint64_t vstart;
+        if (pos < (uint32_t)kWave) {
+            v = cur + pos;
+            vstart = pos ? prev : rc;
+        } else {
+            v = owner_of(rowptr, cur + kWave, vhi, ee);
+            vstart = rowptr[v];
+        }
+        cur = __shfl(v, kWave - 1, kWave);
+        const uint32_t k = (uint32_t)(ee - vstart);
         Hdr h;
         load_hdr<FMT>(g, v, h);
         if (FMT == kLE) {
