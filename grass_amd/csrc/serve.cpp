@@ -152,7 +152,12 @@ struct Slot {
     int launch_status = GRASS_OK;
 };
 
-uint64_t walk_bytes(const grass_request& q) { return q.walkers * ((uint64_t)q.length + 1) * 4; }
+// Bytes of a walk matrix; saturates instead of wrapping (such a request is rejected anyway).
+uint64_t walk_bytes(const grass_request& q) {
+    const uint64_t cols = (uint64_t)q.length + 1;
+    if (q.walkers > (UINT64_MAX / 8) / cols) return UINT64_MAX / 2;
+    return q.walkers * cols * 4;
+}
 
 // k-hop block of a group: src[cap], dst[cap] (cap = sum of the requests' edge bounds), then the
 // hop-major offsets (at most GRASS_KHOP_MAX_HOPS x (K + 1)).
@@ -259,6 +264,12 @@ struct grass_session {
     // Advances slot s; true when some request completed.
     bool progress_slot(Slot& s) {
         bool any = false;
+        if (s.launch_status != GRASS_OK && !s.rw_finished) {
+            // part of the group may still be queued (the input copy, one kind's launches): let it
+            // drain before the group's memory goes back to the arena
+            (void)hipStreamSynchronize(s.s_rw);
+            (void)hipStreamSynchronize(s.s_kh);
+        }
         if (!s.rw_finished && (s.launch_status != GRASS_OK || passed(s.rw_pending, s.ev_rw))) {
             finish_kind(s, GRASS_REQ_RW);
             s.rw_finished = true;
@@ -472,38 +483,47 @@ struct grass_session {
 
     int submit(const grass_request* q, uint64_t* ticket, bool auto_release, double arrival_us) {
         if (!q || !ticket) return GRASS_EINVAL;
-        std::unique_ptr<Req> r(new (std::nothrow) Req());
-        if (!r) return GRASS_ENOMEM;
-        r->q = *q;
         if (q->kind == GRASS_REQ_RW) {
             if (q->walkers == 0 || q->walkers > 0xFFFFFFFFull || q->length == 0xFFFFFFFFu) return GRASS_EINVAL;
-            if (q->starts) r->data.assign(q->starts, q->starts + q->walkers);
         } else if (q->kind == GRASS_REQ_KHOP) {
             if (q->hops == 0 || q->hops > GRASS_KHOP_MAX_HOPS || (q->num_seeds && !q->seeds)) return GRASS_EINVAL;
             for (uint32_t h = 0; h < q->hops; ++h)
                 if (q->fanout[h] == 0 || q->fanout[h] > 64) return GRASS_EINVAL;
-            r->data.assign(q->seeds, q->seeds + q->num_seeds);
         } else {
             return GRASS_EINVAL;
+        }
+        // HBM the request needs in the arena, checked before anything is copied
+        uint64_t need = 0, kbound = 0;
+        if (q->kind == GRASS_REQ_RW) {
+            need = walk_bytes(*q) + (q->starts ? q->walkers * 4 : 0);
+        } else if (q->num_seeds > arena.size() / 4) {
+            need = UINT64_MAX / 2;  // more seeds than the arena holds
+        } else {
+            kbound = khop_edge_bound(*q);
+            need = q->num_seeds * 4 + block_bytes(kbound, 1);
+        }
+        if (need + 4096 > arena.size()) {
+            std::lock_guard<std::mutex> lk(mu);
+            stats.submitted++;
+            stats.rejected++;
+            return GRASS_ECAPACITY;
+        }
+        std::unique_ptr<Req> r(new (std::nothrow) Req());
+        if (!r) return GRASS_ENOMEM;
+        r->q = *q;
+        r->kbound = kbound;
+        if (q->kind == GRASS_REQ_RW) {
+            if (q->starts) r->data.assign(q->starts, q->starts + q->walkers);
+        } else {
+            r->data.assign(q->seeds, q->seeds + q->num_seeds);
         }
         for (uint32_t x : r->data)
             if (x >= g->n) return GRASS_EINVAL;
         // from here on the request refers to its own copy of the ids
         if (q->kind == GRASS_REQ_RW) r->q.starts = r->data.empty() ? nullptr : r->data.data();
         else r->q.seeds = r->data.data();
-        uint64_t need = r->data.size() * 4;
-        if (q->kind == GRASS_REQ_RW) {
-            need += walk_bytes(r->q);
-        } else {
-            r->kbound = khop_edge_bound(r->q);
-            need += block_bytes(r->kbound, 1);
-        }
         std::lock_guard<std::mutex> lk(mu);
         stats.submitted++;
-        if (need + 4096 > arena.size()) {
-            stats.rejected++;
-            return GRASS_ECAPACITY;
-        }
         r->ticket = next_ticket++;
         r->auto_release = auto_release;
         r->res.submit_us = now_us();

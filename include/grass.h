@@ -165,8 +165,8 @@ int grass_khop_bound(const grass_graph *g, const grass_khop_request *reqs, uint3
                      uint64_t *max_edges);
 
 /* Runs every request of the batch in the same launches (a fixed number per hop whatever the
- * batch size; counts stay on the device).  Each request's edges are identical to running it
- * alone.  Replaces KHopApp (LE/src/apps/khop/khop.cu:1-69: fanout [10,25] with replacement)
+ * batch size; counts stay on the device); count <= 131072.  Each request's edges are identical to
+ * running it alone.  Replaces KHopApp (LE/src/apps/khop/khop.cu:1-69: fanout [10,25] with replacement)
  * driven by doTransitParallelSampling (LE/src/grass.cu:1299-1900).  Synchronises to report
  * `total`. */
 int grass_khop_batch(grass_graph *g, const grass_khop_request *reqs, uint32_t count, uint64_t seed,

@@ -86,6 +86,30 @@ def test_admission_cancel_release(env):
     g.close()
 
 
+def test_oversized_and_invalid_requests(env):
+    """Sizes that would overflow or exceed the arena are rejected before any id is read; malformed
+    requests are EINVAL; the session keeps serving afterwards."""
+    grass_amd, serve = env
+    c = load_case("rmat12")
+    g = grass_amd.Graph.from_bytes(c["le_graph"].tobytes(), c["le_offset"].tobytes(), "le")
+    with serve.Session(g, memory_budget=8 << 20, max_batch=8, max_inflight=2, seed=3) as s:
+        big = serve.rw_request(0xFFFFFFFF, 0xFFFFFFFE, 1)  # walkers x (length + 1) x 4 > 2^64
+        assert s.try_submit(big)[0] == grass_amd.ECAPACITY
+        many = serve.khop_request(np.zeros(4, np.uint32), [10], 2)
+        many.num_seeds = 1 << 40  # far more seeds than the arena holds: never read
+        assert s.try_submit(many)[0] == grass_amd.ECAPACITY
+        bad_fan = serve.khop_request(np.zeros(4, np.uint32), [65], 3)
+        assert s.try_submit(bad_fan)[0] == grass_amd.EINVAL
+        bad_id = serve.khop_request(np.array([g.n], np.uint32), [5], 4)
+        assert s.try_submit(bad_id)[0] == grass_amd.EINVAL
+        assert s.try_submit(serve.rw_request(0, 5, 5))[0] == grass_amd.EINVAL
+        t = s.submit(serve.rw_request(100, 10, 6))
+        assert s.wait(t) == 0
+        s.release(t)
+        assert s.stats()["rejected"] == 2
+    g.close()
+
+
 def test_replay_latencies(env):
     grass_amd, serve = env
     c = load_case("rmat12")
