@@ -436,6 +436,19 @@ def main():
                "sample": f"first {len(rs)} of the {len(reqs)} random-walk requests ({re_} edges) and "
                          f"first {nk} of the {R2} 2-hop requests ({ke} edges), the oracle's C "
                          f"restatement with OpenMP over walkers / requests, {dt:.1f} s"}
+        # one core, on the first walk requests only (a few seconds)
+        t = time.perf_counter()
+        e1 = 0
+        for r in reqs:
+            og.random_walk(r["walkers"], r["length"], seed, instance=r["instance"], nthreads=1)
+            e1 += r["walkers"] * r["length"]
+            if time.perf_counter() - t > 3.0:
+                break
+        cpu["single_core_value"] = e1 / (time.perf_counter() - t)
+        # the full decode to CSR on the same cores (decoded .graph + .offset bytes per second)
+        t = time.perf_counter()
+        og.decode_csr(nthreads=cores)
+        cpu["decode_bytes_per_s"] = (len(gb) + len(ob)) / (time.perf_counter() - t)
         log(f"cpu baseline: {cpu}")
 
     line = {
