@@ -1,10 +1,11 @@
 // Full decode of the resident compressed graph to CSR (rowptr u64[n+1], col u32[m]).
 //
 // Edge-parallel: each wave64 owns a contiguous range of output slots (a tuning choice, 2048 by
-// default), so writes are coalesced and work is balanced whatever the degree skew.  A lane
-// finds the vertex of its slot by binary search of rowptr inside the wave's vertex range,
-// parses that vertex's header
-// (lanes of one vertex read the same words) and decodes its element:
+// default), so writes are coalesced and work is balanced whatever the degree skew.  When 64
+// consecutive slots lie in one list (most slots: the long lists hold most entries) the wave reads
+// that vertex's header once and divides through shared multiply-high reciprocals; otherwise each
+// lane finds its vertex from the 64 row ends after the previous owner (one coalesced load and a
+// register search), parses that vertex's header and decodes its element:
 //   LE: O(1) random access (LE/src/bcg.cuh:196-261);
 //   GT: the head of each block is absolute and every body is a gap, so the wave turns
 //       (head | gap+1) values into ids with a segmented prefix sum, carrying the running id
@@ -39,6 +40,65 @@ __device__ __forceinline__ uint32_t owner_of(const uint64_t* __restrict__ rowptr
     return lo;
 }
 
+// Division by a divisor every lane shares, as a multiply-high and one correction: with
+// m = floor(2^32 / d) the estimate floor(n * m / 2^32) is floor(n / d) or one less, n < 2^32.
+struct UDiv {
+    uint32_t d = 1, m = 0;
+    __device__ __forceinline__ void init(uint32_t dd) {
+        d = dd;
+        m = dd > 1 ? (uint32_t)(0x100000000ull / dd) : 0u;
+    }
+    __device__ __forceinline__ uint32_t div(uint32_t n) const {
+        if (d == 1) return n;
+        uint32_t q = __umulhi(n, m);
+        if (n - q * d >= d) ++q;
+        return q;
+    }
+};
+
+// Element k of vertex v when the whole wave is inside v's list (h uniform): the record
+// geometry's divisions go through the shared UDivs d1 = q + 1 and d0 = q.  LE returns the
+// neighbour; GT returns the block head (head = 1) or the gap + 1 (head = 0).
+template <int FMT>
+__device__ __forceinline__ uint32_t element_uniform(const DevGraph& g, const Hdr& h, uint32_t v, uint32_t k,
+                                                    const UDiv& d1, const UDiv& d0, int& head) {
+    head = 1;
+    const uint32_t big = (h.q + 1) * h.R;
+    uint32_t c, i, len;
+    if (k < big) {
+        c = d1.div(k);
+        i = k - c * (h.q + 1);
+        len = h.q + 1;
+    } else {
+        const uint32_t kk = k - big;
+        const uint32_t c2 = d0.div(kk);
+        i = kk - c2 * h.q;
+        c = c2 + h.R;
+        len = h.q;
+    }
+    if (FMT == kLE) {
+        if (h.d == 1) return h.f1;
+        const uint32_t rb = h.f1;
+        const uint64_t win = peek64(g.w, h.pay + (uint64_t)c * rb);
+        const uint32_t r1 = (uint32_t)(win >> (64 - rb));
+        const uint32_t a = c < h.fp ? v - r1 : v + r1;
+        if (i == 0) return a;
+        const uint32_t r2 = (uint32_t)((win << rb) >> (64 - rb));
+        const uint32_t z = c + 1 < h.fp ? v - r2 : v + r2;
+        const uint32_t j = (h.q - 1) * c + (c < h.R ? c : h.R) + i - 1;
+        const uint64_t so = h.pay + (uint64_t)(h.C + 1) * rb + (uint64_t)j * (h.f2 + 1);
+        const uint32_t x = read_bits(g.w, so, h.f2 + 1);
+        const uint32_t mag = x >> 1;
+        const uint32_t step = (len == h.q + 1 ? d1 : d0).div(z - a);  // (z - a) / len
+        return step * i + a + ((x & 1) ? 0u - mag : mag);
+    } else {
+        const uint64_t bo = gt_block_bit(h, c);
+        if (i == 0) return gt_head(g.w, h, v, c, bo);
+        head = 0;
+        return read_bits(g.w, bo + h.f1 + (uint64_t)(i - 1) * h.f2, h.f2) + 1;
+    }
+}
+
 template <int FMT>
 __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
                                                                const uint64_t* __restrict__ rowptr,
@@ -60,62 +120,78 @@ __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
         eb = e0 - i;
     }
     uint32_t carry = 0;
-    uint32_t cur = vlo;  // owner of the iteration's first slot (a lower bound for every lane's)
+    // owner of the iteration's first slot (a lower bound for every lane's), wave-uniform
+    uint32_t cur = __builtin_amdgcn_readfirstlane(vlo);
+    uint32_t hv = 0xFFFFFFFFu;  // vertex whose header and divisors are cached below
+    Hdr uh;
+    UDiv d1, d0;
     for (uint64_t base = eb; base < e1; base += kWave) {
         const uint64_t e = base + lane;
         const bool act = e < e1;
         const uint64_t ee = act ? e : e1 - 1;
-        // Owner of slot ee: one coalesced load of the 64 row ends after `cur`, then a 6-step
-        // search over them in registers; slots more than 64 vertices past `cur` (only possible
-        // across vertices without edges) fall back to a binary search of rowptr.
         const uint64_t rc = rowptr[cur];
-        const uint64_t bj = (uint64_t)cur + 1 + lane <= g.n ? rowptr[cur + 1 + lane] : ~0ull;
-        uint32_t pos = 0;
-#pragma unroll
-        for (uint32_t st = 32; st >= 1; st >>= 1) {
-            const uint64_t t = __shfl(bj, (int)(pos + st - 1), kWave);
-            if (t <= ee) pos += st;
-        }
-        const uint64_t last = __shfl(bj, kWave - 1, kWave);
-        if (pos == kWave - 1 && last <= ee) pos = kWave;
-        const uint64_t prev = __shfl(bj, (int)(pos > 0 ? pos - 1 : 0), kWave);
-        uint32_t v;
-        uint64_t vstart;
-        if (pos < (uint32_t)kWave) {
-            v = cur + pos;
-            vstart = pos ? prev : rc;
-        } else {
-            v = owner_of(rowptr, cur + kWave, vhi, ee);
-            vstart = rowptr[v];
-        }
-        cur = __shfl(v, kWave - 1, kWave);
-        const uint32_t k = (uint32_t)(ee - vstart);
-        Hdr h;
-        load_hdr<FMT>(g, v, h);
-        if (FMT == kLE) {
-            uint32_t x = le_get(g.w, h, v, k);
-            if (act && e >= e0) col[e] = x;
-        } else {
-            uint32_t blk, i;
-            gt_locate(h, k, blk, i);
-            const uint64_t bo = gt_block_bit(h, blk);
-            uint32_t x;
-            int head;
-            if (i == 0) {
-                x = gt_head(g.w, h, v, blk, bo);
-                head = 1;
-            } else {
-                x = read_bits(g.w, bo + h.f1 + (uint64_t)(i - 1) * h.f2, h.f2) + 1;
-                head = 0;
+        const uint64_t rn = rowptr[cur + 1];
+        uint32_t x;
+        int head = 1;
+        if (base >= rc && base + (kWave - 1) < rn) {
+            // the whole wave is inside one list: one header, shared divisors, no owner search
+            if (cur != hv) {
+                hv = cur;
+                load_hdr<FMT>(g, hv, uh);
+                d1.init(uh.q + 1);
+                d0.init(uh.q);
             }
+            x = element_uniform<FMT>(g, uh, hv, (uint32_t)(ee - rc), d1, d0, head);
+        } else {
+            // Owner of slot ee: one coalesced load of the 64 row ends after `cur`, then a 6-step
+            // search over them in registers; slots more than 64 vertices past `cur` (only
+            // possible across vertices without edges) fall back to a binary search of rowptr.
+            const uint64_t bj = (uint64_t)cur + 1 + lane <= g.n ? rowptr[cur + 1 + lane] : ~0ull;
+            uint32_t pos = 0;
+#pragma unroll
+            for (uint32_t st = 32; st >= 1; st >>= 1) {
+                const uint64_t t = __shfl(bj, (int)(pos + st - 1), kWave);
+                if (t <= ee) pos += st;
+            }
+            const uint64_t last = __shfl(bj, kWave - 1, kWave);
+            if (pos == kWave - 1 && last <= ee) pos = kWave;
+            const uint64_t prev = __shfl(bj, (int)(pos > 0 ? pos - 1 : 0), kWave);
+            uint32_t v;
+            uint64_t vstart;
+            if (pos < (uint32_t)kWave) {
+                v = cur + pos;
+                vstart = pos ? prev : rc;
+            } else {
+                v = owner_of(rowptr, cur + kWave, vhi, ee);
+                vstart = rowptr[v];
+            }
+            cur = __builtin_amdgcn_readfirstlane(__shfl(v, kWave - 1, kWave));
+            const uint32_t k = (uint32_t)(ee - vstart);
+            Hdr h;
+            load_hdr<FMT>(g, v, h);
+            if (FMT == kLE) {
+                x = le_get(g.w, h, v, k);
+            } else {
+                uint32_t blk, i;
+                gt_locate(h, k, blk, i);
+                const uint64_t bo = gt_block_bit(h, blk);
+                if (i == 0) {
+                    x = gt_head(g.w, h, v, blk, bo);
+                } else {
+                    x = read_bits(g.w, bo + h.f1 + (uint64_t)(i - 1) * h.f2, h.f2) + 1;
+                    head = 0;
+                }
+            }
+        }
+        if (FMT == kGT) {
             if (lane == 0 && !head) {
                 x += carry;
                 head = 1;
             }
             x = wave_segmented_sum(x, head);
             carry = __shfl(x, kWave - 1, kWave);
-            if (act && e >= e0) col[e] = x;
         }
+        if (act && e >= e0) col[e] = x;
     }
 }
 
