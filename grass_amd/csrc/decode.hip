@@ -40,65 +40,6 @@ __device__ __forceinline__ uint32_t owner_of(const uint64_t* __restrict__ rowptr
     return lo;
 }
 
-// Division by a divisor every lane shares, as a multiply-high and one correction: with
-// m = floor(2^32 / d) the estimate floor(n * m / 2^32) is floor(n / d) or one less, n < 2^32.
-struct UDiv {
-    uint32_t d = 1, m = 0;
-    __device__ __forceinline__ void init(uint32_t dd) {
-        d = dd;
-        m = dd > 1 ? (uint32_t)(0x100000000ull / dd) : 0u;
-    }
-    __device__ __forceinline__ uint32_t div(uint32_t n) const {
-        if (d == 1) return n;
-        uint32_t q = __umulhi(n, m);
-        if (n - q * d >= d) ++q;
-        return q;
-    }
-};
-
-// Element k of vertex v when the whole wave is inside v's list (h uniform): the record
-// geometry's divisions go through the shared UDivs d1 = q + 1 and d0 = q.  LE returns the
-// neighbour; GT returns the block head (head = 1) or the gap + 1 (head = 0).
-template <int FMT>
-__device__ __forceinline__ uint32_t element_uniform(const DevGraph& g, const Hdr& h, uint32_t v, uint32_t k,
-                                                    const UDiv& d1, const UDiv& d0, int& head) {
-    head = 1;
-    const uint32_t big = (h.q + 1) * h.R;
-    uint32_t c, i, len;
-    if (k < big) {
-        c = d1.div(k);
-        i = k - c * (h.q + 1);
-        len = h.q + 1;
-    } else {
-        const uint32_t kk = k - big;
-        const uint32_t c2 = d0.div(kk);
-        i = kk - c2 * h.q;
-        c = c2 + h.R;
-        len = h.q;
-    }
-    if (FMT == kLE) {
-        if (h.d == 1) return h.f1;
-        const uint32_t rb = h.f1;
-        const uint64_t win = peek64(g.w, h.pay + (uint64_t)c * rb);
-        const uint32_t r1 = (uint32_t)(win >> (64 - rb));
-        const uint32_t a = c < h.fp ? v - r1 : v + r1;
-        if (i == 0) return a;
-        const uint32_t r2 = (uint32_t)((win << rb) >> (64 - rb));
-        const uint32_t z = c + 1 < h.fp ? v - r2 : v + r2;
-        const uint32_t j = (h.q - 1) * c + (c < h.R ? c : h.R) + i - 1;
-        const uint64_t so = h.pay + (uint64_t)(h.C + 1) * rb + (uint64_t)j * (h.f2 + 1);
-        const uint32_t x = read_bits(g.w, so, h.f2 + 1);
-        const uint32_t mag = x >> 1;
-        const uint32_t step = (len == h.q + 1 ? d1 : d0).div(z - a);  // (z - a) / len
-        return step * i + a + ((x & 1) ? 0u - mag : mag);
-    } else {
-        const uint64_t bo = gt_block_bit(h, c);
-        if (i == 0) return gt_head(g.w, h, v, c, bo);
-        head = 0;
-        return read_bits(g.w, bo + h.f1 + (uint64_t)(i - 1) * h.f2, h.f2) + 1;
-    }
-}
-
 template <int FMT>
 __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
                                                                const uint64_t* __restrict__ rowptr,
