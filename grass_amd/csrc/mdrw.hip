@@ -14,12 +14,13 @@
 //    decode and one degree load, all wave-uniform.
 // 2. mdrw_verts_kernel: one workgroup per instance sorts frontier + step destinations in LDS
 //    (bitonic) and keeps the first of each run of equal ids.
-// 3. Induced edges: the sampled vertices' adjacency lists are cut into chunks of kIndChunk
-//    entries; one workgroup per instance holds the instance's vertex set in an LDS hash table,
-//    its waves decode the chunks (LE: O(1) random access per lane; GT: block heads plus gaps
-//    turned into ids with a segmented wave sum) and keep the entries found in the table.  A count
-//    pass and a scan give every chunk its output offset, so the write pass emits the edges in
-//    (vertex, position) order without any sorting.
+// 3. Induced edges, per slice of up to 512 instances: a bit mask per vertex marks the slice's
+//    instances whose sample holds it.  The lists of the slice's distinct vertices are cut into
+//    chunks of kIndChunk entries and each chunk is decoded once by one wave (LE: O(1) random
+//    access per lane; GT: block heads plus gaps turned into ids with a segmented wave sum); entry
+//    y of x's list joins the induced list of every instance in mask[x] & mask[y].  A count pass
+//    and a scan give every (instance, vertex, chunk) its output offset, so the write pass emits
+//    each instance's edges in (vertex, position) order without any sorting.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,8 +35,6 @@
 
 namespace grass {
 
-constexpr int kIndBlock = 1024;
-constexpr int kIndWaves = kIndBlock / kWave;
 constexpr uint32_t kIndChunk = 2048;  // adjacency entries per chunk (32 wave iterations)
 constexpr int kSortBlock = 1024;
 
@@ -217,54 +216,136 @@ struct ChunkGen {  // chunks of vertex gv
     }
 };
 
-// chunk c of vertex gv: cv[c] = gv, ck[c] = chunk index within the vertex
-__global__ __launch_bounds__(256) void mdrw_expand_kernel(const uint32_t* __restrict__ verts,
-                                                          const uint32_t* __restrict__ deg,
-                                                          const uint64_t* __restrict__ coff, uint64_t nv,
-                                                          uint32_t* __restrict__ cv, uint32_t* __restrict__ ck) {
-    const uint64_t gv = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (gv >= nv) return;
-    const uint64_t b = coff[gv], e = coff[gv + 1];
-    for (uint64_t c = b; c < e; ++c) {
-        cv[c] = (uint32_t)gv;
-        ck[c] = (uint32_t)(c - b);
+// Membership masks.  The samples of one slice of up to kSlice instances share one pass over the
+// lists of their distinct vertices: mask[x] holds one bit per instance of the slice whose sample
+// contains x, so an adjacency entry (x, y) belongs to the induced lists of exactly the instances in
+// mask[x] & mask[y].  Every list is decoded once per slice, however many samples hold its vertex.
+constexpr uint32_t kSlice = 512;
+constexpr uint32_t kSliceWords = kSlice / 64;
+constexpr int kIndBlock = 256;
+constexpr int kIndWaves = kIndBlock / kWave;
+
+// Instance of pair p: the largest i in [i0, i1) with voff[i] <= p.
+__device__ __forceinline__ uint32_t inst_of(const uint64_t* __restrict__ voff, uint32_t i0, uint32_t i1,
+                                            uint64_t p) {
+    uint32_t lo = i0, hi = i1 - 1;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo + 1) / 2;
+        if (voff[mid] <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// mask[x * W + l / 64] gets bit l % 64 for every pair (instance i0 + l, vertex x) of the slice.
+__global__ __launch_bounds__(256) void mdrw_mask_kernel(const uint32_t* __restrict__ verts,
+                                                        const uint64_t* __restrict__ voff, uint32_t i0,
+                                                        uint32_t i1, uint32_t W,
+                                                        unsigned long long* __restrict__ mask) {
+    const uint64_t p0 = voff[i0], p1 = voff[i1];
+    for (uint64_t p = p0 + (uint64_t)blockIdx.x * 256 + threadIdx.x; p < p1; p += (uint64_t)gridDim.x * 256) {
+        const uint32_t l = inst_of(voff, i0, i1, p) - i0;
+        atomicOr(&mask[(uint64_t)verts[p] * W + (l >> 6)], 1ull << (l & 63));
     }
 }
 
-__device__ __forceinline__ uint32_t ht_slot(uint32_t v, uint32_t mask) { return (v * 0x9E3779B1u) >> 7 & mask; }
+struct PopGen {  // instances of the slice holding vertex x
+    const unsigned long long* mask;
+    uint32_t W;
+    __device__ __forceinline__ uint64_t operator()(uint64_t x) const {
+        uint64_t c = 0;
+        for (uint32_t w = 0; w < W; ++w) c += __popcll(mask[x * W + w]);
+        return c;
+    }
+};
 
+struct UChunkGen {  // chunks of vertex x when some sample of the slice holds it
+    const unsigned long long* mask;
+    uint32_t W;
+    const uint32_t* deg;
+    __device__ __forceinline__ uint64_t operator()(uint64_t x) const {
+        unsigned long long any = 0;
+        for (uint32_t w = 0; w < W; ++w) any |= mask[x * W + w];
+        return any ? (deg[x] + kIndChunk - 1) / kIndChunk : 0;
+    }
+};
+
+// xpair[pbase[x] + r] = p for pair p = (instance i, vertex x), r = rank of i among the slice's
+// instances holding x: from a vertex's mask bits to its pairs (and so to their output slots).
+__global__ __launch_bounds__(256) void mdrw_xpair_kernel(const uint32_t* __restrict__ verts,
+                                                         const uint64_t* __restrict__ voff, uint32_t i0,
+                                                         uint32_t i1, uint32_t W,
+                                                         const unsigned long long* __restrict__ mask,
+                                                         const uint64_t* __restrict__ pbase,
+                                                         uint32_t* __restrict__ xpair) {
+    const uint64_t p0 = voff[i0], p1 = voff[i1];
+    for (uint64_t p = p0 + (uint64_t)blockIdx.x * 256 + threadIdx.x; p < p1; p += (uint64_t)gridDim.x * 256) {
+        const uint32_t l = inst_of(voff, i0, i1, p) - i0;
+        const uint64_t x = verts[p];
+        uint64_t r = 0;
+        for (uint32_t w = 0; w < (l >> 6); ++w) r += __popcll(mask[x * W + w]);
+        r += __popcll(mask[x * W + (l >> 6)] & ((1ull << (l & 63)) - 1));
+        xpair[pbase[x] + r] = (uint32_t)p;
+    }
+}
+
+// chunk c of the slice's distinct vertices: ux[c] = vertex, uk[c] = chunk index in its list
+__global__ __launch_bounds__(256) void mdrw_uexpand_kernel(const uint64_t* __restrict__ uoff, uint32_t n,
+                                                           uint32_t* __restrict__ ux, uint32_t* __restrict__ uk) {
+    for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (uint64_t)gridDim.x * 256) {
+        const uint64_t b = uoff[x], e = uoff[x + 1];
+        for (uint64_t c = b; c < e; ++c) {
+            ux[c] = (uint32_t)x;
+            uk[c] = (uint32_t)(c - b);
+        }
+    }
+}
+
+// One wave per chunk of a distinct vertex x.  The wave decodes the chunk once; for every entry y
+// and every instance i of the slice in mask[x] & mask[y] it counts (COUNT) or writes (WRITE) the
+// edge (x, y) into instance i's induced list.  Output slot of (instance i, x, chunk k) is
+// coff[pair (i, x)] + k in the instance-major chunk enumeration, so a scan of the counts gives
+// every (i, x, k) its position and the edges land ordered by instance, vertex and list position.
+// Lane l keeps the count (or next output slot) of instance 64 w + l of each mask word w in
+// registers; per matched instance one ballot ranks the wave's hits in list order.
 template <int FMT, bool WRITE>
-__global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, const uint64_t* __restrict__ voff,
-                                                                 const uint32_t* __restrict__ verts,
+__global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, const unsigned long long* __restrict__ mask,
+                                                                 uint32_t W, const uint64_t* __restrict__ nchunks,
+                                                                 const uint32_t* __restrict__ ux,
+                                                                 const uint32_t* __restrict__ uk,
+                                                                 const uint64_t* __restrict__ pbase,
+                                                                 const uint32_t* __restrict__ xpair,
                                                                  const uint64_t* __restrict__ coff,
-                                                                 const uint32_t* __restrict__ cv,
-                                                                 const uint32_t* __restrict__ ck,
-                                                                 uint32_t slots, uint32_t per_inst,
                                                                  uint64_t* __restrict__ ccnt,
                                                                  const uint64_t* __restrict__ cpos,
                                                                  uint32_t* __restrict__ isrc,
                                                                  uint32_t* __restrict__ idst) {
-    extern __shared__ uint32_t table[];
-    const uint32_t mask = slots - 1;
-    // per_inst workgroups share an instance: each holds the instance's whole vertex set and
-    // takes every per_inst-th group of kIndWaves chunks
-    const uint32_t inst = blockIdx.x / per_inst, part = blockIdx.x % per_inst;
-    const uint64_t v0 = voff[inst], v1 = voff[inst + 1];
-    for (uint32_t i = threadIdx.x; i < slots; i += kIndBlock) table[i] = 0xFFFFFFFFu;
-    __syncthreads();
-    for (uint64_t i = v0 + threadIdx.x; i < v1; i += kIndBlock) {
-        const uint32_t v = verts[i];
-        uint32_t s = ht_slot(v, mask);
-        while (atomicCAS(&table[s], 0xFFFFFFFFu, v) != 0xFFFFFFFFu) s = (s + 1) & mask;  // ids unique
-    }
-    __syncthreads();
-    const int lane = lane_id(), wid = threadIdx.x / kWave;
-    const uint64_t c0 = coff[v0], c1 = coff[v1];
-    for (uint64_t c = c0 + (uint64_t)part * kIndWaves + wid; c < c1; c += (uint64_t)per_inst * kIndWaves) {
-        const uint32_t x = verts[cv[c]];
+    const int lane = lane_id();
+    const unsigned long long below = (1ull << lane) - 1;
+    const uint64_t NU = *nchunks;
+    const uint64_t waves = (uint64_t)gridDim.x * kIndWaves;
+    for (uint64_t c = (uint64_t)blockIdx.x * kIndWaves + threadIdx.x / kWave; c < NU; c += waves) {
+        const uint32_t x = __builtin_amdgcn_readfirstlane(ux[c]);
+        const uint32_t kc = __builtin_amdgcn_readfirstlane(uk[c]);
+        const uint64_t pb = pbase[x];
+        unsigned long long mx[kSliceWords];
+        uint64_t acc[kSliceWords];  // lane l: instance 64 w + l's count (COUNT) or next slot (WRITE)
+        uint32_t rbase = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kSliceWords; ++w) {
+            const unsigned long long v = w < W ? mask[(uint64_t)x * W + w] : 0ull;
+            mx[w] = readfirstlane_u64(v);
+            acc[w] = 0;
+            if (WRITE && ((mx[w] >> lane) & 1ull))
+                acc[w] = cpos[coff[xpair[pb + rbase + __popcll(mx[w] & below)]] + kc];
+            rbase += __popcll(mx[w]);
+        }
         Hdr h;
         load_hdr<FMT>(g, x, h);
-        const uint32_t kb = ck[c] * kIndChunk;
+        UDiv d1, d0;
+        d1.init(h.q + 1);
+        d0.init(h.q);
+        const uint32_t kb = kc * kIndChunk;
         const uint32_t ke = min(h.d, kb + kIndChunk);
         uint32_t start = kb;
         if (FMT == kGT && kb) {  // back up to the head of the block holding kb
@@ -272,27 +353,13 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
             gt_locate(h, kb, blk, i);
             start = kb - i;
         }
-        uint64_t out = WRITE ? cpos[c] : 0;
-        uint32_t found = 0, carry = 0;
+        uint32_t carry = 0;
         for (uint32_t base = start; base < ke; base += kWave) {
             const uint32_t k = base + lane;
             const bool act = k < ke;
-            const uint32_t kk = act ? k : ke - 1;
-            uint32_t y;
-            if (FMT == kLE) {
-                y = le_get(g.w, h, x, kk);
-            } else {
-                uint32_t blk, i;
-                gt_locate(h, kk, blk, i);
-                const uint64_t bo = gt_block_bit(h, blk);
-                int head;
-                if (i == 0) {
-                    y = gt_head(g.w, h, x, blk, bo);
-                    head = 1;
-                } else {
-                    y = read_bits(g.w, bo + h.f1 + (uint64_t)(i - 1) * h.f2, h.f2) + 1;
-                    head = 0;
-                }
+            int head;
+            uint32_t y = element_uniform<FMT>(g, h, x, act ? k : ke - 1, d1, d0, head);
+            if (FMT == kGT) {
                 if (lane == 0 && !head) {
                     y += carry;
                     head = 1;
@@ -300,29 +367,38 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
                 y = wave_segmented_sum(y, head);
                 carry = __shfl(y, kWave - 1, kWave);
             }
-            bool in = false;
-            if (act && k >= kb) {
-                uint32_t s = ht_slot(y, mask);
-                for (;;) {
-                    const uint32_t t = table[s];
-                    if (t == y) {
-                        in = true;
-                        break;
+            const bool valid = act && k >= kb && y < g.n;
+#pragma unroll
+            for (uint32_t w = 0; w < kSliceWords; ++w) {
+                if (mx[w] == 0ull) continue;  // uniform
+                const unsigned long long m = valid ? (mx[w] & mask[(uint64_t)y * W + w]) : 0ull;
+                unsigned long long bits = mx[w];
+                while (bits) {  // uniform loop over x's instances in this word
+                    const int b = __builtin_ctzll(bits);
+                    bits &= bits - 1;
+                    const bool hit = (m >> b) & 1ull;
+                    const unsigned long long lanes = __ballot(hit);
+                    if (!lanes) continue;
+                    if (WRITE) {
+                        const uint64_t slot = readlane_u64(acc[w], b);
+                        if (hit) {
+                            const uint64_t pos = slot + __popcll(lanes & below);
+                            isrc[pos] = x;
+                            idst[pos] = y;
+                        }
                     }
-                    if (t == 0xFFFFFFFFu) break;
-                    s = (s + 1) & mask;
+                    if (lane == b) acc[w] += __popcll(lanes);
                 }
             }
-            const unsigned long long mk = __ballot(in);
-            if (WRITE && in) {
-                const uint64_t p = out + __popcll(mk & ((1ull << lane) - 1));
-                isrc[p] = x;
-                idst[p] = y;
-            }
-            out += __popcll(mk);
-            found += __popcll(mk);
         }
-        if (!WRITE && lane == 0) ccnt[c] = found;
+        if (!WRITE) {
+            rbase = 0;
+#pragma unroll
+            for (uint32_t w = 0; w < kSliceWords; ++w) {
+                if ((mx[w] >> lane) & 1ull) ccnt[coff[xpair[pb + rbase + __popcll(mx[w] & below)]] + kc] = acc[w];
+                rbase += __popcll(mx[w]);
+            }
+        }
     }
 }
 
@@ -502,33 +578,80 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
     uint64_t NC = 0;
     rc = sync_read(s, &NC, d_coff + NV, 8);
     if (rc) return rc;
-    const uint64_t nbC = scan_blocks(NC);
+
+    // phase 3: per slice of up to kSlice instances, membership masks over the n vertices; every
+    // distinct vertex of the slice has its list decoded once per pass (count, then write)
+    const uint64_t n = g->n;
+    const uint32_t W = (std::min<uint32_t>(I, kSlice) + 63) / 64;
+    const uint64_t nbN = scan_blocks(n), nbC = scan_blocks(NC);
     DevMem tmp3;
-    if (tmp3.alloc(NC * 8 + (NC + 1) * 16 + (nbC + 2) * 8 + 1024) != hipSuccess) return GRASS_ENOMEM;
+    unsigned long long* d_mask = nullptr;
+    uint64_t *d_pbase = nullptr, *d_uoff = nullptr, *d_partN = nullptr, *d_ccnt = nullptr, *d_cpos = nullptr,
+             *d_part3 = nullptr;
+    uint32_t *d_xpair = nullptr, *d_ux = nullptr, *d_uk = nullptr;
+    auto plan3 = [&](Carve& c) {
+        d_mask = c.take<unsigned long long>(n * W);
+        d_pbase = c.take<uint64_t>(n + 1);
+        d_uoff = c.take<uint64_t>(n + 1);
+        d_partN = c.take<uint64_t>(nbN + 2);
+        d_xpair = c.take<uint32_t>(NV);
+        d_ux = c.take<uint32_t>(NC);
+        d_uk = c.take<uint32_t>(NC);
+        d_ccnt = c.take<uint64_t>(NC + 1);
+        d_cpos = c.take<uint64_t>(NC + 1);
+        d_part3 = c.take<uint64_t>(nbC + 2);
+    };
+    Carve need3(nullptr);
+    plan3(need3);
+    if (tmp3.alloc(need3.off + 256) != hipSuccess) return GRASS_ENOMEM;
     Carve c3(tmp3.p);
-    uint32_t* d_cv = c3.take<uint32_t>(NC);
-    uint32_t* d_ck = c3.take<uint32_t>(NC);
-    uint64_t* d_ccnt = c3.take<uint64_t>(NC + 1);
-    uint64_t* d_cpos = c3.take<uint64_t>(NC + 1);
-    uint64_t* d_part3 = c3.take<uint64_t>(nbC + 2);
-    if (NV)
-        hipLaunchKernelGGL(mdrw_expand_kernel, dim3((unsigned)((NV + 255) / 256)), dim3(256), 0, s, o_verts,
-                           dg.deg, d_coff, NV, d_cv, d_ck);
-    const uint32_t slots = 2 * next_pow2(vmax);
-    const size_t tl = (size_t)slots * 4;
-    // enough workgroups to cover the device twice even for a handful of instances
+    plan3(c3);
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const uint32_t per_inst = (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, (2ull * std::max(cus, 1) + I - 1) / I));
-    allow_lds(le ? (const void*)mdrw_induced_kernel<kLE, false> : (const void*)mdrw_induced_kernel<kGT, false>, tl);
-    allow_lds(le ? (const void*)mdrw_induced_kernel<kLE, true> : (const void*)mdrw_induced_kernel<kGT, true>, tl);
-    if (le)
-        hipLaunchKernelGGL((mdrw_induced_kernel<kLE, false>), dim3(I * per_inst), dim3(kIndBlock), tl, s, dg, vert_off,
-                           o_verts, d_coff, d_cv, d_ck, slots, per_inst, d_ccnt, nullptr, nullptr, nullptr);
-    else
-        hipLaunchKernelGGL((mdrw_induced_kernel<kGT, false>), dim3(I * per_inst), dim3(kIndBlock), tl, s, dg, vert_off,
-                           o_verts, d_coff, d_cv, d_ck, slots, per_inst, d_ccnt, nullptr, nullptr, nullptr);
+    const unsigned sgrid = (unsigned)std::max(cus, 1) * 8;  // grid-stride launches
+    // masks, their scans, pair ranks and chunk list of slice [i0, i1)
+    auto prep = [&](uint32_t i0, uint32_t i1) -> hipError_t {
+        hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(sgrid), dim3(256), 0, s, d_mask, n * W, 0ull);
+        hipLaunchKernelGGL(mdrw_mask_kernel, dim3(sgrid), dim3(256), 0, s, (const uint32_t*)o_verts,
+                           (const uint64_t*)vert_off, i0, i1, W, d_mask);
+        hipError_t e = exclusive_scan(PopGen{d_mask, W}, Count{n, nullptr}, n, d_pbase, d_partN, d_partN + nbN + 1, s);
+        if (e == hipSuccess)
+            e = exclusive_scan(UChunkGen{d_mask, W, dg.deg}, Count{n, nullptr}, n, d_uoff, d_partN,
+                               d_partN + nbN + 1, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(mdrw_xpair_kernel, dim3(sgrid), dim3(256), 0, s, (const uint32_t*)o_verts,
+                           (const uint64_t*)vert_off, i0, i1, W, (const unsigned long long*)d_mask,
+                           (const uint64_t*)d_pbase, d_xpair);
+        hipLaunchKernelGGL(mdrw_uexpand_kernel, dim3(sgrid), dim3(256), 0, s, (const uint64_t*)d_uoff,
+                           (uint32_t)n, d_ux, d_uk);
+        return hipGetLastError();
+    };
+    auto induced = [&](bool write, uint32_t* isrc, uint32_t* idst) {
+        const uint64_t* nu = d_uoff + n;
+        if (le && !write)
+            hipLaunchKernelGGL((mdrw_induced_kernel<kLE, false>), dim3(sgrid), dim3(kIndBlock), 0, s, dg,
+                               (const unsigned long long*)d_mask, W, nu, d_ux, d_uk, d_pbase, d_xpair, d_coff,
+                               d_ccnt, d_cpos, isrc, idst);
+        else if (le)
+            hipLaunchKernelGGL((mdrw_induced_kernel<kLE, true>), dim3(sgrid), dim3(kIndBlock), 0, s, dg,
+                               (const unsigned long long*)d_mask, W, nu, d_ux, d_uk, d_pbase, d_xpair, d_coff,
+                               d_ccnt, d_cpos, isrc, idst);
+        else if (!write)
+            hipLaunchKernelGGL((mdrw_induced_kernel<kGT, false>), dim3(sgrid), dim3(kIndBlock), 0, s, dg,
+                               (const unsigned long long*)d_mask, W, nu, d_ux, d_uk, d_pbase, d_xpair, d_coff,
+                               d_ccnt, d_cpos, isrc, idst);
+        else
+            hipLaunchKernelGGL((mdrw_induced_kernel<kGT, true>), dim3(sgrid), dim3(kIndBlock), 0, s, dg,
+                               (const unsigned long long*)d_mask, W, nu, d_ux, d_uk, d_pbase, d_xpair, d_coff,
+                               d_ccnt, d_cpos, isrc, idst);
+        return hipGetLastError();
+    };
+    for (uint32_t i0 = 0; i0 < I; i0 += kSlice) {
+        e = prep(i0, std::min<uint32_t>(I, i0 + kSlice));
+        if (e == hipSuccess) e = induced(false, nullptr, nullptr);
+        if (e != hipSuccess) return status_of(e);
+    }
     e = exclusive_scan(U64Gen{d_ccnt}, Count{NC, nullptr}, NC, d_cpos, d_part3, d_part3 + nbC + 1, s);
     if (e != hipSuccess) return status_of(e);
     uint64_t NI = 0;
@@ -540,12 +663,12 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
     uint32_t* o_idst = o_isrc + NI;
     res->out.ind_src_dev = o_isrc;
     res->out.ind_dst_dev = o_idst;
-    if (le)
-        hipLaunchKernelGGL((mdrw_induced_kernel<kLE, true>), dim3(I * per_inst), dim3(kIndBlock), tl, s, dg, vert_off,
-                           o_verts, d_coff, d_cv, d_ck, slots, per_inst, nullptr, d_cpos, o_isrc, o_idst);
-    else
-        hipLaunchKernelGGL((mdrw_induced_kernel<kGT, true>), dim3(I * per_inst), dim3(kIndBlock), tl, s, dg, vert_off,
-                           o_verts, d_coff, d_cv, d_ck, slots, per_inst, nullptr, d_cpos, o_isrc, o_idst);
+    for (uint32_t i0 = 0; i0 < I; i0 += kSlice) {
+        // a single slice keeps its masks and ranks from the count pass
+        e = I > kSlice ? prep(i0, std::min<uint32_t>(I, i0 + kSlice)) : hipSuccess;
+        if (e == hipSuccess) e = induced(true, o_isrc, o_idst);
+        if (e != hipSuccess) return status_of(e);
+    }
     hipLaunchKernelGGL(mdrw_ind_off_kernel, dim3((I + 256) / 256), dim3(256), 0, s, vert_off, d_coff, d_cpos, I,
                        ind_off);
     e = hipGetLastError();

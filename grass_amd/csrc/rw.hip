@@ -42,12 +42,6 @@ constexpr uint32_t pitch() { return STAGE + 1; }
 
 __device__ __forceinline__ uint64_t bytes_of_bits(uint32_t bits) { return (bits + 7) >> 3; }
 
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Writes staged columns [c0, c0 + ncols) of the wave's rows: STAGE lanes per row, 64 / STAGE rows
 // per store.
 template <uint32_t STAGE>

@@ -48,6 +48,21 @@ def test_mdrw_matches_oracle(ga, oracle_mod, case, fmt):
 
 
 @pytest.mark.parametrize("fmt", FMTS)
+@pytest.mark.parametrize("count", [130, 600])
+def test_mdrw_many_instances(ga, oracle_mod, fmt, count):
+    """Induced lists are built per slice of 512 instances with one membership bit per instance:
+    130 instances use three mask words, 600 two slices (the second of 88 instances)."""
+    print(f"[mdrw] many instances count={count} fmt={fmt}", flush=True)
+    g, og = _pair(ga, oracle_mod, "rmat12", fmt)
+    rng = np.random.default_rng(count)
+    sizes = [int(x) for x in rng.integers(1, 300, size=count)]
+    res = g.mdrw(sizes, target=400, seed=count)
+    exp = og.mdrw(sizes, target=400, seed=count)
+    _same(res, exp)
+    g.close()
+
+
+@pytest.mark.parametrize("fmt", FMTS)
 def test_mdrw_isolated_frontiers(ga, oracle_mod, fmt):
     """Most vertices of `hub` have no edge: many small frontiers stop at step 0 (degree sum 0)."""
     g, og = _pair(ga, oracle_mod, "hub", fmt)
