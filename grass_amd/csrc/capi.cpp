@@ -71,7 +71,7 @@ void free_graph(grass_graph* g) {
     if (g->d_start) (void)hipFree(g->d_start);
     if (g->d_deg) (void)hipFree(g->d_deg);
     if (g->d_hdr) (void)hipFree(g->d_hdr);
-    if (g->d_cp) (void)hipFree(g->d_cp);
+    if (g->d_gtx) (void)hipFree(g->d_gtx);
     if (g->d_err) (void)hipFree(g->d_err);
     g->ws.release();
     g->ws_bytes.release();

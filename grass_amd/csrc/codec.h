@@ -26,11 +26,12 @@ constexpr int kPadWords = 16;  // zero words after the last .graph word (a heade
                                 // <= 252 bits past a corrupt record start before the validator stops)
 
 struct DevGraph {
-    const uint32_t* __restrict__ w;      // big-endian word view of .graph, padded
-    const uint64_t* __restrict__ start;  // bit start of record v, n + 1 entries
+    // Payload words the samplers decode (big-endian word view, padded): the .graph image itself
+    // for LE; for GT the checkpointed payload stream built at load (see gtx_block_bit).
+    const uint32_t* __restrict__ w;
+    const uint64_t* __restrict__ start;  // bit start of record v in .graph, n + 1 entries
     const uint32_t* __restrict__ deg;    // degree of v, decoded and validated at load
     const uint4* __restrict__ hdr;       // decoded header of v (see pack_hdr), built at load
-    const uint32_t* __restrict__ cp;     // GT block checkpoints, built at load
     uint32_t n;
     int fmt;
     uint32_t cp_shift;  // GT checkpoint interval = 1 << cp_shift in-block positions
@@ -42,14 +43,16 @@ struct Hdr {
     uint32_t d;
     uint32_t f1, f2, fp;
     uint32_t C, q, R;  // LE: chunks, q = floor((d-1)/C); GT: blocks, q = floor(d/C); R remainder
-    uint32_t hbits;    // header length in bits
-    uint64_t cpb;      // GT: index of this vertex's first checkpoint
+    uint32_t hbits;    // header length in bits (of the .graph record)
 };
 
-// GT checkpoints: for every block, the absolute id at in-block positions P, 2P, ... with
-// P = 1 << cp_shift (K = floor(q / P) slots per block, K * C per vertex).  A GT access at in-block
-// position i then reads one checkpoint and at most P - 1 gaps instead of i gaps (the reference's
-// O(sqrt d) walk, GT/src/bcg.cuh:154-168).  Only vertices with q >= P have any.
+// GT checkpoints.  The GT payload stream on the device keeps the .graph blocks and their order
+// but stores, at every in-block position that is a nonzero multiple of P = 1 << cp_shift, the
+// absolute id (32 bits) in place of the gap.  A GT access at in-block position i then reads one
+// contiguous span, the absolute value at position i & ~(P - 1) (a checkpoint, or the block
+// head) and at most P - 1 gaps after it, instead of the head and i gaps (the reference's
+// O(sqrt d) walk, GT/src/bcg.cuh:154-168): one memory line where a separate checkpoint array
+// needed two.  Every vertex's payload starts on a 32-bit word.
 constexpr uint32_t kGtCheckShiftDefault = 2;
 
 // Decoded-header cache: one 16-byte entry per vertex, so that a sampling step reads one aligned
@@ -94,19 +97,11 @@ __device__ __forceinline__ void unpack_hdr(uint4 e, Hdr& h) {
     h.R = D - h.C * h.q;
 }
 
-// LE entries are 16 bytes; GT entries are 32 bytes (the second half holds the checkpoint base),
-// 32-byte aligned so that both halves share one memory sector.
+// The payload bit of a GT entry points into the device's GT payload stream once the loader
+// has built it (into .graph before that).
 template <int FMT>
 __device__ __forceinline__ void load_hdr(const DevGraph& g, uint32_t v, Hdr& h) {
-    if (FMT == kGT) {
-        const uint4 e0 = g.hdr[2 * (uint64_t)v];
-        const uint4 e1 = g.hdr[2 * (uint64_t)v + 1];
-        unpack_hdr<FMT>(e0, h);
-        h.cpb = ((uint64_t)e1.y << 32) | e1.x;
-    } else {
-        unpack_hdr<FMT>(g.hdr[v], h);
-        h.cpb = 0;
-    }
+    unpack_hdr<FMT>(g.hdr[v], h);
 }
 
 __host__ __device__ __forceinline__ uint32_t isqrt_u32(uint32_t x) {
@@ -279,10 +274,27 @@ __device__ __forceinline__ void gt_locate(const Hdr& h, uint32_t k, uint32_t& bl
     }
 }
 
-// Bit where block blk of a GT record starts (GT/src/bcg.cuh:161).
-__device__ __forceinline__ uint64_t gt_block_bit(const Hdr& h, uint32_t blk) {
+// Bit where block blk of a GT record starts in .graph (GT/src/bcg.cuh:161).
+__device__ __forceinline__ uint64_t gt_file_block_bit(const Hdr& h, uint32_t blk) {
     return h.pay + (uint64_t)blk * (h.f2 * (h.q - 1) + h.f1) +
            (uint64_t)(blk < h.R ? blk : h.R) * h.f2;
+}
+
+// Device GT payload stream geometry (P = 1 << sh): bit of in-block position i from the block
+// start, i.e. the head, i - 1 fields and one 32-bit checkpoint per P of them.
+__host__ __device__ __forceinline__ uint64_t gtx_field(const Hdr& h, uint32_t i, uint32_t sh) {
+    return i ? h.f1 + (uint64_t)(i - 1) * h.f2 + (uint64_t)((i - 1) >> sh) * (32 - h.f2) : 0;
+}
+
+// Bit where block blk starts: R blocks of q + 1 positions, then blocks of q.
+__host__ __device__ __forceinline__ uint64_t gtx_block_bit(const Hdr& h, uint32_t blk, uint32_t sh) {
+    const uint32_t big = blk < h.R ? blk : h.R;
+    return h.pay + (uint64_t)big * gtx_field(h, h.q + 1, sh) + (uint64_t)(blk - big) * gtx_field(h, h.q, sh);
+}
+
+// Payload bits of a GT vertex in the device stream.
+__host__ __device__ __forceinline__ uint64_t gtx_payload_bits(const Hdr& h, uint32_t sh) {
+    return h.d ? (uint64_t)h.R * gtx_field(h, h.q + 1, sh) + (uint64_t)(h.C - h.R) * gtx_field(h, h.q, sh) : 0;
 }
 
 // Absolute value of the head of block blk.
@@ -314,44 +326,47 @@ __device__ __forceinline__ uint32_t sum_fields(const uint32_t* __restrict__ w, u
     return sum;
 }
 
-// Neighbour k of vertex v, GT (GT/src/bcg.cuh:154-168): head plus i gaps of (body + 1).
-__device__ __forceinline__ uint32_t gt_get(const uint32_t* __restrict__ w, const Hdr& h,
-                                           uint32_t v, uint32_t k, uint32_t* fbits = nullptr) {
-    uint32_t blk, i;
-    gt_locate(h, k, blk, i);
-    uint64_t bo = gt_block_bit(h, blk);
-    uint32_t val = gt_head(w, h, v, blk, bo);
-    if (i) val += sum_fields(w, bo + h.f1, i, h.f2) + i;
-    if (fbits) *fbits = h.f1 + i * h.f2;
-    return val;
-}
-
-
-// GT random access through the block checkpoints: neighbour k of v is the checkpoint at or
-// before its in-block position (or the block head) plus at most P - 1 gaps.  fbits receives the
-// bits read: 32 for a checkpoint (or head_bit for a head) plus the gap fields.
+// GT random access in the device stream (GT/src/bcg.cuh:154-168 reads the head and i gaps):
+// neighbour k of v is the absolute value at in-block position i & ~(P - 1) (the block head, or
+// a checkpoint) plus the at most P - 1 gaps (body + 1) stored right after it.  fbits receives
+// the bits the same access costs in the .graph layout with a separate checkpoint table: 32 for
+// a checkpoint (head_bit for a head) plus the gap fields.
 __device__ __forceinline__ uint32_t gt_get_cp(const DevGraph& g, const Hdr& h, uint32_t v, uint32_t k,
                                               uint32_t* fbits = nullptr) {
     uint32_t blk, i;
     gt_locate(h, k, blk, i);
-    const uint64_t bo = gt_block_bit(h, blk);
     const uint32_t sh = g.cp_shift;
-    const uint32_t j = i >> sh;  // 0 when the vertex has no checkpoints (q < P)
-    uint32_t val, rem;
+    const uint64_t bo = gtx_block_bit(h, blk, sh);
+    const uint32_t b = (i >> sh) << sh;  // 0 when the vertex has no checkpoints (q < P)
+    const uint32_t rem = i - b;
+    uint32_t val;
     uint64_t gaps;
-    if (j) {
-        const uint32_t K = h.q >> sh;
-        val = g.cp[h.cpb + (uint64_t)blk * K + (j - 1)];
-        rem = i - (j << sh);
-        gaps = bo + h.f1 + (uint64_t)(j << sh) * h.f2;
+    if (b) {
+        const uint64_t at = bo + gtx_field(h, b, sh);
+        val = read_bits(g.w, at, 32);
+        gaps = at + 32;
     } else {
         val = gt_head(g.w, h, v, blk, bo);
-        rem = i;
         gaps = bo + h.f1;
     }
     if (rem) val += sum_fields(g.w, gaps, rem, h.f2) + rem;
-    if (fbits) *fbits = (j ? 32u : h.f1) + rem * h.f2;
+    if (fbits) *fbits = (b ? 32u : h.f1) + rem * h.f2;
     return val;
+}
+
+// Position i of block blk of a GT vertex in the device stream: the absolute id (head = 1) at
+// the block head and at checkpoints, else the gap + 1 (head = 0); a segmented prefix sum over
+// consecutive positions turns them into ids.
+__device__ __forceinline__ uint32_t gt_element(const DevGraph& g, const Hdr& h, uint32_t v, uint32_t blk,
+                                               uint32_t i, int& head) {
+    const uint32_t sh = g.cp_shift;
+    const uint64_t bo = gtx_block_bit(h, blk, sh);
+    head = 1;
+    if (i == 0) return gt_head(g.w, h, v, blk, bo);
+    const uint64_t at = bo + gtx_field(h, i, sh);
+    if ((i & ((1u << sh) - 1)) == 0) return read_bits(g.w, at, 32);
+    head = 0;
+    return read_bits(g.w, at, h.f2) + 1;
 }
 
 // Division by a divisor every lane shares, as a multiply-high and one correction: with
@@ -372,7 +387,8 @@ struct UDiv {
 
 // Element k of vertex v when the whole wave is inside v's list (h uniform): the record
 // geometry's divisions go through the shared UDivs d1 = q + 1 and d0 = q.  LE returns the
-// neighbour; GT returns the block head (head = 1) or the gap + 1 (head = 0).
+// neighbour; GT returns an absolute id (block head or checkpoint, head = 1) or the gap + 1
+// (head = 0).
 template <int FMT>
 __device__ __forceinline__ uint32_t element_uniform(const DevGraph& g, const Hdr& h, uint32_t v, uint32_t k,
                                                     const UDiv& d1, const UDiv& d0, int& head) {
@@ -406,10 +422,7 @@ __device__ __forceinline__ uint32_t element_uniform(const DevGraph& g, const Hdr
         const uint32_t step = (len == h.q + 1 ? d1 : d0).div(z - a);  // (z - a) / len
         return step * i + a + ((x & 1) ? 0u - mag : mag);
     } else {
-        const uint64_t bo = gt_block_bit(h, c);
-        if (i == 0) return gt_head(g.w, h, v, c, bo);
-        head = 0;
-        return read_bits(g.w, bo + h.f1 + (uint64_t)(i - 1) * h.f2, h.f2) + 1;
+        return gt_element(g, h, v, c, i, head);
     }
 }
 

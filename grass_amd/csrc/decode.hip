@@ -7,11 +7,12 @@
 // lane finds its vertex from the 64 row ends after the previous owner (one coalesced load and a
 // register search), parses that vertex's header and decodes its element:
 //   LE: O(1) random access (LE/src/bcg.cuh:196-261);
-//   GT: the head of each block is absolute and every body is a gap, so the wave turns
-//       (head | gap+1) values into ids with a segmented prefix sum, carrying the running id
-//       from lane 63 into the next 64 slots (the cooperative decode of GT/src/bcg.cuh:171-345
-//       and GT/src/grass.cu:525-553, without its per-warp cache limits).  A wave whose range
-//       starts inside a block first backs up to that block's head.
+//   GT: block heads and checkpoints are absolute and every other body is a gap, so the wave
+//       turns (absolute | gap+1) values into ids with a segmented prefix sum, carrying the
+//       running id from lane 63 into the next 64 slots (the cooperative decode of
+//       GT/src/bcg.cuh:171-345 and GT/src/grass.cu:525-553, without its per-warp cache
+//       limits).  A wave whose range starts between two absolute positions first backs up to
+//       the earlier one.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
         load_hdr<kGT>(g, vlo, h);
         uint32_t blk, i;
         gt_locate(h, (uint32_t)(e0 - rowptr[vlo]), blk, i);
-        eb = e0 - i;
+        eb = e0 - (i & ((1u << g.cp_shift) - 1));  // back to the block head or checkpoint
     }
     uint32_t carry = 0;
     // owner of the iteration's first slot (a lower bound for every lane's), wave-uniform
@@ -115,13 +116,7 @@ __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
             } else {
                 uint32_t blk, i;
                 gt_locate(h, k, blk, i);
-                const uint64_t bo = gt_block_bit(h, blk);
-                if (i == 0) {
-                    x = gt_head(g.w, h, v, blk, bo);
-                } else {
-                    x = read_bits(g.w, bo + h.f1 + (uint64_t)(i - 1) * h.f2, h.f2) + 1;
-                    head = 0;
-                }
+                x = gt_element(g, h, v, blk, i, head);
             }
         }
         if (FMT == kGT) {

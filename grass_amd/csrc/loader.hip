@@ -2,7 +2,8 @@
 //   1. byte swap of the .graph and .offset images into big-endian word views,
 //   2. record-length extraction from .offset fused into a reduce-then-scan (bit starts),
 //   3. a validator that parses every record header, checks header + payload == record
-//      length, and writes the degree array.
+//      length, and writes the degree array and the decoded-header cache,
+//   4. GT: the checkpointed payload stream the samplers decode (gt_expand_kernel).
 // Replaces loadGraph (LE/src/grass.cu:1015-1095), copyPartitionToGPU (LE/src/utils.hpp:328-353)
 // with its decode_offset kernel (LE/src/bcg.cuh:284-329) and CUB scan.
 #include <hip/hip_runtime.h>
@@ -90,12 +91,7 @@ __global__ __launch_bounds__(kValBlock) void validate_kernel(const uint32_t* __r
         }
         h.hbits = (uint32_t)(h.pay - b0);
         deg[v] = h.d;
-        if (FMT == kGT) {
-            hdr[2 * (uint64_t)v] = pack_hdr(h, FMT);
-            hdr[2 * (uint64_t)v + 1] = make_uint4(0u, 0u, 0u, 0u);  // checkpoint base, set later
-        } else {
-            hdr[v] = pack_hdr(h, FMT);
-        }
+        hdr[v] = pack_hdr(h, FMT);
         sum += h.d;
         mx = h.d > mx ? h.d : mx;
     }
@@ -109,55 +105,59 @@ __global__ __launch_bounds__(kValBlock) void validate_kernel(const uint32_t* __r
     }
 }
 
-// ------------------------------------------------------------------ GT block checkpoints
+// ------------------------------------------------------------------ GT payload stream
 
-// Checkpoint slots of a GT vertex: K = floor(q / P) per block, C blocks (P = 1 << shift).
-struct CpCountGen {
+// Words of vertex v's payload in the device GT stream (P = 1 << shift).
+struct GtxWordsGen {
     const uint4* hdr;
     uint32_t shift;
     __device__ __forceinline__ uint64_t operator()(uint64_t v) const {
-        const uint4 e = hdr[2 * v];
-        const uint32_t d = e.z, C = e.w & 0xFFFFu;
-        if (d == 0 || C == 0) return 0;
-        const uint32_t q = d / C;
-        return (uint64_t)C * (q >> shift);
+        Hdr h;
+        unpack_hdr<kGT>(hdr[v], h);
+        return (gtx_payload_bits(h, shift) + 31) >> 5;
     }
 };
 
-__global__ void cp_base_kernel(uint4* __restrict__ hdr, const uint64_t* __restrict__ base, uint32_t n) {
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-         v += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t b = base[v];
-        hdr[2 * v + 1] = make_uint4((uint32_t)b, (uint32_t)(b >> 32), 0u, 0u);
-    }
-}
+constexpr int kExpBlock = 256;
+constexpr uint32_t kExpWords = kWave + 2;  // words that one pass of 64 fields (<= 32 bits) can touch
 
-// One wave per vertex with checkpoints: the wave decodes the whole list 64 elements at a time
-// (block heads absolute, bodies as gaps, a segmented prefix sum with the running id carried
-// between passes) and stores the id at every in-block position that is a multiple of P.
-__global__ __launch_bounds__(256) void gt_checkpoint_kernel(DevGraph g, uint32_t* __restrict__ cp) {
+// One wave per GT vertex: decodes the .graph record 64 positions at a time (heads absolute,
+// bodies as gaps, a segmented prefix sum carrying the running id from pass to pass) and writes
+// the vertex's payload in the device stream at word woff[v]: heads and bodies copied, the 32-bit
+// id at every checkpoint position.  A pass assembles its fields in LDS, stores the words that lie
+// inside it and ORs in its first and last word (shared with the neighbouring passes; `out` starts
+// zeroed).  Finally the header's payload bit is moved to the stream.
+__global__ __launch_bounds__(kExpBlock) void gt_expand_kernel(DevGraph g, const uint64_t* __restrict__ woff,
+                                                              uint4* __restrict__ hdr, uint32_t* __restrict__ out) {
+    __shared__ uint32_t buf[kExpBlock / kWave][kExpWords];
     const int lane = lane_id();
-    const uint64_t nw = (uint64_t)gridDim.x * 256 / kWave;
-    for (uint64_t v = ((uint64_t)blockIdx.x * 256 + threadIdx.x) / kWave; v < g.n; v += nw) {
+    uint32_t* B = buf[threadIdx.x / kWave];
+    const uint32_t sh = g.cp_shift, P = 1u << sh;
+    const uint64_t nw = (uint64_t)gridDim.x * kExpBlock / kWave;
+    for (uint64_t v = ((uint64_t)blockIdx.x * kExpBlock + threadIdx.x) / kWave; v < g.n; v += nw) {
+        uint4 e = hdr[v];
         Hdr h;
-        load_hdr<kGT>(g, (uint32_t)v, h);  // same vertex in every lane: uniform branches
-        const uint32_t sh = g.cp_shift, P = 1u << sh;
-        if (h.d == 0 || h.q < P) continue;
-        const uint32_t K = h.q >> sh;
+        unpack_hdr<kGT>(e, h);  // payload bit in .graph
+        Hdr hx = h;
+        hx.pay = woff[v] << 5;  // payload bit in the stream
         uint32_t carry = 0;
         for (uint32_t base = 0; base < h.d; base += kWave) {
             const uint32_t k = base + lane;
             const bool act = k < h.d;
             uint32_t blk, i;
             gt_locate(h, act ? k : h.d - 1, blk, i);
-            const uint64_t bo = gt_block_bit(h, blk);
-            uint32_t x;
+            const uint64_t bo = gt_file_block_bit(h, blk);
+            uint32_t raw, x, wdt;
             int head;
             if (i == 0) {
-                x = gt_head(g.w, h, (uint32_t)v, blk, bo);
+                raw = read_bits(g.w, bo, h.f1);
+                x = blk >= h.fp ? (uint32_t)v + raw + 1 : (uint32_t)v - raw - 1;
+                wdt = h.f1;
                 head = 1;
             } else {
-                x = read_bits(g.w, bo + h.f1 + (uint64_t)(i - 1) * h.f2, h.f2) + 1;
+                raw = read_bits0(g.w, bo + h.f1 + (uint64_t)(i - 1) * h.f2, h.f2);
+                x = raw + 1;
+                wdt = h.f2;
                 head = 0;
             }
             if (lane == 0 && !head) {
@@ -166,7 +166,38 @@ __global__ __launch_bounds__(256) void gt_checkpoint_kernel(DevGraph g, uint32_t
             }
             x = wave_segmented_sum(x, head);
             carry = __shfl(x, kWave - 1, kWave);
-            if (act && i >= P && (i & (P - 1)) == 0) cp[h.cpb + (uint64_t)blk * K + (i >> sh) - 1] = x;
+            if (i && (i & (P - 1)) == 0) {
+                raw = x;
+                wdt = 32;
+            }
+            const uint64_t at = gtx_block_bit(hx, blk, sh) + gtx_field(hx, i, sh);
+            // words [w0, w0 + nwords) hold this pass's fields; the last active lane ends them
+            const uint32_t la = min((uint32_t)kWave - 1, h.d - 1 - base);
+            const uint64_t w0 = __shfl(at, 0, kWave) >> 5;
+            const uint64_t end = __shfl(at + wdt, (int)la, kWave);
+            const uint32_t nwords = (uint32_t)(((end - 1) >> 5) - w0 + 1);
+            for (uint32_t j = lane; j < kExpWords; j += kWave) B[j] = 0;
+            wave_sync();
+            if (act && wdt) {
+                const uint32_t rel = (uint32_t)((at >> 5) - w0), s = (uint32_t)(at & 31);
+                if (s + wdt <= 32) {
+                    atomicOr(&B[rel], raw << (32 - s - wdt));
+                } else {
+                    atomicOr(&B[rel], raw >> (s + wdt - 32));
+                    atomicOr(&B[rel + 1], raw << (64 - s - wdt));
+                }
+            }
+            wave_sync();
+            for (uint32_t j = lane; j < nwords; j += kWave) {
+                if (j == 0 || j == nwords - 1) atomicOr(&out[w0 + j], B[j]);
+                else out[w0 + j] = B[j];
+            }
+            wave_sync();
+        }
+        if (lane == 0) {
+            e.x = (uint32_t)hx.pay;
+            e.y = (e.y & ~0xFFu) | (uint32_t)(hx.pay >> 32);
+            hdr[v] = e;
         }
     }
 }
@@ -237,7 +268,7 @@ uint64_t host_field(const uint8_t* tailb, uint64_t tlen, uint64_t file_bytes, ui
 
 }  // namespace
 
-int build_checkpoints(grass_graph* g, uint64_t* count);
+int build_gt_stream(grass_graph* g, uint64_t* words);
 
 int load_graph(grass_graph* g, ByteSource& gsrc, ByteSource& osrc) {
     const uint64_t gbytes = gsrc.size(), obytes = osrc.size();
@@ -288,7 +319,7 @@ int load_graph(grass_graph* g, ByteSource& gsrc, ByteSource& osrc) {
 
     if (hipMalloc(&g->d_start, (n + 1) * 8) != hipSuccess) return fail(GRASS_ENOMEM);
     if (hipMalloc(&g->d_deg, n * 4) != hipSuccess) return fail(GRASS_ENOMEM);
-    const uint64_t hdr_bytes = n * 16 * (g->fmt == kGT ? 2 : 1);
+    const uint64_t hdr_bytes = n * 16;
     if (hipMalloc(&g->d_hdr, hdr_bytes) != hipSuccess) return fail(GRASS_ENOMEM);
     const uint64_t nb = scan_blocks(n);
     uint64_t* d_tmp = nullptr;  // partials[nb], total, stats[4]
@@ -324,41 +355,49 @@ int load_graph(grass_graph* g, ByteSource& gsrc, ByteSource& osrc) {
     g->total_bits = total;
     g->m = stats[0];
     g->max_degree = stats[1];
-    uint64_t ncp = 0;
+    uint64_t xwords = 0;
     if (g->fmt == kGT) {
-        int rc2 = build_checkpoints(g, &ncp);
+        int rc2 = build_gt_stream(g, &xwords);
         if (rc2) return rc2;
     }
-    g->device_bytes = gwords * 4 + (n + 1) * 8 + n * 4 + hdr_bytes + ncp * 4;
+    g->device_bytes = gwords * 4 + (n + 1) * 8 + n * 4 + hdr_bytes + xwords * 4;
     return GRASS_OK;
 }
 
-int build_checkpoints(grass_graph* g, uint64_t* count) {
+// Builds the device GT payload stream (g->d_gtx) from .graph and points the headers at it.
+int build_gt_stream(grass_graph* g, uint64_t* words) {
     hipStream_t s = g->stream;
     const uint64_t n = g->n, nb = scan_blocks(n);
-    uint64_t* d_base = nullptr;  // n + 1 bases, partials, total
-    if (hipMalloc(&d_base, (n + 1 + nb + 2) * 8) != hipSuccess) return GRASS_ENOMEM;
-    hipError_t e = exclusive_scan(CpCountGen{g->d_hdr, g->cp_shift}, Count{n, nullptr}, n, d_base, d_base + n + 1,
-                                  d_base + n + 1 + nb, s);
-    const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 16384);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(cp_base_kernel, dim3(grid), dim3(256), 0, s, g->d_hdr, (const uint64_t*)d_base,
-                           (uint32_t)n);
-        e = hipGetLastError();
-    }
+    uint64_t* d_woff = nullptr;  // n + 1 word offsets, partials, total
+    if (hipMalloc(&d_woff, (n + 1 + nb + 2) * 8) != hipSuccess) return GRASS_ENOMEM;
+    hipError_t e = exclusive_scan(GtxWordsGen{g->d_hdr, g->cp_shift}, Count{n, nullptr}, n, d_woff,
+                                  d_woff + n + 1, d_woff + n + 1 + nb, s);
     uint64_t total = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&total, d_base + n, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, d_woff + n, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipFree(d_base);
-    if (e != hipSuccess) return status_of(e);
-    if (hipMalloc(&g->d_cp, std::max<uint64_t>(total, 1) * 4) != hipSuccess) return GRASS_ENOMEM;
-    if (total) {
-        hipLaunchKernelGGL(gt_checkpoint_kernel, dim3(4096), dim3(256), 0, s, g->dev(), g->d_cp);
-        e = hipGetLastError();
+    uint32_t* out = nullptr;
+    int rc = status_of(e);
+    // payload bits must fit the header cache's 40-bit field
+    if (!rc && total >= (1ull << 35)) rc = GRASS_EFORMAT;
+    if (!rc && hipMalloc(&out, (total + kPadWords) * 4) != hipSuccess) rc = GRASS_ENOMEM;
+    if (!rc) {
+        e = hipMemsetAsync(out, 0, (total + kPadWords) * 4, s);
+        if (e == hipSuccess) {
+            // g->dev() still decodes .graph here: d_gtx is set once the stream is complete
+            hipLaunchKernelGGL(gt_expand_kernel, dim3(4096), dim3(kExpBlock), 0, s, g->dev(),
+                               (const uint64_t*)d_woff, g->d_hdr, out);
+            e = hipGetLastError();
+        }
         if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) return status_of(e);
+        rc = status_of(e);
     }
-    *count = total;
+    (void)hipFree(d_woff);
+    if (rc) {
+        if (out) (void)hipFree(out);
+        return rc;
+    }
+    g->d_gtx = out;
+    *words = total + kPadWords;
     return GRASS_OK;
 }
 

@@ -348,10 +348,10 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
         const uint32_t kb = kc * kIndChunk;
         const uint32_t ke = min(h.d, kb + kIndChunk);
         uint32_t start = kb;
-        if (FMT == kGT && kb) {  // back up to the head of the block holding kb
+        if (FMT == kGT && kb) {  // back up to the block head or checkpoint before kb
             uint32_t blk, i;
             gt_locate(h, kb, blk, i);
-            start = kb - i;
+            start = kb - (i & ((1u << g.cp_shift) - 1));
         }
         uint32_t carry = 0;
         for (uint32_t base = start; base < ke; base += kWave) {

@@ -126,8 +126,8 @@ struct grass_graph {
     uint32_t* d_words = nullptr;
     uint64_t* d_start = nullptr;
     uint32_t* d_deg = nullptr;
-    uint4* d_hdr = nullptr;  // decoded-header cache, 16 B (LE) or 32 B (GT) per vertex
-    uint32_t* d_cp = nullptr;  // GT block checkpoints
+    uint4* d_hdr = nullptr;  // decoded-header cache, 16 B per vertex
+    uint32_t* d_gtx = nullptr;  // GT: checkpointed payload stream the samplers decode
     uint32_t cp_shift = grass::kGtCheckShiftDefault;  // GT checkpoint interval 1 << cp_shift
     uint32_t* d_err = nullptr;  // [0] first error kind, [1] detail
     hipStream_t stream = nullptr;
@@ -138,11 +138,10 @@ struct grass_graph {
 
     grass::DevGraph dev() const {
         grass::DevGraph d;
-        d.w = d_words;
+        d.w = d_gtx ? d_gtx : d_words;
         d.start = d_start;
         d.deg = d_deg;
         d.hdr = d_hdr;
-        d.cp = d_cp;
         d.n = (uint32_t)n;
         d.fmt = fmt;
         d.cp_shift = cp_shift;
