@@ -48,7 +48,8 @@ typedef struct {
     int32_t format;        /* grass_format */
     int32_t device;
     uint64_t device_bytes; /* HBM held by the resident graph */
-    uint32_t gt_checkpoint; /* GT: in-block interval of the checkpoint index built at load (the
+    uint32_t gt_checkpoint; /* GT: in-block interval of the checkpoints (absolute ids) that the
+                               device payload stream built at load stores in place of gaps (the
                                environment variable GRASS_GT_CHECKPOINT, a power of two in
                                [2, 1024], overrides the default 4); 0 for LE */
 } grass_graph_info;
