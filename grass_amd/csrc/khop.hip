@@ -51,14 +51,16 @@ __device__ __forceinline__ uint32_t seg_of(const uint64_t* __restrict__ off, uin
 }
 
 // ------------------------------------------------------------------ hash table (dedup)
-// keys: (gen << 48) | (request << 31) | vertex; a slot whose generation is not the current one
-// is free.  vals: ((0xFFFF - gen) << 32) | position, so atomicMin keeps the smallest position of
-// the current generation over any stale value.
+// Slot h = {key, val} in one aligned 16-byte pair, so that a probe, its insert and its atomicMin
+// touch one memory line.  key: (gen << 48) | (request << 31) | vertex; a slot whose generation is
+// not the current one is free.  val: ((0xFFFF - gen) << 32) | position, so atomicMin keeps the
+// smallest position of the current generation over any stale value.
 struct HashTab {
-    unsigned long long* keys;
-    unsigned long long* vals;
+    unsigned long long* slot;  // 2 * (mask + 1) words
     uint64_t mask;
     uint32_t shift;
+    __device__ __forceinline__ unsigned long long* key(uint64_t h) const { return slot + 2 * h; }
+    __device__ __forceinline__ unsigned long long* val(uint64_t h) const { return slot + 2 * h + 1; }
 };
 
 __device__ __forceinline__ uint64_t ht_home(const HashTab& t, uint64_t key48) {
@@ -68,8 +70,8 @@ __device__ __forceinline__ uint64_t ht_home(const HashTab& t, uint64_t key48) {
 __global__ void ht_clear_kernel(HashTab t) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= t.mask;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        t.keys[i] = 0ull;
-        t.vals[i] = ~0ull;
+        *t.key(i) = 0ull;
+        *t.val(i) = ~0ull;
     }
 }
 
@@ -98,10 +100,10 @@ __global__ __launch_bounds__(kKhBlock) void dedup_insert_kernel(DedupIn in, Coun
         uint64_t h = ht_home(t, key48);
         uint64_t probes = 0;
         for (;;) {
-            unsigned long long cur = __atomic_load_n(&t.keys[h], __ATOMIC_RELAXED);
+            unsigned long long cur = __atomic_load_n(t.key(h), __ATOMIC_RELAXED);
             if (cur == full) break;
             if ((uint32_t)(cur >> 48) != gen) {
-                unsigned long long prev = atomicCAS(&t.keys[h], cur, full);
+                unsigned long long prev = atomicCAS(t.key(h), cur, full);
                 if (prev == cur || prev == full) break;
                 continue;  // the slot was taken meanwhile: look at it again
             }
@@ -111,7 +113,7 @@ __global__ __launch_bounds__(kKhBlock) void dedup_insert_kernel(DedupIn in, Coun
                 break;
             }
         }
-        atomicMin(&t.vals[h], ((unsigned long long)(0xFFFFu - gen) << 32) | (uint32_t)p);
+        atomicMin(t.val(h), ((unsigned long long)(0xFFFFu - gen) << 32) | (uint32_t)p);
     }
 }
 
@@ -130,8 +132,9 @@ __global__ __launch_bounds__(kKhBlock) void dedup_mark_kernel(DedupIn in, Count 
             unsigned long long full = ((unsigned long long)gen << 48) | key48;
             uint64_t h = ht_home(t, key48);
             for (uint64_t probes = 0; probes <= t.mask; ++probes) {
-                if (t.keys[h] == full) {
-                    first = (uint32_t)t.vals[h] == (uint32_t)p;
+                const ulonglong2 kv = *(const ulonglong2*)t.key(h);  // key and val in one load
+                if (kv.x == full) {
+                    first = (uint32_t)kv.y == (uint32_t)p;
                     break;
                 }
                 h = (h + 1) & t.mask;
@@ -361,8 +364,7 @@ static hipError_t ensure_table(Workspace& ws, uint64_t entries, HashTab& t) {
         ws.ht_slots = slots;
         ws.ht_gen = 0xFFFF;  // force a clear
     }
-    t.keys = (unsigned long long*)ws.ht.ptr;
-    t.vals = t.keys + ws.ht_slots;
+    t.slot = (unsigned long long*)ws.ht.ptr;
     t.mask = ws.ht_slots - 1;
     uint32_t lg = 0;
     while ((1ull << lg) < ws.ht_slots) ++lg;
