@@ -14,13 +14,13 @@
 //    decode and one degree load, all wave-uniform.
 // 2. mdrw_verts_kernel: one workgroup per instance sorts frontier + step destinations in LDS
 //    (bitonic) and keeps the first of each run of equal ids.
-// 3. Induced edges, per slice of up to 512 instances: a bit mask per vertex marks the slice's
-//    instances whose sample holds it.  The lists of the slice's distinct vertices are cut into
+// 3. Induced edges, per slice of up to 512 instances: a bit mask row per distinct vertex of the
+//    slice marks the instances whose sample holds it.  The lists of those vertices are cut into
 //    chunks of kIndChunk entries and each chunk is decoded once by one wave (LE: O(1) random
-//    access per lane; GT: block heads plus gaps turned into ids with a segmented wave sum); entry
-//    y of x's list joins the induced list of every instance in mask[x] & mask[y].  A count pass
-//    and a scan give every (instance, vertex, chunk) its output offset, so the write pass emits
-//    each instance's edges in (vertex, position) order without any sorting.
+//    access per lane; GT: absolute values plus gaps turned into ids with a segmented wave sum);
+//    entry y of x's list joins the induced list of every instance in mask(x) & mask(y).  A count
+//    pass and a scan give every (instance, vertex, chunk) its output offset, so the write pass
+//    emits each instance's edges in (vertex, position) order without any sorting.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -82,8 +82,17 @@ __global__ __launch_bounds__(kWave) void mdrw_walk_kernel(DevGraph g, const MdDe
         uint64_t W = wave_sum(bs);
         uint32_t t = 0;
         bool bad = false;
+        // The draws of a step do not depend on the walk: lane l computes those of step t0 + l once
+        // per 64 steps, and each step reads its own from that lane.
+        U4 ahead{0, 0, 0, 0};
         for (; t < q.steps && W > 0; ++t) {
-            const U4 r = philox4x32_10(q.inst, t, kTagMdrwStep, 0, seed);
+            const int tl = (int)(t & (kWave - 1));
+            if (tl == 0) ahead = philox4x32_10(q.inst, t + lane, kTagMdrwStep, 0, seed);
+            U4 r;
+            r.x = (uint32_t)__builtin_amdgcn_readlane(ahead.x, tl);
+            r.y = (uint32_t)__builtin_amdgcn_readlane(ahead.y, tl);
+            r.z = (uint32_t)__builtin_amdgcn_readlane(ahead.z, tl);
+            r.w = (uint32_t)__builtin_amdgcn_readlane(ahead.w, tl);
             const uint64_t x = mulhi64(((uint64_t)r.y << 32) | r.x, W);
             // block of the pick: first lane whose inclusive block sum exceeds x (exists: total W > x)
             const uint64_t incl = wave_inclusive_sum(bs);
@@ -217,13 +226,19 @@ struct ChunkGen {  // chunks of vertex gv
 };
 
 // Membership masks.  The samples of one slice of up to kSlice instances share one pass over the
-// lists of their distinct vertices: mask[x] holds one bit per instance of the slice whose sample
-// contains x, so an adjacency entry (x, y) belongs to the induced lists of exactly the instances in
-// mask[x] & mask[y].  Every list is decoded once per slice, however many samples hold its vertex.
+// lists of their distinct vertices: the mask row of x holds one bit per instance of the slice whose
+// sample contains x, so an adjacency entry (x, y) belongs to the induced lists of exactly the
+// instances in mask(x) & mask(y).  Every list is decoded once per slice, however many samples
+// hold its vertex.  Rows are indexed by a dense id (dmap[v] = the id, or kNoRow for a vertex no
+// sample of the slice holds; dvert inverts it), so the rows that the induced pass reads at random
+// (W words per sampled vertex, 162 MB for 512 instances at scale 24) fit the Infinity Cache where
+// rows for all n vertices would not.
+constexpr uint32_t kNoRow = 0xFFFFFFFFu;
 constexpr uint32_t kSlice = 512;
 constexpr uint32_t kSliceWords = kSlice / 64;
 constexpr int kIndBlock = 256;
 constexpr int kIndWaves = kIndBlock / kWave;
+constexpr uint32_t kRowPitch = kSliceWords + 1;  // u64 words per staged row in LDS (odd: fewer bank conflicts)
 
 // Instance of pair p: the largest i in [i0, i1) with voff[i] <= p.
 __device__ __forceinline__ uint32_t inst_of(const uint64_t* __restrict__ voff, uint32_t i0, uint32_t i1,
@@ -237,80 +252,114 @@ __device__ __forceinline__ uint32_t inst_of(const uint64_t* __restrict__ voff, u
     return lo;
 }
 
-// mask[x * W + l / 64] gets bit l % 64 for every pair (instance i0 + l, vertex x) of the slice.
-__global__ __launch_bounds__(256) void mdrw_mask_kernel(const uint32_t* __restrict__ verts,
+// dmap[x] = 1 for every vertex x that a sample of slice [i0, i1) holds (dmap starts zeroed).
+__global__ __launch_bounds__(256) void mdrw_flag_kernel(const uint32_t* __restrict__ verts,
                                                         const uint64_t* __restrict__ voff, uint32_t i0,
-                                                        uint32_t i1, uint32_t W,
-                                                        unsigned long long* __restrict__ mask) {
+                                                        uint32_t i1, uint32_t* __restrict__ dmap) {
     const uint64_t p0 = voff[i0], p1 = voff[i1];
-    for (uint64_t p = p0 + (uint64_t)blockIdx.x * 256 + threadIdx.x; p < p1; p += (uint64_t)gridDim.x * 256) {
-        const uint32_t l = inst_of(voff, i0, i1, p) - i0;
-        atomicOr(&mask[(uint64_t)verts[p] * W + (l >> 6)], 1ull << (l & 63));
+    for (uint64_t p = p0 + (uint64_t)blockIdx.x * 256 + threadIdx.x; p < p1; p += (uint64_t)gridDim.x * 256)
+        dmap[verts[p]] = 1u;
+}
+
+// From the flags and their exclusive scan `did`: dmap[x] = dense id (or kNoRow), dvert[id] = x.
+__global__ __launch_bounds__(256) void mdrw_dense_kernel(const uint64_t* __restrict__ did, uint32_t n,
+                                                         uint32_t* __restrict__ dmap, uint32_t* __restrict__ dvert) {
+    for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (uint64_t)gridDim.x * 256) {
+        if (dmap[x]) {
+            const uint32_t d = (uint32_t)did[x];
+            dmap[x] = d;
+            dvert[d] = (uint32_t)x;
+        } else {
+            dmap[x] = kNoRow;
+        }
     }
 }
 
-struct PopGen {  // instances of the slice holding vertex x
-    const unsigned long long* mask;
+// rows[0, *nd * W) = 0
+__global__ __launch_bounds__(256) void mdrw_zero_rows_kernel(unsigned long long* __restrict__ rows,
+                                                             const uint64_t* __restrict__ nd, uint32_t W) {
+    const uint64_t count = *nd * W;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256)
+        rows[i] = 0ull;
+}
+
+// Row of x, word l / 64, gets bit l % 64 for every pair (instance i0 + l, vertex x) of the slice.
+__global__ __launch_bounds__(256) void mdrw_mask_kernel(const uint32_t* __restrict__ verts,
+                                                        const uint64_t* __restrict__ voff, uint32_t i0,
+                                                        uint32_t i1, uint32_t W, const uint32_t* __restrict__ dmap,
+                                                        unsigned long long* __restrict__ rows) {
+    const uint64_t p0 = voff[i0], p1 = voff[i1];
+    for (uint64_t p = p0 + (uint64_t)blockIdx.x * 256 + threadIdx.x; p < p1; p += (uint64_t)gridDim.x * 256) {
+        const uint32_t l = inst_of(voff, i0, i1, p) - i0;
+        atomicOr(&rows[(uint64_t)dmap[verts[p]] * W + (l >> 6)], 1ull << (l & 63));
+    }
+}
+
+struct PopGen {  // instances of the slice holding dense vertex d
+    const unsigned long long* rows;
     uint32_t W;
-    __device__ __forceinline__ uint64_t operator()(uint64_t x) const {
+    __device__ __forceinline__ uint64_t operator()(uint64_t d) const {
         uint64_t c = 0;
-        for (uint32_t w = 0; w < W; ++w) c += __popcll(mask[x * W + w]);
+        for (uint32_t w = 0; w < W; ++w) c += __popcll(rows[d * W + w]);
         return c;
     }
 };
 
-struct UChunkGen {  // chunks of vertex x when some sample of the slice holds it
-    const unsigned long long* mask;
-    uint32_t W;
+struct UChunkGen {  // chunks of the list of dense vertex d
+    const uint32_t* dvert;
     const uint32_t* deg;
-    __device__ __forceinline__ uint64_t operator()(uint64_t x) const {
-        unsigned long long any = 0;
-        for (uint32_t w = 0; w < W; ++w) any |= mask[x * W + w];
-        return any ? (deg[x] + kIndChunk - 1) / kIndChunk : 0;
+    __device__ __forceinline__ uint64_t operator()(uint64_t d) const {
+        return (deg[dvert[d]] + kIndChunk - 1) / kIndChunk;
     }
 };
 
-// xpair[pbase[x] + r] = p for pair p = (instance i, vertex x), r = rank of i among the slice's
-// instances holding x: from a vertex's mask bits to its pairs (and so to their output slots).
+// xpair[pbase[d] + r] = p for pair p = (instance i, vertex x), d = dmap[x], r = rank of i among
+// the slice's instances holding x: from a row's bits to its pairs (and so to their output slots).
 __global__ __launch_bounds__(256) void mdrw_xpair_kernel(const uint32_t* __restrict__ verts,
                                                          const uint64_t* __restrict__ voff, uint32_t i0,
-                                                         uint32_t i1, uint32_t W,
-                                                         const unsigned long long* __restrict__ mask,
+                                                         uint32_t i1, uint32_t W, const uint32_t* __restrict__ dmap,
+                                                         const unsigned long long* __restrict__ rows,
                                                          const uint64_t* __restrict__ pbase,
                                                          uint32_t* __restrict__ xpair) {
     const uint64_t p0 = voff[i0], p1 = voff[i1];
     for (uint64_t p = p0 + (uint64_t)blockIdx.x * 256 + threadIdx.x; p < p1; p += (uint64_t)gridDim.x * 256) {
         const uint32_t l = inst_of(voff, i0, i1, p) - i0;
-        const uint64_t x = verts[p];
+        const uint64_t d = dmap[verts[p]];
         uint64_t r = 0;
-        for (uint32_t w = 0; w < (l >> 6); ++w) r += __popcll(mask[x * W + w]);
-        r += __popcll(mask[x * W + (l >> 6)] & ((1ull << (l & 63)) - 1));
-        xpair[pbase[x] + r] = (uint32_t)p;
+        for (uint32_t w = 0; w < (l >> 6); ++w) r += __popcll(rows[d * W + w]);
+        r += __popcll(rows[d * W + (l >> 6)] & ((1ull << (l & 63)) - 1));
+        xpair[pbase[d] + r] = (uint32_t)p;
     }
 }
 
-// chunk c of the slice's distinct vertices: ux[c] = vertex, uk[c] = chunk index in its list
-__global__ __launch_bounds__(256) void mdrw_uexpand_kernel(const uint64_t* __restrict__ uoff, uint32_t n,
+// chunk c of the slice's distinct vertices: ux[c] = dense id, uk[c] = chunk index in its list
+__global__ __launch_bounds__(256) void mdrw_uexpand_kernel(const uint64_t* __restrict__ uoff,
+                                                           const uint64_t* __restrict__ nd,
                                                            uint32_t* __restrict__ ux, uint32_t* __restrict__ uk) {
-    for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (uint64_t)gridDim.x * 256) {
-        const uint64_t b = uoff[x], e = uoff[x + 1];
+    const uint64_t ND = *nd;
+    for (uint64_t d = (uint64_t)blockIdx.x * 256 + threadIdx.x; d < ND; d += (uint64_t)gridDim.x * 256) {
+        const uint64_t b = uoff[d], e = uoff[d + 1];
         for (uint64_t c = b; c < e; ++c) {
-            ux[c] = (uint32_t)x;
+            ux[c] = (uint32_t)d;
             uk[c] = (uint32_t)(c - b);
         }
     }
 }
 
 // One wave per chunk of a distinct vertex x.  The wave decodes the chunk once; for every entry y
-// and every instance i of the slice in mask[x] & mask[y] it counts (COUNT) or writes (WRITE) the
-// edge (x, y) into instance i's induced list.  Output slot of (instance i, x, chunk k) is
+// and every instance i of the slice in mask(x) & mask(y) it counts (COUNT) or writes (WRITE) the
+// edge (x, y) into instance i's induced list (its destination y; mdrw_src_kernel fills the
+// sources, x for the whole range of pair (i, x)).  Output slot of (instance i, x, chunk k) is
 // coff[pair (i, x)] + k in the instance-major chunk enumeration, so a scan of the counts gives
 // every (i, x, k) its position and the edges land ordered by instance, vertex and list position.
-// Lane l keeps the count (or next output slot) of instance 64 w + l of each mask word w in
-// registers; per matched instance one ballot ranks the wave's hits in list order.
+// COUNT: each lane adds its own entry's matches to per-wave LDS counters (the order of the adds
+// does not matter).  WRITE: lane l keeps the next output slot of instance 64 w + l of each mask
+// word w in registers, and per matched instance one ballot ranks the wave's hits in list order.
 template <int FMT, bool WRITE>
-__global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, const unsigned long long* __restrict__ mask,
-                                                                 uint32_t W, const uint64_t* __restrict__ nchunks,
+__global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, const unsigned long long* __restrict__ rows,
+                                                                 uint32_t W, const uint32_t* __restrict__ dmap,
+                                                                 const uint32_t* __restrict__ dvert,
+                                                                 const uint64_t* __restrict__ nchunks,
                                                                  const uint32_t* __restrict__ ux,
                                                                  const uint32_t* __restrict__ uk,
                                                                  const uint64_t* __restrict__ pbase,
@@ -318,28 +367,34 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
                                                                  const uint64_t* __restrict__ coff,
                                                                  uint64_t* __restrict__ ccnt,
                                                                  const uint64_t* __restrict__ cpos,
-                                                                 uint32_t* __restrict__ isrc,
                                                                  uint32_t* __restrict__ idst) {
+    __shared__ uint32_t counts[WRITE ? 1 : kIndWaves][kSlice];  // COUNT: per wave and instance
+    __shared__ unsigned long long staged[kIndWaves][kWave * kRowPitch];  // the window's rows
     const int lane = lane_id();
     const unsigned long long below = (1ull << lane) - 1;
     const uint64_t NU = *nchunks;
     const uint64_t waves = (uint64_t)gridDim.x * kIndWaves;
+    uint32_t* cnt = counts[WRITE ? 0 : threadIdx.x / kWave];
+    unsigned long long* R = staged[threadIdx.x / kWave];
     for (uint64_t c = (uint64_t)blockIdx.x * kIndWaves + threadIdx.x / kWave; c < NU; c += waves) {
-        const uint32_t x = __builtin_amdgcn_readfirstlane(ux[c]);
+        const uint32_t dx = __builtin_amdgcn_readfirstlane(ux[c]);
         const uint32_t kc = __builtin_amdgcn_readfirstlane(uk[c]);
-        const uint64_t pb = pbase[x];
+        const uint32_t x = __builtin_amdgcn_readfirstlane(dvert[dx]);
+        const uint64_t pb = pbase[dx];
         unsigned long long mx[kSliceWords];
-        uint64_t acc[kSliceWords];  // lane l: instance 64 w + l's count (COUNT) or next slot (WRITE)
+        uint64_t acc[kSliceWords];  // WRITE, lane l: next output slot of instance 64 w + l
         uint32_t rbase = 0;
 #pragma unroll
         for (uint32_t w = 0; w < kSliceWords; ++w) {
-            const unsigned long long v = w < W ? mask[(uint64_t)x * W + w] : 0ull;
+            const unsigned long long v = w < W ? rows[(uint64_t)dx * W + w] : 0ull;
             mx[w] = readfirstlane_u64(v);
             acc[w] = 0;
             if (WRITE && ((mx[w] >> lane) & 1ull))
                 acc[w] = cpos[coff[xpair[pb + rbase + __popcll(mx[w] & below)]] + kc];
+            if (!WRITE) cnt[w * 64 + lane] = 0;
             rbase += __popcll(mx[w]);
         }
+        if (!WRITE) wave_sync();
         Hdr h;
         load_hdr<FMT>(g, x, h);
         UDiv d1, d0;
@@ -367,11 +422,33 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
                 y = wave_segmented_sum(y, head);
                 carry = __shfl(y, kWave - 1, kWave);
             }
-            const bool valid = act && k >= kb && y < g.n;
+            // row of y: none when no sample of the slice holds y (or the entry is not this chunk's)
+            const uint32_t dy = (act && k >= kb && y < g.n) ? dmap[y] : kNoRow;
+            // Stage the 64 rows in LDS.  With W >= 2 the W / 2 lanes of a row each load 16 bytes
+            // of it, so a row is one memory access instead of one per mask word.
+            if (W == 1) {
+                R[lane * kRowPitch] = dy != kNoRow ? rows[dy] : 0ull;
+            } else {
+                const uint32_t cpr = W >> 1, per = kWave / cpr;  // chunks per row, rows per round
+                for (uint32_t r = 0; r < cpr; ++r) {
+                    const uint32_t e = r * per + lane / cpr, q = lane % cpr;
+                    const uint32_t de = __shfl(dy, (int)e, kWave);
+                    ulonglong2 v = make_ulonglong2(0ull, 0ull);
+                    if (de != kNoRow) v = *(const ulonglong2*)(rows + (uint64_t)de * W + 2 * q);
+                    R[e * kRowPitch + 2 * q] = v.x;
+                    R[e * kRowPitch + 2 * q + 1] = v.y;
+                }
+            }
+            wave_sync();
 #pragma unroll
             for (uint32_t w = 0; w < kSliceWords; ++w) {
                 if (mx[w] == 0ull) continue;  // uniform
-                const unsigned long long m = valid ? (mx[w] & mask[(uint64_t)y * W + w]) : 0ull;
+                const unsigned long long m = mx[w] & R[lane * kRowPitch + w];
+                if (!WRITE) {
+                    for (unsigned long long b2 = m; b2; b2 &= b2 - 1)
+                        atomicAdd(&cnt[w * 64 + __builtin_ctzll(b2)], 1u);
+                    continue;
+                }
                 unsigned long long bits = mx[w];
                 while (bits) {  // uniform loop over x's instances in this word
                     const int b = __builtin_ctzll(bits);
@@ -379,26 +456,41 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
                     const bool hit = (m >> b) & 1ull;
                     const unsigned long long lanes = __ballot(hit);
                     if (!lanes) continue;
-                    if (WRITE) {
+                    if (WRITE) {  // the sources are filled per pair by mdrw_src_kernel
                         const uint64_t slot = readlane_u64(acc[w], b);
-                        if (hit) {
-                            const uint64_t pos = slot + __popcll(lanes & below);
-                            isrc[pos] = x;
-                            idst[pos] = y;
-                        }
+                        if (hit) idst[slot + __popcll(lanes & below)] = y;
                     }
                     if (lane == b) acc[w] += __popcll(lanes);
                 }
             }
+            wave_sync();  // the rows are restaged for the next window
         }
         if (!WRITE) {
+            wave_sync();
             rbase = 0;
 #pragma unroll
             for (uint32_t w = 0; w < kSliceWords; ++w) {
-                if ((mx[w] >> lane) & 1ull) ccnt[coff[xpair[pb + rbase + __popcll(mx[w] & below)]] + kc] = acc[w];
+                if ((mx[w] >> lane) & 1ull)
+                    ccnt[coff[xpair[pb + rbase + __popcll(mx[w] & below)]] + kc] = cnt[w * 64 + lane];
                 rbase += __popcll(mx[w]);
             }
+            wave_sync();  // the counters are cleared for the next chunk
         }
+    }
+}
+
+// The induced edges of pair p = (instance, x) fill [cpos[coff[p]], cpos[coff[p + 1]]) and all
+// have source x: one wave per pair writes them with coalesced stores.
+__global__ __launch_bounds__(256) void mdrw_src_kernel(const uint32_t* __restrict__ verts,
+                                                       const uint64_t* __restrict__ coff,
+                                                       const uint64_t* __restrict__ cpos, uint64_t NV,
+                                                       uint32_t* __restrict__ isrc) {
+    const int lane = lane_id();
+    const uint64_t waves = (uint64_t)gridDim.x * (256 / kWave);
+    for (uint64_t p = ((uint64_t)blockIdx.x * 256 + threadIdx.x) / kWave; p < NV; p += waves) {
+        const uint32_t x = verts[p];
+        const uint64_t b = cpos[coff[p]], e = cpos[coff[p + 1]];
+        for (uint64_t i = b + lane; i < e; i += kWave) isrc[i] = x;
     }
 }
 
@@ -579,21 +671,27 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
     rc = sync_read(s, &NC, d_coff + NV, 8);
     if (rc) return rc;
 
-    // phase 3: per slice of up to kSlice instances, membership masks over the n vertices; every
-    // distinct vertex of the slice has its list decoded once per pass (count, then write)
+    // phase 3: per slice of up to kSlice instances, membership rows for the slice's distinct
+    // vertices; every distinct vertex has its list decoded once per pass (count, then write)
     const uint64_t n = g->n;
-    const uint32_t W = (std::min<uint32_t>(I, kSlice) + 63) / 64;
+    // mask words per row, a power of two (16-byte aligned row chunks for the staged loads)
+    uint32_t W = 1;
+    while (W * 64 < std::min<uint32_t>(I, kSlice)) W <<= 1;
     const uint64_t nbN = scan_blocks(n), nbC = scan_blocks(NC);
     DevMem tmp3;
-    unsigned long long* d_mask = nullptr;
-    uint64_t *d_pbase = nullptr, *d_uoff = nullptr, *d_partN = nullptr, *d_ccnt = nullptr, *d_cpos = nullptr,
-             *d_part3 = nullptr;
-    uint32_t *d_xpair = nullptr, *d_ux = nullptr, *d_uk = nullptr;
+    unsigned long long* d_rows = nullptr;
+    uint64_t *d_did = nullptr, *d_pbase = nullptr, *d_uoff = nullptr, *d_nu = nullptr, *d_partN = nullptr,
+             *d_ccnt = nullptr, *d_cpos = nullptr, *d_part3 = nullptr;
+    uint32_t *d_dmap = nullptr, *d_dvert = nullptr, *d_xpair = nullptr, *d_ux = nullptr, *d_uk = nullptr;
     auto plan3 = [&](Carve& c) {
-        d_mask = c.take<unsigned long long>(n * W);
+        d_rows = c.take<unsigned long long>(n * W);  // the slice's distinct vertices use a prefix
+        d_did = c.take<uint64_t>(n + 1);
         d_pbase = c.take<uint64_t>(n + 1);
         d_uoff = c.take<uint64_t>(n + 1);
+        d_nu = c.take<uint64_t>(1);
         d_partN = c.take<uint64_t>(nbN + 2);
+        d_dmap = c.take<uint32_t>(n);
+        d_dvert = c.take<uint32_t>(n);
         d_xpair = c.take<uint32_t>(NV);
         d_ux = c.take<uint32_t>(NC);
         d_uk = c.take<uint32_t>(NC);
@@ -610,46 +708,56 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const unsigned sgrid = (unsigned)std::max(cus, 1) * 8;  // grid-stride launches
-    // masks, their scans, pair ranks and chunk list of slice [i0, i1)
+    const uint64_t* d_nd = d_did + n;                        // distinct vertices of the slice
+    // dense ids, rows, their scans, pair ranks and chunk list of slice [i0, i1)
     auto prep = [&](uint32_t i0, uint32_t i1) -> hipError_t {
-        hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(sgrid), dim3(256), 0, s, d_mask, n * W, 0ull);
-        hipLaunchKernelGGL(mdrw_mask_kernel, dim3(sgrid), dim3(256), 0, s, (const uint32_t*)o_verts,
-                           (const uint64_t*)vert_off, i0, i1, W, d_mask);
-        hipError_t e = exclusive_scan(PopGen{d_mask, W}, Count{n, nullptr}, n, d_pbase, d_partN, d_partN + nbN + 1, s);
-        if (e == hipSuccess)
-            e = exclusive_scan(UChunkGen{d_mask, W, dg.deg}, Count{n, nullptr}, n, d_uoff, d_partN,
-                               d_partN + nbN + 1, s);
+        const uint32_t* verts = o_verts;
+        const uint64_t* voff = vert_off;
+        hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(sgrid), dim3(256), 0, s, d_dmap, n, 0u);
+        hipLaunchKernelGGL(mdrw_flag_kernel, dim3(sgrid), dim3(256), 0, s, verts, voff, i0, i1, d_dmap);
+        hipError_t e = exclusive_scan(U32Gen{d_dmap}, Count{n, nullptr}, n, d_did, d_partN, d_partN + nbN + 1, s);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(mdrw_xpair_kernel, dim3(sgrid), dim3(256), 0, s, (const uint32_t*)o_verts,
-                           (const uint64_t*)vert_off, i0, i1, W, (const unsigned long long*)d_mask,
-                           (const uint64_t*)d_pbase, d_xpair);
-        hipLaunchKernelGGL(mdrw_uexpand_kernel, dim3(sgrid), dim3(256), 0, s, (const uint64_t*)d_uoff,
-                           (uint32_t)n, d_ux, d_uk);
+        hipLaunchKernelGGL(mdrw_dense_kernel, dim3(sgrid), dim3(256), 0, s, (const uint64_t*)d_did, (uint32_t)n,
+                           d_dmap, d_dvert);
+        hipLaunchKernelGGL(mdrw_zero_rows_kernel, dim3(sgrid), dim3(256), 0, s, d_rows, d_nd, W);
+        hipLaunchKernelGGL(mdrw_mask_kernel, dim3(sgrid), dim3(256), 0, s, verts, voff, i0, i1, W,
+                           (const uint32_t*)d_dmap, d_rows);
+        e = exclusive_scan(PopGen{d_rows, W}, Count{0, d_nd}, n, d_pbase, d_partN, d_partN + nbN + 1, s);
+        if (e == hipSuccess)
+            e = exclusive_scan(UChunkGen{d_dvert, dg.deg}, Count{0, d_nd}, n, d_uoff, d_partN, d_nu, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(mdrw_xpair_kernel, dim3(sgrid), dim3(256), 0, s, verts, voff, i0, i1, W,
+                           (const uint32_t*)d_dmap, (const unsigned long long*)d_rows, (const uint64_t*)d_pbase,
+                           d_xpair);
+        hipLaunchKernelGGL(mdrw_uexpand_kernel, dim3(sgrid), dim3(256), 0, s, (const uint64_t*)d_uoff, d_nd,
+                           d_ux, d_uk);
         return hipGetLastError();
     };
-    auto induced = [&](bool write, uint32_t* isrc, uint32_t* idst) {
-        const uint64_t* nu = d_uoff + n;
+    auto induced = [&](bool write, uint32_t* idst) {
+        const unsigned long long* rows = d_rows;
+        const uint32_t* dmap = d_dmap;
+        const uint32_t* dvert = d_dvert;
         if (le && !write)
-            hipLaunchKernelGGL((mdrw_induced_kernel<kLE, false>), dim3(sgrid), dim3(kIndBlock), 0, s, dg,
-                               (const unsigned long long*)d_mask, W, nu, d_ux, d_uk, d_pbase, d_xpair, d_coff,
-                               d_ccnt, d_cpos, isrc, idst);
+            hipLaunchKernelGGL((mdrw_induced_kernel<kLE, false>), dim3(sgrid), dim3(kIndBlock), 0, s, dg, rows, W,
+                               dmap, dvert, (const uint64_t*)d_nu, d_ux, d_uk, d_pbase, d_xpair, d_coff, d_ccnt,
+                               d_cpos, idst);
         else if (le)
-            hipLaunchKernelGGL((mdrw_induced_kernel<kLE, true>), dim3(sgrid), dim3(kIndBlock), 0, s, dg,
-                               (const unsigned long long*)d_mask, W, nu, d_ux, d_uk, d_pbase, d_xpair, d_coff,
-                               d_ccnt, d_cpos, isrc, idst);
+            hipLaunchKernelGGL((mdrw_induced_kernel<kLE, true>), dim3(sgrid), dim3(kIndBlock), 0, s, dg, rows, W,
+                               dmap, dvert, (const uint64_t*)d_nu, d_ux, d_uk, d_pbase, d_xpair, d_coff, d_ccnt,
+                               d_cpos, idst);
         else if (!write)
-            hipLaunchKernelGGL((mdrw_induced_kernel<kGT, false>), dim3(sgrid), dim3(kIndBlock), 0, s, dg,
-                               (const unsigned long long*)d_mask, W, nu, d_ux, d_uk, d_pbase, d_xpair, d_coff,
-                               d_ccnt, d_cpos, isrc, idst);
+            hipLaunchKernelGGL((mdrw_induced_kernel<kGT, false>), dim3(sgrid), dim3(kIndBlock), 0, s, dg, rows, W,
+                               dmap, dvert, (const uint64_t*)d_nu, d_ux, d_uk, d_pbase, d_xpair, d_coff, d_ccnt,
+                               d_cpos, idst);
         else
-            hipLaunchKernelGGL((mdrw_induced_kernel<kGT, true>), dim3(sgrid), dim3(kIndBlock), 0, s, dg,
-                               (const unsigned long long*)d_mask, W, nu, d_ux, d_uk, d_pbase, d_xpair, d_coff,
-                               d_ccnt, d_cpos, isrc, idst);
+            hipLaunchKernelGGL((mdrw_induced_kernel<kGT, true>), dim3(sgrid), dim3(kIndBlock), 0, s, dg, rows, W,
+                               dmap, dvert, (const uint64_t*)d_nu, d_ux, d_uk, d_pbase, d_xpair, d_coff, d_ccnt,
+                               d_cpos, idst);
         return hipGetLastError();
     };
     for (uint32_t i0 = 0; i0 < I; i0 += kSlice) {
         e = prep(i0, std::min<uint32_t>(I, i0 + kSlice));
-        if (e == hipSuccess) e = induced(false, nullptr, nullptr);
+        if (e == hipSuccess) e = induced(false, nullptr);
         if (e != hipSuccess) return status_of(e);
     }
     e = exclusive_scan(U64Gen{d_ccnt}, Count{NC, nullptr}, NC, d_cpos, d_part3, d_part3 + nbC + 1, s);
@@ -666,9 +774,11 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
     for (uint32_t i0 = 0; i0 < I; i0 += kSlice) {
         // a single slice keeps its masks and ranks from the count pass
         e = I > kSlice ? prep(i0, std::min<uint32_t>(I, i0 + kSlice)) : hipSuccess;
-        if (e == hipSuccess) e = induced(true, o_isrc, o_idst);
+        if (e == hipSuccess) e = induced(true, o_idst);
         if (e != hipSuccess) return status_of(e);
     }
+    hipLaunchKernelGGL(mdrw_src_kernel, dim3(sgrid), dim3(256), 0, s, (const uint32_t*)o_verts,
+                       (const uint64_t*)d_coff, (const uint64_t*)d_cpos, NV, o_isrc);
     hipLaunchKernelGGL(mdrw_ind_off_kernel, dim3((I + 256) / 256), dim3(256), 0, s, vert_off, d_coff, d_cpos, I,
                        ind_off);
     e = hipGetLastError();
