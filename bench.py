@@ -426,26 +426,46 @@ def main():
         nk = 1
         while nk < R2 and sum(len(x) for x in kseeds[:nk]) * 260 < kh_rate * half:
             nk += 1
-        t = time.perf_counter()
-        for r in rs:
-            og.random_walk(r["walkers"], r["length"], seed, instance=r["instance"], nthreads=cores)
-        kres = og.khop(kseeds[:nk], kfans[:nk], seed=seed, instances=list(range(10000, 10000 + nk)),
-                       nthreads=cores) if R2 else {"dst": []}
-        dt = time.perf_counter() - t
-        ke = len(kres["dst"])
+        def sample_once():
+            t0 = time.perf_counter()
+            for r in rs:
+                og.random_walk(r["walkers"], r["length"], seed, instance=r["instance"], nthreads=cores)
+            kres = og.khop(kseeds[:nk], kfans[:nk], seed=seed, instances=list(range(10000, 10000 + nk)),
+                           nthreads=cores) if R2 else {"dst": []}
+            return time.perf_counter() - t0, len(kres["dst"])
+
+        sample_once()  # warm-up
+        runs = [sample_once() for _ in range(5)]
+        dt = float(np.median([x[0] for x in runs]))
+        ke = runs[0][1]
+        model = "unknown"
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
         cpu = {"value": (re_ + ke) / dt, "unit": "edges/s", "cores": cores, "kind": "port",
                "sample": f"first {len(rs)} of the {len(reqs)} random-walk requests ({re_} edges) and "
                          f"first {nk} of the {R2} 2-hop requests ({ke} edges), the oracle's C "
-                         f"restatement with OpenMP over walkers / requests, {dt:.1f} s"}
-        # one core, on the first walk requests only (a few seconds)
+                         f"restatement with OpenMP over walkers / requests; median of 5 runs after "
+                         f"a warm-up, {dt:.2f} s each",
+               "cpu_model": model}
+        # one core: the first walk requests that take about a second, median of 5 runs
         t = time.perf_counter()
-        e1 = 0
+        r1, e1 = [], 0
         for r in reqs:
             og.random_walk(r["walkers"], r["length"], seed, instance=r["instance"], nthreads=1)
+            r1.append(r)
             e1 += r["walkers"] * r["length"]
-            if time.perf_counter() - t > 3.0:
+            if time.perf_counter() - t > 1.0:
                 break
-        cpu["single_core_value"] = e1 / (time.perf_counter() - t)
+        t1 = []
+        for _ in range(5):
+            t = time.perf_counter()
+            for r in r1:
+                og.random_walk(r["walkers"], r["length"], seed, instance=r["instance"], nthreads=1)
+            t1.append(time.perf_counter() - t)
+        cpu["single_core_value"] = e1 / float(np.median(t1))
         # the full decode to CSR on the same cores (decoded .graph + .offset bytes per second)
         t = time.perf_counter()
         og.decode_csr(nthreads=cores)
