@@ -154,15 +154,19 @@ __global__ __launch_bounds__(kKhBlock) void compact_kernel(DedupIn in, Count E,
                                                            uint32_t R,
                                                            const uint32_t* __restrict__ flag,
                                                            const uint64_t* __restrict__ fpos,
+                                                           const uint32_t* __restrict__ deg,
                                                            uint32_t* __restrict__ fv,
-                                                           uint32_t* __restrict__ fr) {
+                                                           uint32_t* __restrict__ fr,
+                                                           uint32_t* __restrict__ fd) {
     const uint64_t n = E.get();
     for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
          p += (uint64_t)gridDim.x * blockDim.x) {
         if (!flag[p]) continue;
         uint64_t q = fpos[p];
-        fv[q] = in.at(p);
+        const uint32_t v = in.at(p);
+        fv[q] = v;
         fr[q] = seg_of(eoff_r, R, p);
+        fd[q] = deg[v];  // read here once; the count scan and the sampler read it in order
     }
 }
 
@@ -196,16 +200,15 @@ __global__ __launch_bounds__(kKhBlock) void seed_gather_kernel(const KhReq* __re
 }
 
 // ------------------------------------------------------------------ per hop
-struct CountGen {
-    const uint32_t* fv;
+struct CountGen {  // draws of frontier item i at hop h: min(fanout, degree)
+    const uint32_t* fd;
     const uint32_t* fr;
     const KhReq* reqs;
-    const uint32_t* deg;
     uint32_t h;
     __device__ __forceinline__ uint64_t operator()(uint64_t i) const {
         const KhReq& q = reqs[fr[i]];
         uint32_t f = h < q.hops ? q.fan[h] : 0u;
-        uint32_t d = deg[fv[i]];
+        uint32_t d = fd[i];
         return d < f ? d : f;
     }
 };
@@ -226,6 +229,7 @@ __global__ void hop_offsets_kernel(const uint64_t* __restrict__ foff, uint32_t R
 template <int FMT, int G>
 __global__ __launch_bounds__(kKhBlock) void sample_kernel(DevGraph g, const uint32_t* __restrict__ fv,
                                                           const uint32_t* __restrict__ fr,
+                                                          const uint32_t* __restrict__ fd,
                                                           const uint64_t* __restrict__ icount,
                                                           const KhReq* __restrict__ reqs, uint32_t h,
                                                           uint64_t seed,
@@ -249,7 +253,7 @@ __global__ __launch_bounds__(kKhBlock) void sample_kernel(DevGraph g, const uint
             v = fv[i];
             const KhReq& q = reqs[fr[i]];
             uint32_t f = h < q.hops ? q.fan[h] : 0u;
-            d = g.deg[v];
+            d = fd[i];
             c = d < f ? d : f;
             inst = q.inst;
             ebase = hbase + eoff[i];
@@ -343,11 +347,11 @@ static unsigned sample_grid() {
 // largest fanout, so that a fanout-10 hop does not leave two thirds of a 32-lane group idle.
 template <int FMT>
 static void launch_sample(grass_graph* g, uint32_t G, hipStream_t s, const uint32_t* fv, const uint32_t* fr,
-                          const uint64_t* ic, const KhReq* reqs, uint32_t h, uint64_t seed, const uint64_t* eoff,
+                          const uint32_t* fd, const uint64_t* ic, const KhReq* reqs, uint32_t h, uint64_t seed, const uint64_t* eoff,
                           const uint64_t* hb, uint32_t* osrc, uint32_t* odst, uint64_t cap) {
 #define GRASS_SAMPLE(GG)                                                                                   \
     hipLaunchKernelGGL((sample_kernel<FMT, GG>), dim3(sample_grid<FMT, GG>()), dim3(kKhBlock), 0, s, g->dev(), \
-                       fv, fr, ic, reqs, h, seed, eoff, hb, osrc, odst, cap, g->d_err)
+                       fv, fr, fd, ic, reqs, h, seed, eoff, hb, osrc, odst, cap, g->d_err)
     if (G <= 8) GRASS_SAMPLE(8);
     else if (G <= 16) GRASS_SAMPLE(16);
     else if (G <= 32) GRASS_SAMPLE(32);
@@ -427,7 +431,7 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
     const uint64_t nbI = scan_blocks(Imax), nbD = scan_blocks(Dmax);
 
     size_t need = Carve::need(R * sizeof(KhReq)) + Carve::need((R + 1) * 8) * 4 +
-                  Carve::need(S * 4 + 4) + Carve::need(Imax * 4) * 4 + Carve::need((Imax + 1) * 8) +
+                  Carve::need(S * 4 + 4) + Carve::need(Imax * 4) * 6 + Carve::need((Imax + 1) * 8) +
                   Carve::need(Dmax * 4) + Carve::need((Dmax + 1) * 8) +
                   Carve::need(std::max(nbI, nbD) * 8) + Carve::need(64 * 8);
     hipError_t e = ws.scratch.reserve(need);
@@ -440,6 +444,7 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
     uint32_t* d_seedv = cv.take<uint32_t>(S + 1);
     uint32_t* d_fv[2] = {cv.take<uint32_t>(Imax), cv.take<uint32_t>(Imax)};
     uint32_t* d_fr[2] = {cv.take<uint32_t>(Imax), cv.take<uint32_t>(Imax)};
+    uint32_t* d_fd[2] = {cv.take<uint32_t>(Imax), cv.take<uint32_t>(Imax)};  // frontier degrees
     uint64_t* d_eoff = cv.take<uint64_t>(Imax + 1);
     uint32_t* d_flag = cv.take<uint32_t>(Dmax);
     uint64_t* d_fpos = cv.take<uint64_t>(Dmax + 1);
@@ -475,7 +480,7 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
         e = exclusive_scan(FlagGen{d_flag}, Count{S, nullptr}, Dmax, d_fpos, d_part, d_ftot, s);
         if (e != hipSuccess) return status_of(e);
         hipLaunchKernelGGL(compact_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, Count{S, nullptr}, d_sbeg, R,
-                           d_flag, d_fpos, d_fv[0], d_fr[0]);
+                           d_flag, d_fpos, (const uint32_t*)g->d_deg, d_fv[0], d_fr[0], d_fd[0]);
         hipLaunchKernelGGL(front_offsets_kernel, dim3((R + 256) / 256), dim3(256), 0, s, d_sbeg, R, d_fpos,
                            d_foff[0], d_icount[0]);
     }
@@ -490,16 +495,16 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
     }
     for (uint32_t h = 0; h < H; ++h) {
         const int cur = h & 1, nxt = cur ^ 1;
-        CountGen cg{d_fv[cur], d_fr[cur], d_req, g->d_deg, h};
+        CountGen cg{d_fd[cur], d_fr[cur], d_req, h};
         e = exclusive_scan(cg, Count{0, d_icount[cur]}, Imax, d_eoff, d_part, d_etot, s);
         if (e != hipSuccess) return status_of(e);
         hipLaunchKernelGGL(hop_offsets_kernel, dim3((R + 256) / 256), dim3(256), 0, s, d_foff[cur], R, d_eoff,
                            d_eoff_r, d_hb, h, out->offsets_dev);
         if (g->fmt == kLE)
-            launch_sample<kLE>(g, lanes[h], s, d_fv[cur], d_fr[cur], d_icount[cur], d_req, h, seed, d_eoff,
+            launch_sample<kLE>(g, lanes[h], s, d_fv[cur], d_fr[cur], d_fd[cur], d_icount[cur], d_req, h, seed, d_eoff,
                                d_hb, out->src_dev, out->dst_dev, out->capacity);
         else
-            launch_sample<kGT>(g, lanes[h], s, d_fv[cur], d_fr[cur], d_icount[cur], d_req, h, seed, d_eoff,
+            launch_sample<kGT>(g, lanes[h], s, d_fv[cur], d_fr[cur], d_fd[cur], d_icount[cur], d_req, h, seed, d_eoff,
                                d_hb, out->src_dev, out->dst_dev, out->capacity);
         if (h + 1 == H) break;
         DedupIn in{out->dst_dev, d_hb + h, out->capacity};
@@ -512,7 +517,7 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
         e = exclusive_scan(FlagGen{d_flag}, E, Dmax, d_fpos, d_part, d_ftot, s);
         if (e != hipSuccess) return status_of(e);
         hipLaunchKernelGGL(compact_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, E, d_eoff_r, R, d_flag, d_fpos,
-                           d_fv[nxt], d_fr[nxt]);
+                           (const uint32_t*)g->d_deg, d_fv[nxt], d_fr[nxt], d_fd[nxt]);
         hipLaunchKernelGGL(front_offsets_kernel, dim3((R + 256) / 256), dim3(256), 0, s, d_eoff_r, R, d_fpos,
                            d_foff[nxt], d_icount[nxt]);
     }
