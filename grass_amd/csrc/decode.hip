@@ -41,9 +41,22 @@ __device__ __forceinline__ uint32_t owner_of(const uint64_t* __restrict__ rowptr
     return lo;
 }
 
+// wstart[w] = the vertex that holds slot w * per_wave, the first slot of decode wave w: each
+// vertex writes the waves that start inside its list.
+__global__ __launch_bounds__(256) void wave_start_kernel(const uint64_t* __restrict__ rowptr, uint32_t n,
+                                                         uint32_t per_wave, uint64_t waves,
+                                                         uint32_t* __restrict__ wstart) {
+    for (uint64_t v = (uint64_t)blockIdx.x * 256 + threadIdx.x; v < n; v += (uint64_t)gridDim.x * 256) {
+        const uint64_t b = rowptr[v], e = rowptr[v + 1];
+        for (uint64_t w = (b + per_wave - 1) / per_wave; w < waves && w * per_wave < e; ++w)
+            wstart[w] = (uint32_t)v;
+    }
+}
+
 template <int FMT>
 __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
                                                                const uint64_t* __restrict__ rowptr,
+                                                               const uint32_t* __restrict__ wstart,
                                                                uint32_t* __restrict__ col,
                                                                uint64_t m, uint32_t per_wave) {
     const uint64_t wave = ((uint64_t)blockIdx.x * kDecBlock + threadIdx.x) / kWave;
@@ -51,8 +64,9 @@ __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
     const uint64_t e0 = wave * per_wave;
     if (e0 >= m) return;  // uniform per wave
     const uint64_t e1 = e0 + per_wave < m ? e0 + per_wave : m;
-    const uint32_t vlo = owner_of(rowptr, 0, g.n - 1, e0);
-    const uint32_t vhi = owner_of(rowptr, vlo, g.n - 1, e1 - 1);
+    // owner of e0, and a bound on the owner of e1 - 1 (the next wave's first owner)
+    const uint32_t vlo = wstart[wave];
+    const uint32_t vhi = e1 < m ? wstart[wave + 1] : g.n - 1;
     uint64_t eb = e0;
     if (FMT == kGT) {
         Hdr h;
@@ -150,12 +164,36 @@ int decode_csr(grass_graph* g, uint64_t* rowptr, uint32_t* col, hipStream_t s) {
     const uint64_t waves = (m + per_wave - 1) / per_wave;
     const uint64_t blocks = (waves + kDecBlock / kWave - 1) / (kDecBlock / kWave);
     if (blocks > 0x7fffffffull) return GRASS_EINVAL;
+    // The waves' first vertices depend only on the graph and per_wave: built once per per_wave
+    // value from these row pointers, complete before it is published, never changed after.
+    uint32_t slot = 0;
+    while ((512u << slot) < per_wave && slot + 1 < kWaveStartSlots) ++slot;
+    if ((512u << slot) != per_wave) return GRASS_EINVAL;
+    const uint32_t* wstart;
+    {
+        std::lock_guard<std::mutex> lock(g->mu);
+        if (!g->d_wstart[slot]) {
+            uint32_t* t = nullptr;
+            if (hipMalloc(&t, (waves + 1) * 4) != hipSuccess) return GRASS_ENOMEM;
+            const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 16384);
+            hipLaunchKernelGGL(wave_start_kernel, dim3(grid), dim3(256), 0, s, (const uint64_t*)rowptr,
+                               (uint32_t)n, per_wave, waves, t);
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) {
+                (void)hipFree(t);
+                return status_of(e);
+            }
+            g->d_wstart[slot] = t;
+        }
+        wstart = g->d_wstart[slot];
+    }
     if (g->fmt == kLE)
         hipLaunchKernelGGL(decode_csr_kernel<kLE>, dim3((unsigned)blocks), dim3(kDecBlock), 0, s,
-                           g->dev(), (const uint64_t*)rowptr, col, m, per_wave);
+                           g->dev(), (const uint64_t*)rowptr, wstart, col, m, per_wave);
     else
         hipLaunchKernelGGL(decode_csr_kernel<kGT>, dim3((unsigned)blocks), dim3(kDecBlock), 0, s,
-                           g->dev(), (const uint64_t*)rowptr, col, m, per_wave);
+                           g->dev(), (const uint64_t*)rowptr, wstart, col, m, per_wave);
     return status_of(hipGetLastError());
 }
 

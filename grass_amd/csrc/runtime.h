@@ -114,6 +114,8 @@ enum : uint32_t {
     kErrBadStart = 2,     // a caller-supplied vertex id >= n
 };
 
+constexpr uint32_t kWaveStartSlots = 5;  // decode slots per wave: 512, 1024, ..., 8192
+
 }  // namespace grass
 
 struct grass_graph {
@@ -130,6 +132,8 @@ struct grass_graph {
     uint32_t* d_gtx = nullptr;  // GT: checkpointed payload stream the samplers decode
     uint32_t cp_shift = grass::kGtCheckShiftDefault;  // GT checkpoint interval 1 << cp_shift
     uint32_t* d_err = nullptr;  // [0] first error kind, [1] detail
+    // full decode: first vertex of every wave, one table per slots-per-wave value 512 << i
+    uint32_t* d_wstart[grass::kWaveStartSlots] = {};
     hipStream_t stream = nullptr;
     std::mutex mu;
     grass::Workspace ws;     // working memory of the synchronous entry points (under mu)
