@@ -161,17 +161,9 @@ __device__ __forceinline__ uint32_t read_bits0(const uint32_t* __restrict__ w, u
     return nbits ? read_bits(w, b, nbits) : 0u;
 }
 
-// Elias-gamma value at b, b advanced past it (LE/src/bcg.cuh:74-98).  Valid input only: the
-// load-time validator has checked every header of the graph.
-__device__ __forceinline__ uint32_t gamma_at(const uint32_t* __restrict__ w, uint64_t& b) {
-    uint32_t L = __clz(peek32(w, b));
-    uint32_t y = peek32(w, b + L);
-    b += 2 * L + 1;
-    return (y >> (31 - L)) - 1u;
-}
-
-// Checked gamma for the validator: false on an all-zero 32-bit window (a code the
-// reference's 32-bit clz cannot represent).
+// Elias-gamma value at b, b advanced past it (LE/src/bcg.cuh:74-98), for the load-time
+// validator: false on an all-zero 32-bit window (a code the reference's 32-bit clz cannot
+// represent).  Samplers never parse headers: they read the decoded-header cache.
 __device__ __forceinline__ bool gamma_checked(const uint32_t* __restrict__ w, uint64_t& b,
                                               uint32_t& val) {
     uint32_t x = peek32(w, b);
@@ -181,28 +173,6 @@ __device__ __forceinline__ bool gamma_checked(const uint32_t* __restrict__ w, ui
     b += 2 * L + 1;
     val = (y >> (31 - L)) - 1u;
     return true;
-}
-
-template <int FMT>
-__device__ __forceinline__ void parse_header(const uint32_t* __restrict__ w, uint64_t b, Hdr& h) {
-    uint64_t b0 = b;
-    h.d = gamma_at(w, b);
-    h.f1 = h.f2 = h.fp = h.C = h.q = h.R = 0;
-    if (FMT == kLE) {
-        if (h.d >= 1) h.f1 = gamma_at(w, b);
-        if (h.d >= 2) {
-            h.f2 = gamma_at(w, b);
-            h.fp = gamma_at(w, b);
-            le_geometry(h.d, h.C, h.q, h.R);
-        }
-    } else {
-        h.f1 = gamma_at(w, b);
-        h.f2 = gamma_at(w, b);
-        h.fp = gamma_at(w, b);
-        if (h.d) gt_geometry(h.d, h.C, h.q, h.R);
-    }
-    h.pay = b;
-    h.hbits = (uint32_t)(b - b0);
 }
 
 // Payload length implied by a header; the validator requires header + payload == record.
