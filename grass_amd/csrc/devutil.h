@@ -108,6 +108,31 @@ __device__ __forceinline__ T block_sum(T x, T* lds) {
     return tot;
 }
 
+// Bit-stream passes (MSB-first bits in big-endian-bit 32-bit words): a wave assembles the fields
+// of up to 64 positions in an LDS buffer whose first word is stream word w0, then writes it out.
+
+// ORs the `wdt`-bit field `val` (wdt <= 32, val < 2^wdt) at stream bit `at` into the buffer.
+__device__ __forceinline__ void put_field(uint32_t* B, uint64_t w0, uint64_t at, uint32_t wdt, uint32_t val) {
+    if (!wdt) return;
+    const uint32_t rel = (uint32_t)((at >> 5) - w0), s = (uint32_t)(at & 31);
+    if (s + wdt <= 32) {
+        atomicOr(&B[rel], val << (32 - s - wdt));
+    } else {
+        atomicOr(&B[rel], val >> (s + wdt - 32));
+        atomicOr(&B[rel + 1], val << (64 - s - wdt));
+    }
+}
+
+// Writes the buffer to stream words [w0, w0 + nwords): the first and last words, which the
+// neighbouring passes may share, are ORed in (`out` starts zeroed); the others are stored.
+__device__ __forceinline__ void flush_pass(uint32_t* out, const uint32_t* B, uint64_t w0, uint32_t nwords,
+                                           int lane) {
+    for (uint32_t j = lane; j < nwords; j += kWave) {
+        if (j == 0 || j == nwords - 1) atomicOr(&out[w0 + j], B[j]);
+        else out[w0 + j] = B[j];
+    }
+}
+
 // p[0..count) = value.  Used instead of hipMemsetAsync for anything a caller might capture in a
 // HIP graph.
 template <typename T>

@@ -26,6 +26,7 @@
 
 #include "../../include/grass.h"
 #include "../../include/grass_ingest.h"
+#include "blob.h"
 #include "codec.h"
 #include "philox.h"
 
@@ -33,10 +34,6 @@ struct grass_csr {
     uint64_t n = 0, m = 0;
     std::vector<uint64_t> rowptr;
     std::vector<uint32_t> col;
-};
-
-struct grass_blob {
-    std::vector<uint8_t> bytes;
 };
 
 struct grass_edges {

@@ -178,20 +178,9 @@ __global__ __launch_bounds__(kExpBlock) void gt_expand_kernel(DevGraph g, const 
             const uint32_t nwords = (uint32_t)(((end - 1) >> 5) - w0 + 1);
             for (uint32_t j = lane; j < kExpWords; j += kWave) B[j] = 0;
             wave_sync();
-            if (act && wdt) {
-                const uint32_t rel = (uint32_t)((at >> 5) - w0), s = (uint32_t)(at & 31);
-                if (s + wdt <= 32) {
-                    atomicOr(&B[rel], raw << (32 - s - wdt));
-                } else {
-                    atomicOr(&B[rel], raw >> (s + wdt - 32));
-                    atomicOr(&B[rel + 1], raw << (64 - s - wdt));
-                }
-            }
+            if (act) put_field(B, w0, at, wdt, raw);
             wave_sync();
-            for (uint32_t j = lane; j < nwords; j += kWave) {
-                if (j == 0 || j == nwords - 1) atomicOr(&out[w0 + j], B[j]);
-                else out[w0 + j] = B[j];
-            }
+            flush_pass(out, B, w0, nwords, lane);
             wave_sync();
         }
         if (lane == 0) {

@@ -1,5 +1,5 @@
-"""Host ingest through libgrass_amd.so (include/grass_ingest.h): R-MAT edge lists, graph_sort
-preprocessing and the LE / GT compressors.  No GPU is needed for these calls."""
+"""Ingest through libgrass_amd.so (include/grass_ingest.h): R-MAT edge lists, graph_sort
+preprocessing and the LE / GT compressors.  Only compress_device needs a GPU."""
 import ctypes as C
 import os
 
@@ -23,6 +23,7 @@ def lib():
             "grass_csr_arrays": (I32, [P, P, P]),
             "grass_csr_free": (None, [P]),
             "grass_compress": (I32, [P, I32, I32, P, P]),
+            "grass_compress_device": (I32, [P, P, U64, I32, I32, P, P, P]),
             "grass_blob_data": (I32, [P, P, P]),
             "grass_blob_free": (None, [P]),
             "grass_write_files": (I32, [C.c_char_p, P, P]),
@@ -84,6 +85,23 @@ class Csr:
             self.close()
         except Exception:
             pass
+
+
+def compress_device(rowptr, col, fmt="le", device=0, stream=None):
+    """(.graph bytes, .offset bytes) of the LE or GT compressor, encoded on the GPU from a CSR in
+    device memory: rowptr (int64/uint64, n + 1) and col (int32/uint32, m) torch tensors on `device`,
+    rows sorted without duplicates as graph_sort writes them."""
+    from . import _stream_ptr
+    n = rowptr.numel() - 1
+    g, o = C.c_void_p(), C.c_void_p()
+    _check(lib().grass_compress_device(C.c_void_p(rowptr.data_ptr()), C.c_void_p(col.data_ptr()), n,
+                                       _FMT[fmt], device, C.c_void_p(_stream_ptr(stream)), C.byref(g),
+                                       C.byref(o)), "grass_compress_device")
+    try:
+        return _blob_bytes(g), _blob_bytes(o)
+    finally:
+        lib().grass_blob_free(g)
+        lib().grass_blob_free(o)
 
 
 def _blob_bytes(h):

@@ -45,6 +45,16 @@ void grass_csr_free(grass_csr *csr);
 int grass_compress(const grass_csr *csr, int format, int nthreads, grass_blob **graph,
                    grass_blob **offset);
 
+/* The same compressors run on a GPU: the CSR (rowptr n + 1 entries, col rowptr[n] entries) is in
+ * device memory of `device`, each row sorted and free of duplicates as grass_graph_sort writes it.
+ * The images are byte-identical to grass_compress's and come back as host blobs; `stream` (a
+ * hipStream_t, NULL for the default stream) orders the work after the producer of the CSR.  The
+ * reference encodes on the CPU only (same compressor.hpp lines as grass_compress); this entry
+ * point has no counterpart there and is for callers whose graph is already on the GPU. */
+int grass_compress_device(const uint64_t *rowptr_dev, const uint32_t *col_dev, uint64_t n,
+                          int format, int device, void *stream, grass_blob **graph,
+                          grass_blob **offset);
+
 int grass_blob_data(const grass_blob *blob, const uint8_t **data, uint64_t *size);
 void grass_blob_free(grass_blob *blob);
 
