@@ -73,6 +73,25 @@ class MdrwOutput(C.Structure):
                 ("steps", C.c_uint64), ("verts", C.c_uint64), ("induced", C.c_uint64)]
 
 
+class KhopBlocks(C.Structure):
+    _fields_ = [("nodes_dev", C.c_void_p), ("node_capacity", C.c_uint64), ("node_off_dev", C.c_void_p),
+                ("lsrc_dev", C.c_void_p), ("ldst_dev", C.c_void_p), ("nodes", C.c_uint64)]
+
+
+# grass_mdrw_array: the arrays of an MDRW result, in grass_mdrw_to_dlpack's numbering
+MDRW_ARRAYS = ("step_off", "step_src", "step_dst", "vert_off", "verts", "ind_off", "ind_src", "ind_dst")
+_DLTENSOR = b"dltensor"  # capsule name a DLPack consumer expects (kept alive with the module)
+
+
+def _from_dlpack(ptr):
+    """torch tensor over a DLManagedTensor* without a copy; torch calls its deleter."""
+    torch = _torch()
+    new = C.pythonapi.PyCapsule_New
+    new.restype = C.py_object
+    new.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
+    return torch.utils.dlpack.from_dlpack(new(ptr, _DLTENSOR, None))
+
+
 class Tuning(C.Structure):
     _fields_ = [("rw_stage", C.c_uint32), ("rw_claim", C.c_uint32), ("khop_lane_scale", C.c_uint32),
                 ("decode_edges_per_wave", C.c_uint32)]
@@ -122,6 +141,9 @@ def lib():
             "grass_mdrw_output_get": (I32, [P, P]),
             "grass_mdrw_fetch": (I32, [P, P, P, P, P, P, P, P, P]),
             "grass_mdrw_free": (None, [P]),
+            "grass_khop_blocks_build": (I32, [P, P, U32, P, P, P]),
+            "grass_mdrw_local_edges": (I32, [P, P, I32, P, P, P]),
+            "grass_mdrw_to_dlpack": (I32, [P, I32, P]),
             "grass_plan_create": (I32, [P, P, U32, P, U32, P, U64, P]),
             "grass_plan_launch": (I32, [P, P]),
             "grass_plan_khop_total": (I32, [P, P]),
@@ -375,6 +397,68 @@ class Graph:
         return {"step_off": so, "vert_off": vo, "ind_off": io, "step_src": ss[: o.steps],
                 "step_dst": sd[: o.steps], "verts": vv[: o.verts], "ind_src": isrc[: o.induced],
                 "ind_dst": idst[: o.induced]}
+
+    def khop_minibatch(self, seeds_list, fanouts_list, seed, instances=None, stream=None):
+        """A k-hop batch and its mini-batch blocks, as device tensors: "src", "dst" (int32, batch
+        edges hop-major), "offsets" (int64 [H][R+1]), "nodes" (int32: each request's vertices by
+        first appearance -- seeds, hop-0 destinations, hop-1 destinations, ...), "node_off"
+        (int64 [R+1]) and "lsrc", "ldst" (int32 local ids of every batch edge in its request)."""
+        torch = _torch()
+        R = len(seeds_list)
+        arr, keep = self.khop_request_array(seeds_list, fanouts_list, instances)
+        cap = self.khop_bound(arr, R)
+        H = max([len(f) for f in fanouts_list] + [0])
+        dev = self._cuda()
+        src = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+        dst = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+        off = torch.zeros(max(H * (R + 1), 1), dtype=torch.int64, device=dev)
+        out = KhopOutput(src.data_ptr(), dst.data_ptr(), cap, off.data_ptr(), 0, 0)
+        sp = C.c_void_p(_stream_ptr(stream))
+        _check(lib().grass_khop_batch(self._h, arr, R, seed, C.byref(out), sp), "grass_khop_batch")
+        total = out.total
+        ncap = sum(int(k.numel()) for k in keep) + total
+        nodes = torch.empty(max(ncap, 1), dtype=torch.int32, device=dev)
+        node_off = torch.empty(R + 1, dtype=torch.int64, device=dev)
+        lsrc = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+        ldst = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+        blk = KhopBlocks(nodes.data_ptr(), ncap, node_off.data_ptr(), lsrc.data_ptr(), ldst.data_ptr(), 0)
+        _check(lib().grass_khop_blocks_build(self._h, arr, R, C.byref(out), C.byref(blk), sp),
+               "grass_khop_blocks_build")
+        self.sync(stream)
+        return {"src": src[:total], "dst": dst[:total], "offsets": off[: H * (R + 1)].view(H, R + 1),
+                "nodes": nodes[: blk.nodes], "node_off": node_off, "lsrc": lsrc[:total],
+                "ldst": ldst[:total], "total": total}
+
+    def mdrw_tensors(self, frontier_sizes, target, seed, instances=None, local=True, stream=None):
+        """Frontier sampling with the result handed to torch through DLPack (no copy): a dict of
+        device tensors named as mdrw()'s arrays (offsets int64, ids int32) and, with local, the
+        local ids "ind_lsrc", "ind_ldst", "step_lsrc", "step_ldst" (ranks in the instance's verts)."""
+        torch = _torch()
+        I = len(frontier_sizes)
+        arr = (MdrwRequest * max(I, 1))()
+        for i, m in enumerate(frontier_sizes):
+            arr[i] = MdrwRequest(int(instances[i]) if instances is not None else i, int(m))
+        res = C.c_void_p()
+        sp = C.c_void_p(_stream_ptr(stream))
+        _check(lib().grass_mdrw(self._h, arr, I, target, seed, sp, C.byref(res)), "grass_mdrw")
+        try:
+            out = {}
+            for k, name in enumerate(MDRW_ARRAYS):
+                mt = C.c_void_p()
+                _check(lib().grass_mdrw_to_dlpack(res, k, C.byref(mt)), "grass_mdrw_to_dlpack")
+                out[name] = _from_dlpack(mt.value)
+            if local:
+                for which, pre in ((0, "ind"), (1, "step")):
+                    n = out[pre + "_src"].numel()
+                    ls = torch.empty(max(n, 1), dtype=torch.int32, device=self._cuda())
+                    ld = torch.empty(max(n, 1), dtype=torch.int32, device=self._cuda())
+                    _check(lib().grass_mdrw_local_edges(self._h, res, which, _dptr(ls), _dptr(ld), sp),
+                           "grass_mdrw_local_edges")
+                    out[pre + "_lsrc"], out[pre + "_ldst"] = ls[:n], ld[:n]
+                self.sync(stream)
+        finally:
+            lib().grass_mdrw_free(res)  # the tensors keep the result's memory alive
+        return out
 
     def tuning(self):
         t = Tuning()

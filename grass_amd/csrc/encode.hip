@@ -305,19 +305,11 @@ __global__ void bswap64_kernel(unsigned long long* w, uint64_t count) {
         w[i] = __builtin_bswap64(w[i]);
 }
 
-struct DevBuf {
-    void* p = nullptr;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, std::max<size_t>(bytes, 256)); }
-};
-
 int compress_device(const uint64_t* rowptr, const uint32_t* col, uint64_t n, int format, hipStream_t s,
                     grass_blob* gb, grass_blob* ob) {
     const bool le = format == GRASS_FMT_LE;
     const uint64_t nb = scan_blocks(n);
-    DevBuf tmp;  // len[n], start[n + 1], widths[n], partials, total, maxlen
+    DevMem tmp;  // len[n], start[n + 1], widths[n], partials, total, maxlen
     Carve need(nullptr);
     auto plan = [&](Carve& c, uint64_t** len, uint64_t** start, uint2** widths, uint64_t** part,
                     unsigned long long** most) {
@@ -353,7 +345,7 @@ int compress_device(const uint64_t* rowptr, const uint32_t* col, uint64_t n, int
     if (total >= (1ull << 40)) return GRASS_EINVAL;  // past what a reader's 40-bit offsets address
     // .graph
     const uint64_t gwords = (total + 31) / 32 + 2;
-    DevBuf gbuf;
+    DevMem gbuf;
     if (gbuf.alloc(gwords * 4) != hipSuccess) return GRASS_ENOMEM;
     uint32_t* d_g = (uint32_t*)gbuf.p;
     e = hipMemsetAsync(d_g, 0, gwords * 4, s);
@@ -370,7 +362,7 @@ int compress_device(const uint64_t* rowptr, const uint32_t* col, uint64_t n, int
     while (ubl < 64 && (maxlen >> ubl)) ++ubl;
     const uint64_t obits = 8 + n * (uint64_t)ubl;
     const uint64_t owords = (obits + 63) / 64 + 1;
-    DevBuf obuf;
+    DevMem obuf;
     if (obuf.alloc(owords * 8) != hipSuccess) return GRASS_ENOMEM;
     unsigned long long* d_o = (unsigned long long*)obuf.p;
     e = hipMemsetAsync(d_o, 0, owords * 8, s);

@@ -1,0 +1,386 @@
+// Sample export: k-hop mini-batch blocks (per-request local numbering), local ids of MDRW
+// samples, and DLPack hand-over of MDRW results.
+//
+// k-hop blocks.  Request r's sequence is its seeds, then its hop-0 destinations, its hop-1
+// destinations, ...; the sequences of all requests are laid end to end (request-major, starts
+// Q[r]) so that one position space covers the batch.
+//   1. rel_len_kernel + scan: Q.
+//   2. rel_insert_kernel: every position p inserts the key (r, vertex) into a hash table and
+//      atomicMins p into the key's slot -- the slot ends up holding the first appearance.
+//   3. rel_mark_kernel: flag[p] = p is its key's first appearance; a scan of the flags gives F,
+//      and the local id of the vertex at p is F[first(p)] - F[Q[r]].
+//   4. rel_emit_kernel: writes each first appearance to nodes[F[p]] and the local ids of every
+//      batch edge (its destination is a position, its source a key looked up in the table).
+// Every step is a data-parallel pass over the positions; the result does not depend on timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <new>
+
+#include "../../include/grass_export.h"
+#include "devutil.h"
+#include "runtime.h"
+#include "scan.h"
+
+namespace grass {
+namespace {
+
+constexpr int kExBlock = 256;
+constexpr unsigned kExGrid = 4096;
+constexpr uint32_t kNoId = 0xFFFFFFFFu;  // local id of a vertex that is not in the sample
+
+struct RelReq {
+    const uint32_t* seeds;
+    uint64_t num;
+};
+
+// Largest r in [0, R) with Q[r] <= p (Q ascending).
+__device__ __forceinline__ uint32_t req_of(const uint64_t* __restrict__ Q, uint32_t R, uint64_t p) {
+    uint32_t lo = 0, hi = R - 1;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo + 1) / 2;
+        if (Q[mid] <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Open-addressing table of {key, first position} pairs, 16 bytes per slot; key 0 = empty.
+struct RelTab {
+    unsigned long long* slot;  // 2 * (mask + 1) words
+    uint64_t mask;
+    uint32_t shift;
+    __device__ __forceinline__ unsigned long long* key(uint64_t h) const { return slot + 2 * h; }
+    __device__ __forceinline__ unsigned long long* val(uint64_t h) const { return slot + 2 * h + 1; }
+    __device__ __forceinline__ uint64_t home(unsigned long long k) const {
+        return ((k * 0x9E3779B97F4A7C15ull) >> shift) & mask;
+    }
+    // First position of key k; ~0 when absent.
+    __device__ __forceinline__ uint64_t first(unsigned long long k) const {
+        uint64_t h = home(k);
+        for (uint64_t probes = 0; probes <= mask; ++probes) {
+            const ulonglong2 kv = *(const ulonglong2*)key(h);
+            if (kv.x == k) return kv.y;
+            if (kv.x == 0) break;
+            h = (h + 1) & mask;
+        }
+        return ~0ull;
+    }
+};
+
+__device__ __forceinline__ unsigned long long rel_key(uint32_t r, uint32_t v) {
+    return (((unsigned long long)r << 32) | v) + 1;
+}
+
+__global__ void rel_clear_kernel(RelTab t) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= t.mask;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        *t.key(i) = 0ull;
+        *t.val(i) = ~0ull;
+    }
+}
+
+// The batch as one position space.
+struct RelSeq {
+    const RelReq* req;
+    const uint64_t* off;  // batch offsets, hop-major [H][R + 1]
+    const uint32_t* src;
+    const uint32_t* dst;
+    const uint64_t* Q;  // R + 1 request starts
+    uint32_t R, H;
+    // Vertex at position p < Q[R]; r = its request, e = its batch edge (~0 for a seed).
+    __device__ __forceinline__ uint32_t at(uint64_t p, uint32_t& r, uint64_t& e) const {
+        r = req_of(Q, R, p);
+        uint64_t j = p - Q[r];
+        const RelReq q = req[r];
+        e = ~0ull;
+        if (j < q.num) return q.seeds[j];
+        j -= q.num;
+        for (uint32_t h = 0; h < H; ++h) {
+            const uint64_t e0 = off[(uint64_t)h * (R + 1) + r];
+            const uint64_t c = off[(uint64_t)h * (R + 1) + r + 1] - e0;
+            if (j < c) {
+                e = e0 + j;
+                return dst[e];
+            }
+            j -= c;
+        }
+        return 0;  // past the request's edges: Q and the offsets disagree (reported by the host)
+    }
+};
+
+__global__ void rel_len_kernel(const RelReq* __restrict__ req, const uint64_t* __restrict__ off, uint32_t R,
+                               uint32_t H, uint64_t* __restrict__ len) {
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < R; r += gridDim.x * blockDim.x) {
+        uint64_t L = req[r].num;
+        for (uint32_t h = 0; h < H; ++h) L += off[(uint64_t)h * (R + 1) + r + 1] - off[(uint64_t)h * (R + 1) + r];
+        len[r] = L;
+    }
+}
+
+struct LenGen {
+    const uint64_t* len;
+    __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return len[i]; }
+};
+
+__global__ __launch_bounds__(kExBlock) void rel_insert_kernel(RelSeq q, uint64_t P, RelTab t, uint32_t* err) {
+    for (uint64_t p = (uint64_t)blockIdx.x * kExBlock + threadIdx.x; p < P; p += (uint64_t)gridDim.x * kExBlock) {
+        uint32_t r;
+        uint64_t e;
+        const uint32_t v = q.at(p, r, e);
+        const unsigned long long k = rel_key(r, v);
+        uint64_t h = t.home(k);
+        bool placed = false;
+        for (uint64_t probes = 0; probes <= t.mask; ++probes) {
+            unsigned long long cur = __atomic_load_n(t.key(h), __ATOMIC_RELAXED);
+            if (cur == 0) {
+                cur = atomicCAS(t.key(h), 0ull, k);
+                if (cur == 0) cur = k;
+            }
+            if (cur == k) {
+                placed = true;
+                break;
+            }
+            h = (h + 1) & t.mask;
+        }
+        if (placed) atomicMin(t.val(h), (unsigned long long)p);
+        else atomicCAS(err, kErrNone, kErrBadNeighbor);  // cannot happen: slots > 2 positions
+    }
+}
+
+__global__ __launch_bounds__(kExBlock) void rel_mark_kernel(RelSeq q, uint64_t P, RelTab t,
+                                                            uint32_t* __restrict__ flag) {
+    for (uint64_t p = (uint64_t)blockIdx.x * kExBlock + threadIdx.x; p < P; p += (uint64_t)gridDim.x * kExBlock) {
+        uint32_t r;
+        uint64_t e;
+        const uint32_t v = q.at(p, r, e);
+        flag[p] = t.first(rel_key(r, v)) == p;
+    }
+}
+
+struct FlagGen {
+    const uint32_t* flag;
+    __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return flag[i]; }
+};
+
+__global__ __launch_bounds__(kExBlock) void rel_emit_kernel(RelSeq q, uint64_t P, RelTab t,
+                                                            const uint32_t* __restrict__ flag,
+                                                            const uint64_t* __restrict__ F,
+                                                            uint32_t* __restrict__ nodes, uint64_t cap,
+                                                            uint32_t* __restrict__ lsrc,
+                                                            uint32_t* __restrict__ ldst, uint32_t* err) {
+    for (uint64_t p = (uint64_t)blockIdx.x * kExBlock + threadIdx.x; p < P; p += (uint64_t)gridDim.x * kExBlock) {
+        uint32_t r;
+        uint64_t e;
+        const uint32_t v = q.at(p, r, e);
+        if (flag[p] && F[p] < cap) nodes[F[p]] = v;
+        if (e == ~0ull) continue;
+        const uint64_t base = F[q.Q[r]];
+        const uint64_t fd = flag[p] ? p : t.first(rel_key(r, v));
+        const uint64_t fs = t.first(rel_key(r, q.src[e]));
+        if (fd >= P || fs >= P) {  // a source that is neither a seed nor an earlier destination
+            atomicCAS(err, kErrNone, kErrBadNeighbor);
+            ldst[e] = lsrc[e] = kNoId;
+            continue;
+        }
+        ldst[e] = (uint32_t)(F[fd] - base);
+        lsrc[e] = (uint32_t)(F[fs] - base);
+    }
+}
+
+__global__ void rel_node_off_kernel(const uint64_t* __restrict__ Q, const uint64_t* __restrict__ F, uint32_t R,
+                                    uint64_t* __restrict__ node_off) {
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r <= R; r += gridDim.x * blockDim.x) node_off[r] = F[Q[r]];
+}
+
+int khop_blocks(grass_graph* g, const grass_khop_request* reqs, uint32_t R, const grass_khop_output* b,
+                grass_khop_blocks* out, hipStream_t s) {
+    std::lock_guard<std::mutex> lock(g->mu);
+    Workspace& ws = g->ws;
+    out->nodes = 0;
+    const uint32_t H = b->max_hops;
+    if (H > GRASS_KHOP_MAX_HOPS || (H && !b->offsets_dev) || b->total > b->capacity || !out->node_off_dev)
+        return GRASS_EINVAL;
+    if (b->total && (!b->src_dev || !b->dst_dev || !out->lsrc_dev || !out->ldst_dev)) return GRASS_EINVAL;
+    if (ws.reserve_stage(R * sizeof(RelReq) + 64) != hipSuccess) return GRASS_ENOMEM;
+    RelReq* hreq = (RelReq*)ws.stage();
+    uint64_t S = 0;
+    for (uint32_t r = 0; r < R; ++r) {
+        if (reqs[r].hops > H || (reqs[r].num_seeds && !reqs[r].seeds_dev)) return GRASS_EINVAL;
+        hreq[r] = RelReq{reqs[r].seeds_dev, reqs[r].num_seeds};
+        S += reqs[r].num_seeds;
+    }
+    const uint64_t P = S + b->total;
+    if (P && (!out->nodes_dev && out->node_capacity)) return GRASS_EINVAL;
+    uint64_t slots = 1024;
+    while (slots < 2 * P + 2) slots <<= 1;
+    const uint64_t nb = scan_blocks(std::max<uint64_t>(std::max<uint64_t>(P, R), 1));
+    const size_t need = Carve::need(R * sizeof(RelReq)) + Carve::need(R * 8) + Carve::need((R + 1) * 8) +
+                        Carve::need(P * 4 + 4) + Carve::need((P + 1) * 8) + Carve::need((nb + 2) * 8) +
+                        Carve::need(64);
+    if (ws.scratch.reserve(need) != hipSuccess || ws.ht.reserve(slots * 16) != hipSuccess) return GRASS_ENOMEM;
+    // this overwrites the k-hop dedup table's memory: the next k-hop batch clears it first
+    ws.ht_gen = 0xFFFF;
+    Carve cv(ws.scratch.ptr);
+    RelReq* d_req = cv.take<RelReq>(R);
+    uint64_t* d_len = cv.take<uint64_t>(R);
+    uint64_t* d_Q = cv.take<uint64_t>(R + 1);
+    uint32_t* d_flag = cv.take<uint32_t>(P + 1);
+    uint64_t* d_F = cv.take<uint64_t>(P + 1);
+    uint64_t* d_part = cv.take<uint64_t>(nb + 2);
+    uint64_t* d_sc = cv.take<uint64_t>(8);  // [0] sequence total, [1] node total
+    RelTab t;
+    t.slot = (unsigned long long*)ws.ht.ptr;
+    t.mask = slots - 1;
+    uint32_t lg = 0;
+    while ((1ull << lg) < slots) ++lg;
+    t.shift = 64 - lg;
+
+    hipError_t e = hipMemcpyAsync(d_req, hreq, R * sizeof(RelReq), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return status_of(e);
+    const unsigned gR = (unsigned)std::min<uint64_t>((R + 255) / 256 + 1, 1024);
+    hipLaunchKernelGGL(rel_len_kernel, dim3(gR), dim3(256), 0, s, (const RelReq*)d_req,
+                       (const uint64_t*)b->offsets_dev, R, H, d_len);
+    e = exclusive_scan(LenGen{d_len}, Count{R, nullptr}, R, d_Q, d_part, d_sc, s);
+    if (e != hipSuccess) return status_of(e);
+    RelSeq q{d_req, b->offsets_dev, b->src_dev, b->dst_dev, d_Q, R, H};
+    const unsigned gP = (unsigned)std::min<uint64_t>((P + kExBlock - 1) / kExBlock + 1, kExGrid);
+    hipLaunchKernelGGL(rel_clear_kernel, dim3(kExGrid), dim3(256), 0, s, t);
+    if (P) {
+        hipLaunchKernelGGL(rel_insert_kernel, dim3(gP), dim3(kExBlock), 0, s, q, P, t, g->d_err);
+        hipLaunchKernelGGL(rel_mark_kernel, dim3(gP), dim3(kExBlock), 0, s, q, P, t, d_flag);
+    }
+    e = exclusive_scan(FlagGen{d_flag}, Count{P, nullptr}, P, d_F, d_part, d_sc + 1, s);
+    if (e != hipSuccess) return status_of(e);
+    if (P)
+        hipLaunchKernelGGL(rel_emit_kernel, dim3(gP), dim3(kExBlock), 0, s, q, P, t, (const uint32_t*)d_flag,
+                           (const uint64_t*)d_F, out->nodes_dev, out->node_capacity, out->lsrc_dev, out->ldst_dev,
+                           g->d_err);
+    hipLaunchKernelGGL(rel_node_off_kernel, dim3(gR), dim3(256), 0, s, (const uint64_t*)d_Q, (const uint64_t*)d_F, R,
+                       out->node_off_dev);
+    uint64_t* hres = ws.result_slot();
+    e = hipMemcpyAsync(hres, d_Q + R, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(hres + 1, d_F + P, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return status_of(e);
+    if (hres[0] != P) return GRASS_EINVAL;  // the offsets are not the batch of these requests
+    out->nodes = hres[1];
+    return out->nodes > out->node_capacity ? GRASS_ECAPACITY : GRASS_OK;
+}
+
+// ------------------------------------------------------------------ MDRW local ids
+
+// Rank of x in the ascending a[0, n); kNoId when absent.
+__device__ __forceinline__ uint32_t rank_in(const uint32_t* __restrict__ a, uint64_t n, uint32_t x) {
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (a[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n && a[lo] == x ? (uint32_t)lo : kNoId;
+}
+
+__global__ __launch_bounds__(kExBlock) void mdrw_local_kernel(const uint64_t* __restrict__ eoff,
+                                                              const uint32_t* __restrict__ esrc,
+                                                              const uint32_t* __restrict__ edst, uint64_t E,
+                                                              uint32_t I, const uint64_t* __restrict__ voff,
+                                                              const uint32_t* __restrict__ verts,
+                                                              uint32_t* __restrict__ lsrc,
+                                                              uint32_t* __restrict__ ldst) {
+    for (uint64_t e = (uint64_t)blockIdx.x * kExBlock + threadIdx.x; e < E; e += (uint64_t)gridDim.x * kExBlock) {
+        const uint32_t i = req_of(eoff, I, e);
+        const uint32_t* V = verts + voff[i];
+        const uint64_t nv = voff[i + 1] - voff[i];
+        lsrc[e] = rank_in(V, nv, esrc[e]);
+        ldst[e] = rank_in(V, nv, edst[e]);
+    }
+}
+
+// ------------------------------------------------------------------ DLPack
+
+struct DlHold {
+    grass_dl_managed_tensor mt;
+    int64_t shape[1];
+    grass_mdrw_result* res;
+};
+
+void dl_release(grass_dl_managed_tensor* self) {
+    if (!self) return;
+    DlHold* h = (DlHold*)self->manager_ctx;
+    grass_mdrw_result* res = h->res;
+    delete h;
+    res->release();
+}
+
+}  // namespace
+}  // namespace grass
+
+extern "C" {
+
+int grass_khop_blocks_build(grass_graph* g, const grass_khop_request* reqs, uint32_t count,
+                            const grass_khop_output* batch, grass_khop_blocks* out, void* stream) {
+    if (!g || !batch || !out || (count && !reqs) || count > (1u << 17)) return GRASS_EINVAL;
+    if (count == 0) {
+        out->nodes = 0;
+        return GRASS_OK;
+    }
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    if (hipSetDevice(g->device) != hipSuccess) return GRASS_EDEVICE;
+    hipStream_t s = stream ? (hipStream_t)stream : g->stream;
+    int rc = grass::khop_blocks(g, reqs, count, batch, out, s);
+    (void)hipSetDevice(prev);
+    return rc;
+}
+
+int grass_mdrw_local_edges(grass_graph* g, const grass_mdrw_result* res, int which, uint32_t* lsrc_dev,
+                           uint32_t* ldst_dev, void* stream) {
+    if (!g || !res || (which != GRASS_MDRW_EDGES_INDUCED && which != GRASS_MDRW_EDGES_STEPS)) return GRASS_EINVAL;
+    const grass_mdrw_output& o = res->out;
+    const bool ind = which == GRASS_MDRW_EDGES_INDUCED;
+    const uint64_t E = ind ? o.induced : o.steps;
+    if (E == 0 || o.count == 0) return GRASS_OK;
+    if (!lsrc_dev || !ldst_dev) return GRASS_EINVAL;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    if (hipSetDevice(g->device) != hipSuccess) return GRASS_EDEVICE;
+    hipStream_t s = stream ? (hipStream_t)stream : g->stream;
+    const unsigned grid = (unsigned)std::min<uint64_t>((E + grass::kExBlock - 1) / grass::kExBlock, grass::kExGrid);
+    hipLaunchKernelGGL(grass::mdrw_local_kernel, dim3(grid), dim3(grass::kExBlock), 0, s,
+                       ind ? o.ind_off_dev : o.step_off_dev, ind ? o.ind_src_dev : o.step_src_dev,
+                       ind ? o.ind_dst_dev : o.step_dst_dev, E, o.count, o.vert_off_dev, o.verts_dev, lsrc_dev,
+                       ldst_dev);
+    const int rc = grass::status_of(hipGetLastError());
+    (void)hipSetDevice(prev);
+    return rc;
+}
+
+int grass_mdrw_to_dlpack(grass_mdrw_result* res, int which, grass_dl_managed_tensor** out) {
+    if (!res || !out || which < GRASS_MDRW_STEP_OFF || which > GRASS_MDRW_IND_DST) return GRASS_EINVAL;
+    const grass_mdrw_output& o = res->out;
+    const uint64_t no = (uint64_t)o.count + 1;
+    const void* data[8] = {o.step_off_dev, o.step_src_dev, o.step_dst_dev, o.vert_off_dev,
+                           o.verts_dev,    o.ind_off_dev,  o.ind_src_dev,  o.ind_dst_dev};
+    const uint64_t len[8] = {no, o.steps, o.steps, no, o.verts, no, o.induced, o.induced};
+    const bool offs = which == GRASS_MDRW_STEP_OFF || which == GRASS_MDRW_VERT_OFF || which == GRASS_MDRW_IND_OFF;
+    grass::DlHold* h = new (std::nothrow) grass::DlHold();
+    if (!h) return GRASS_ENOMEM;
+    h->shape[0] = (int64_t)len[which];
+    grass_dl_tensor& t = h->mt.dl_tensor;
+    t.data = const_cast<void*>(data[which]);
+    t.device = grass_dl_device{10, res->device};
+    t.ndim = 1;
+    t.dtype = grass_dl_dtype{0, (uint8_t)(offs ? 64 : 32), 1};
+    t.shape = h->shape;
+    t.strides = nullptr;
+    t.byte_offset = 0;
+    h->mt.manager_ctx = h;
+    h->mt.deleter = grass::dl_release;
+    res->refs.fetch_add(1, std::memory_order_relaxed);
+    h->res = res;
+    *out = &h->mt;
+    return GRASS_OK;
+}
+
+}  // extern "C"

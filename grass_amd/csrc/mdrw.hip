@@ -503,25 +503,6 @@ __global__ void mdrw_ind_off_kernel(const uint64_t* __restrict__ voff, const uin
 
 // ---------------------------------------------------------------------------- host
 
-struct DevMem {  // owned device allocation
-    void* p = nullptr;
-    ~DevMem() {
-        if (p) (void)hipFree(p);
-    }
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, std::max<size_t>(bytes, 256)); }
-    template <typename T>
-    T* as() const { return (T*)p; }
-};
-
-}  // namespace grass
-
-struct grass_mdrw_result {
-    grass_mdrw_output out{};
-    grass::DevMem steps, verts, ind, offs;
-};
-
-namespace grass {
-
 static uint32_t next_pow2(uint32_t x) {
     uint32_t p = 1;
     while (p < x) p <<= 1;
@@ -563,6 +544,7 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
     }
     std::unique_ptr<grass_mdrw_result> res(new (std::nothrow) grass_mdrw_result());
     if (!res) return GRASS_ENOMEM;
+    res->device = g->device;
     res->out.count = I;
     if (res->offs.alloc(3 * (uint64_t)(I + 1) * 8) != hipSuccess) return GRASS_ENOMEM;
     uint64_t* step_off = res->offs.as<uint64_t>();
@@ -832,11 +814,7 @@ int grass_mdrw_fetch(const grass_mdrw_result* res, uint64_t* step_off, uint32_t*
 }
 
 void grass_mdrw_free(grass_mdrw_result* res) {
-    if (!res) return;
-    int prev = 0;
-    (void)hipGetDevice(&prev);
-    delete res;
-    (void)hipSetDevice(prev);
+    if (res) res->release();
 }
 
 }  // extern "C"

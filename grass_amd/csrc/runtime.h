@@ -60,6 +60,20 @@ struct PinnedBuf {
     }
 };
 
+// An owned device allocation, freed with its owner.
+struct DevMem {
+    void* p = nullptr;
+    DevMem() = default;
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    ~DevMem() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes > 256 ? bytes : 256); }
+    template <typename T>
+    T* as() const { return (T*)p; }
+};
+
 // Working memory of one in-flight batch: device scratch for descriptors, frontiers and scans,
 // a private k-hop dedup table (generation-tagged, so it is reused across batches without
 // clearing) and pinned host memory whose first kResultBytes receive the batch's results (the
@@ -150,6 +164,18 @@ struct grass_graph {
         d.fmt = fmt;
         d.cp_shift = cp_shift;
         return d;
+    }
+};
+
+// An MDRW batch's output.  Its owner and every exported DLPack tensor of it hold a reference;
+// the device memory goes with the last one.
+struct grass_mdrw_result {
+    grass_mdrw_output out{};
+    grass::DevMem steps, verts, ind, offs;
+    int device = 0;
+    std::atomic<int> refs{1};
+    void release() {
+        if (refs.fetch_sub(1, std::memory_order_acq_rel) == 1) delete this;
     }
 };
 

@@ -43,13 +43,6 @@ grass_tuning current(const grass_graph* g) {
     return t;
 }
 
-struct DevMem {
-    void* p = nullptr;
-    ~DevMem() {
-        if (p) (void)hipFree(p);
-    }
-};
-
 // Best of `reps` timed runs of fn (after one warm-up run), in milliseconds; < 0 on an error.
 template <class F>
 double time_best(hipStream_t s, int reps, F fn) {

@@ -39,3 +39,13 @@ def test_strerror_and_null_handles():
     assert L.grass_open(None, 0, 0, C.byref(h)) == grass_amd.EINVAL
     assert L.grass_open(b"/nonexistent/x", 0, 0, C.byref(h)) == grass_amd.EIO
     assert L.grass_open(b"/nonexistent/x", 7, 0, C.byref(h)) in (grass_amd.EINVAL, grass_amd.EIO)
+
+
+def test_export_argument_errors():
+    import grass_amd
+    L = grass_amd.lib()
+    out = C.c_void_p()
+    assert L.grass_mdrw_to_dlpack(None, 0, C.byref(out)) == grass_amd.EINVAL
+    assert L.grass_khop_blocks_build(None, None, 0, None, None, None) == grass_amd.EINVAL
+    assert L.grass_mdrw_local_edges(None, None, 0, None, None, None) == grass_amd.EINVAL
+    assert L.grass_compress_device(None, None, 0, 0, 0, None, None, None) == grass_amd.EINVAL

@@ -1,0 +1,136 @@
+"""Sample export: k-hop mini-batch blocks (each request's vertices numbered by first appearance
+over seeds, hop-0 destinations, hop-1 destinations, ...; every batch edge in those local ids),
+local ids of MDRW samples, and MDRW results handed to torch through DLPack without a copy."""
+import numpy as np
+import pytest
+
+from conftest import load_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ga():
+    import torch
+
+    import grass_amd
+    assert torch.cuda.is_available()
+    grass_amd.lib()
+    return grass_amd
+
+
+def _np(t):
+    return t.cpu().numpy()
+
+
+def blocks_by_dict(seeds_list, offsets, src, dst):
+    """The numbering restated with a dict per request: first appearance in the sequence seeds,
+    hop-0 destinations, hop-1 destinations, ..."""
+    R, H = len(seeds_list), offsets.shape[0]
+    nodes, node_off = [], [0]
+    lsrc = np.full(len(src), -1, np.int64)
+    ldst = np.full(len(src), -1, np.int64)
+    for r in range(R):
+        ids = {}
+        for v in seeds_list[r]:
+            ids.setdefault(int(v), len(ids))
+        for h in range(H):
+            for e in range(int(offsets[h, r]), int(offsets[h, r + 1])):
+                assert int(src[e]) in ids  # a source is a seed or an earlier destination
+                lsrc[e] = ids[int(src[e])]
+                ldst[e] = ids.setdefault(int(dst[e]), len(ids))
+        nodes.extend(ids.keys())
+        node_off.append(len(nodes))
+    return np.array(nodes, np.int64), np.array(node_off, np.int64), lsrc, ldst
+
+
+def _check_blocks(res, seeds_list):
+    off = _np(res["offsets"])
+    src, dst = _np(res["src"]).view(np.uint32), _np(res["dst"]).view(np.uint32)
+    nodes, node_off, lsrc, ldst = blocks_by_dict(seeds_list, off, src, dst)
+    np.testing.assert_array_equal(_np(res["node_off"]), node_off)
+    np.testing.assert_array_equal(_np(res["nodes"]).view(np.uint32).astype(np.int64), nodes)
+    np.testing.assert_array_equal(_np(res["lsrc"]).astype(np.int64), lsrc)
+    np.testing.assert_array_equal(_np(res["ldst"]).astype(np.int64), ldst)
+
+
+@pytest.mark.parametrize("fmt", ["le", "gt"])
+def test_khop_blocks_small(ga, oracle_mod, fmt):
+    print(f"[export khop blocks] rmat12 fmt={fmt}", flush=True)
+    c = load_case("rmat12")
+    gb, ob = c[f"{fmt}_graph"].tobytes(), c[f"{fmt}_offset"].tobytes()
+    g = ga.Graph.from_bytes(gb, ob, fmt)
+    rng = np.random.default_rng(4)
+    seeds = [rng.integers(0, g.n, size=k).astype(np.uint32) for k in (1, 40, 300, 7, 2)]
+    seeds[3] = np.array([5, 5, 9, 5, 9], np.uint32)  # duplicate seeds keep their first local id
+    seeds[4] = np.zeros(0, np.uint32)                # an empty request
+    fans = [[25, 10], [10, 10, 5], [64], [3, 3], [4]]
+    res = g.khop_minibatch(seeds, fans, seed=5, instances=[1, 2, 3, 4, 5])
+    # the batch part is the plain k-hop batch, which equals the oracle
+    kexp = oracle_mod.Graph(gb, ob, fmt).khop(seeds, fans, seed=5, instances=[1, 2, 3, 4, 5])
+    assert res["total"] == len(kexp["dst"])
+    _check_blocks(res, seeds)
+    # the distinct seeds come first, in order
+    n0 = _np(res["node_off"])
+    np.testing.assert_array_equal(_np(res["nodes"])[n0[3]:n0[3] + 2], [5, 9])
+    assert n0[5] == n0[4]
+    g.close()
+
+
+def test_khop_blocks_rmat18(ga):
+    """A serving-sized batch (64 requests of 200 seeds, [25, 10]) on R-MAT-18, checked against
+    the dict numbering and the invariants nodes[local] == global id, one local id per vertex."""
+    from grass_amd import ingest
+    src, dst = ingest.rmat_edges(18, 16, seed=18, threads=16)
+    rowptr, col = ingest.graph_sort(src, dst, threads=16).arrays()
+    import torch
+    rp = torch.from_numpy(rowptr.astype(np.int64)).cuda()
+    cl = torch.from_numpy(col.view(np.int32)).cuda()
+    gb, ob = ingest.compress_device(rp, cl, "gt")
+    g = ga.Graph.from_bytes(gb, ob, "gt")
+    rng = np.random.default_rng(18)
+    seeds = [rng.integers(0, g.n, size=200).astype(np.uint32) for _ in range(64)]
+    res = g.khop_minibatch(seeds, [[25, 10]] * 64, seed=7)
+    print(f"[export khop blocks] rmat18 edges={res['total']} nodes={res['nodes'].numel()}", flush=True)
+    _check_blocks(res, seeds)
+    nodes = _np(res["nodes"]).view(np.uint32)
+    n0 = _np(res["node_off"])
+    off = _np(res["offsets"])
+    for r in (0, 31, 63):
+        seg = nodes[n0[r]:n0[r + 1]]
+        assert len(np.unique(seg)) == len(seg)
+        for h in range(off.shape[0]):
+            a, b = off[h, r], off[h, r + 1]
+            np.testing.assert_array_equal(seg[_np(res["lsrc"])[a:b]], _np(res["src"]).view(np.uint32)[a:b])
+            np.testing.assert_array_equal(seg[_np(res["ldst"])[a:b]], _np(res["dst"]).view(np.uint32)[a:b])
+    g.close()
+
+
+@pytest.mark.parametrize("fmt", ["le", "gt"])
+def test_mdrw_dlpack_and_local_ids(ga, fmt):
+    print(f"[export mdrw dlpack] rmat12 fmt={fmt}", flush=True)
+    import torch
+    c = load_case("rmat12")
+    g = ga.Graph.from_bytes(c[f"{fmt}_graph"].tobytes(), c[f"{fmt}_offset"].tobytes(), fmt)
+    sizes = [5, 60, 200, 1]
+    host = g.mdrw(sizes, target=400, seed=9)
+    dev = g.mdrw_tensors(sizes, target=400, seed=9)  # the result was freed; the tensors hold it
+    torch.cuda.synchronize()
+    for k in ga.MDRW_ARRAYS:
+        t = dev[k]
+        assert t.is_cuda and t.dtype == (torch.int64 if k.endswith("_off") else torch.int32), k
+        np.testing.assert_array_equal(_np(t).view(np.uint64 if k.endswith("_off") else np.uint32), host[k],
+                                      err_msg=k)
+    for pre in ("ind", "step"):
+        off = host[pre + "_off"]
+        for i in range(len(sizes)):
+            V = host["verts"][host["vert_off"][i]:host["vert_off"][i + 1]]
+            a, b = int(off[i]), int(off[i + 1])
+            np.testing.assert_array_equal(_np(dev[pre + "_lsrc"])[a:b], np.searchsorted(V, host[pre + "_src"][a:b]))
+            np.testing.assert_array_equal(_np(dev[pre + "_ldst"])[a:b], np.searchsorted(V, host[pre + "_dst"][a:b]))
+            assert (V[_np(dev[pre + "_lsrc"])[a:b]] == host[pre + "_src"][a:b]).all()
+    # the memory goes with the last tensor: drop them and run another batch over the same sizes
+    del dev
+    again = g.mdrw_tensors(sizes, target=400, seed=9, local=False)
+    np.testing.assert_array_equal(_np(again["ind_dst"]).view(np.uint32), host["ind_dst"])
+    g.close()
