@@ -1,8 +1,12 @@
 // C ABI entry points of libgrass_amd.so (declared in include/grass.h).
+#include <fcntl.h>
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <new>
 #include <string>
@@ -22,21 +26,42 @@ int status_of(hipError_t e) {
 
 namespace {
 
+// A file mapped read-only, so that the loader copies from the page cache with several threads;
+// read() (pread) serves files that cannot be mapped.
 struct FileSource : ByteSource {
-    FILE* f = nullptr;
-    uint64_t sz = 0;
+    int fd = -1;
+    uint64_t sz = 0, pos = 0;
+    const uint8_t* map = nullptr;
     explicit FileSource(const std::string& path) {
-        f = fopen(path.c_str(), "rb");
-        if (f && fseeko(f, 0, SEEK_END) == 0) {
-            sz = (uint64_t)ftello(f);
-            fseeko(f, 0, SEEK_SET);
+        fd = ::open(path.c_str(), O_RDONLY | O_CLOEXEC);
+        struct stat st;
+        if (fd < 0 || fstat(fd, &st) != 0) return;
+        sz = (uint64_t)st.st_size;
+        if (sz == 0) return;
+        void* p = mmap(nullptr, sz, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (p != MAP_FAILED) {
+            map = (const uint8_t*)p;
+            (void)madvise(p, sz, MADV_SEQUENTIAL | MADV_WILLNEED);
         }
     }
     ~FileSource() override {
-        if (f) fclose(f);
+        if (map) munmap((void*)map, sz);
+        if (fd >= 0) ::close(fd);
     }
+    bool ok() const { return fd >= 0; }
     uint64_t size() const override { return sz; }
-    bool read(void* dst, uint64_t len) override { return f && fread(dst, 1, len, f) == len; }
+    const uint8_t* data() const override { return map; }
+    bool read(void* dst, uint64_t len) override {
+        uint8_t* d = (uint8_t*)dst;
+        while (len) {
+            const ssize_t got = pread(fd, d, len, (off_t)pos);
+            if (got <= 0) return false;
+            d += got;
+            pos += (uint64_t)got;
+            len -= (uint64_t)got;
+        }
+        return true;
+    }
 };
 
 struct MemSource : ByteSource {
@@ -44,6 +69,7 @@ struct MemSource : ByteSource {
     uint64_t sz, pos = 0;
     MemSource(const void* ptr, uint64_t n) : p((const uint8_t*)ptr), sz(n) {}
     uint64_t size() const override { return sz; }
+    const uint8_t* data() const override { return p; }
     bool read(void* dst, uint64_t len) override {
         if (pos + len > sz) return false;
         memcpy(dst, p + pos, len);
@@ -142,7 +168,7 @@ int grass_open(const char* prefix, int format, int device, grass_graph** out) {
     if (!prefix || !out) return GRASS_EINVAL;
     std::string p(prefix);
     FileSource gs(p + ".graph"), os(p + ".offset");
-    if (!gs.f || !os.f) {
+    if (!gs.ok() || !os.ok()) {
         *out = nullptr;
         return GRASS_EIO;
     }

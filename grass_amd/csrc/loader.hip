@@ -7,9 +7,11 @@
 // Replaces loadGraph (LE/src/grass.cu:1015-1095), copyPartitionToGPU (LE/src/utils.hpp:328-353)
 // with its decode_offset kernel (LE/src/bcg.cuh:284-329) and CUB scan.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "devutil.h"
@@ -195,8 +197,40 @@ namespace {
 
 constexpr uint64_t kChunk = 64ull << 20;  // pinned staging chunk
 
-// Streams `src` into device memory through two pinned chunks so that the read of chunk i+1
-// overlaps the DMA of chunk i.
+// Host threads that copy a chunk into pinned memory: GRASS_LOAD_THREADS in [1, 64], else up to 8.
+unsigned load_threads() {
+    if (const char* e = getenv("GRASS_LOAD_THREADS")) {
+        const unsigned long t = strtoul(e, nullptr, 10);
+        if (t >= 1 && t <= 64) return (unsigned)t;
+    }
+    return std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+}
+
+// memcpy split over up to `threads` threads (slices of at least 4 MB); a thread that cannot be
+// started leaves its slice to the calling thread.
+void parallel_copy(void* dst, const uint8_t* src, uint64_t len, unsigned threads) {
+    const uint64_t t = std::min<uint64_t>(threads, std::max<uint64_t>(1, len >> 22));
+    if (t <= 1) {
+        memcpy(dst, src, len);
+        return;
+    }
+    const uint64_t step = ((len + t - 1) / t + 4095) & ~4095ull;
+    std::vector<std::thread> pool;
+    std::vector<uint64_t> mine = {0};
+    for (uint64_t lo = step; lo < len; lo += step) {
+        const uint64_t n = std::min(step, len - lo);
+        try {
+            pool.emplace_back([=] { memcpy((uint8_t*)dst + lo, src + lo, n); });
+        } catch (...) {
+            mine.push_back(lo);
+        }
+    }
+    for (uint64_t lo : mine) memcpy((uint8_t*)dst + lo, src + lo, std::min(step, len - lo));
+    for (std::thread& th : pool) th.join();
+}
+
+// Streams `src` into device memory through two pinned chunks so that the copy of chunk i+1
+// overlaps the DMA of chunk i.  A source held in host memory is copied with several threads.
 int stage_to_device(ByteSource& src, char* dst, hipStream_t stream, uint64_t* tail,
                     unsigned tail_len, uint8_t* tail_out) {
     const uint64_t total = src.size();
@@ -213,10 +247,13 @@ int stage_to_device(ByteSource& src, char* dst, hipStream_t stream, uint64_t* ta
     uint64_t off = 0;
     int slot = 0;
     bool used[2] = {false, false};
+    const uint8_t* mem = src.data();
+    const unsigned threads = load_threads();
     while (rc == GRASS_OK && off < total) {
         uint64_t len = std::min(chunk, total - off);
         if (used[slot] && hipEventSynchronize(ev[slot]) != hipSuccess) { rc = GRASS_EDEVICE; break; }
-        if (!src.read(pin[slot], len)) { rc = GRASS_EIO; break; }
+        if (mem) parallel_copy(pin[slot], mem + off, len, threads);
+        else if (!src.read(pin[slot], len)) { rc = GRASS_EIO; break; }
         // keep the last tail_len bytes of the stream on the host
         if (tail_out && off + len > total - std::min<uint64_t>(tail_len, total)) {
             uint64_t t0 = total - std::min<uint64_t>(tail_len, total);

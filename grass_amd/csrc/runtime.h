@@ -198,6 +198,9 @@ inline unsigned persistent_grid(const void* kernel, int block) {
 struct ByteSource {
     virtual ~ByteSource() {}
     virtual uint64_t size() const = 0;
+    // The whole stream in host memory (a mapping or a caller's buffer), or nullptr when it can
+    // only be read in order.
+    virtual const uint8_t* data() const { return nullptr; }
     // Reads len bytes at the current position; false on a short read.
     virtual bool read(void* dst, uint64_t len) = 0;
 };
