@@ -2,8 +2,19 @@
 #pragma once
 #include <stdint.h>
 
-#include <vector>
+#include <memory>
+#include <new>
 
 struct grass_blob {
-    std::vector<uint8_t> bytes;
+    std::unique_ptr<uint8_t[]> buf;
+    uint64_t len = 0;
+    // Storage for n bytes, left uninitialised (every writer fills all of it); false when it
+    // cannot be allocated.
+    bool resize(uint64_t n) {
+        buf.reset(new (std::nothrow) uint8_t[n ? n : 1]);
+        len = buf ? n : 0;
+        return buf != nullptr;
+    }
+    uint8_t* data() const { return buf.get(); }
+    uint64_t size() const { return len; }
 };

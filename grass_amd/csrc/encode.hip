@@ -305,6 +305,46 @@ __global__ void bswap64_kernel(unsigned long long* w, uint64_t count) {
         w[i] = __builtin_bswap64(w[i]);
 }
 
+// Device -> host through two pinned chunks: the threaded copy out of one chunk overlaps the DMA
+// into the other.
+int fetch_to_host(uint8_t* dst, const void* src, uint64_t len, hipStream_t s) {
+    if (len == 0) return GRASS_OK;
+    const uint64_t chunk = std::min<uint64_t>(64ull << 20, len);
+    void* pin[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int rc = GRASS_OK;
+    for (int i = 0; i < 2 && !rc; ++i) {
+        if (hipHostMalloc(&pin[i], chunk, hipHostMallocDefault) != hipSuccess) rc = GRASS_ENOMEM;
+        else if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) rc = GRASS_EDEVICE;
+    }
+    const unsigned threads = load_threads();
+    const uint64_t nch = (len + chunk - 1) / chunk;
+    auto issue = [&](uint64_t c) {
+        const uint64_t off = c * chunk, n = std::min(chunk, len - off);
+        return hipMemcpyAsync(pin[c & 1], (const char*)src + off, n, hipMemcpyDeviceToHost, s) == hipSuccess &&
+               hipEventRecord(ev[c & 1], s) == hipSuccess;
+    };
+    if (!rc && !issue(0)) rc = GRASS_EDEVICE;
+    for (uint64_t c = 0; c < nch && !rc; ++c) {
+        if (c + 1 < nch && !issue(c + 1)) {
+            rc = GRASS_EDEVICE;
+            break;
+        }
+        if (hipEventSynchronize(ev[c & 1]) != hipSuccess) {
+            rc = GRASS_EDEVICE;
+            break;
+        }
+        const uint64_t off = c * chunk;
+        parallel_copy(dst + off, (const uint8_t*)pin[c & 1], std::min(chunk, len - off), threads);
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = GRASS_EDEVICE;
+    for (int i = 0; i < 2; ++i) {
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+        if (pin[i]) (void)hipHostFree(pin[i]);
+    }
+    return rc;
+}
+
 int compress_device(const uint64_t* rowptr, const uint32_t* col, uint64_t n, int format, hipStream_t s,
                     grass_blob* gb, grass_blob* ob) {
     const bool le = format == GRASS_FMT_LE;
@@ -374,16 +414,9 @@ int compress_device(const uint64_t* rowptr, const uint32_t* col, uint64_t n, int
     hipLaunchKernelGGL(bswap64_kernel, dim3(256), dim3(256), 0, s, d_o, owords);
     e = hipGetLastError();
     if (e != hipSuccess) return status_of(e);
-    try {
-        gb->bytes.resize((total + 7) / 8);
-        ob->bytes.resize((obits + 7) / 8);
-    } catch (const std::bad_alloc&) {
-        return GRASS_ENOMEM;
-    }
-    e = hipMemcpyAsync(gb->bytes.data(), d_g, gb->bytes.size(), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(ob->bytes.data(), d_o, ob->bytes.size(), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    return status_of(e);
+    if (!gb->resize((total + 7) / 8) || !ob->resize((obits + 7) / 8)) return GRASS_ENOMEM;
+    int rc = fetch_to_host(gb->data(), d_g, gb->size(), s);
+    return rc ? rc : fetch_to_host(ob->data(), d_o, ob->size(), s);
 }
 
 }  // namespace

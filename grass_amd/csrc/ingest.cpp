@@ -248,11 +248,12 @@ void gt_write(BitWriter& bw, uint32_t v, const uint32_t* N, uint32_t d, const Gt
 }
 
 // Big-endian-bit words -> bytes.
-void words_to_bytes(const std::vector<uint64_t>& w, uint64_t nbytes, std::vector<uint8_t>& out,
+void words_to_bytes(const std::vector<uint64_t>& w, uint64_t nbytes, grass_blob& out,
                     int nt) {
-    out.assign(nbytes, 0);
+    if (!out.resize(nbytes)) throw std::bad_alloc();
+    uint8_t* o = out.data();
     parallel_chunks(nt, nbytes, [&](int, uint64_t lo, uint64_t hi) {
-        for (uint64_t j = lo; j < hi; ++j) out[j] = (uint8_t)(w[j >> 3] >> (56 - 8 * (j & 7)));
+        for (uint64_t j = lo; j < hi; ++j) o[j] = (uint8_t)(w[j >> 3] >> (56 - 8 * (j & 7)));
     });
 }
 
@@ -516,7 +517,7 @@ int grass_compress(const grass_csr* csr, int format, int nthreads, grass_blob** 
                 else gt_write(bw, (uint32_t)v, N, d, gt_plan((uint32_t)v, N, d));
             }
         });
-        words_to_bytes(w, (total + 7) / 8, gb->bytes, nt);
+        words_to_bytes(w, (total + 7) / 8, *gb, nt);
         // .offset: byte 0 = ubl, then n fields of ubl bits holding the record lengths
         // (LE/chunk_compress/compressor.hpp:217-285, 392-403)
         const int ubl = bit_num((int64_t)maxlen);
@@ -529,7 +530,7 @@ int grass_compress(const grass_csr* csr, int format, int nthreads, grass_blob** 
             for (uint64_t v = lo; v < hi; ++v) bw.put(start[v + 1] - start[v], ubl);
         });
         ow[0] |= (uint64_t)ubl << 56;
-        words_to_bytes(ow, (obits + 7) / 8, ob->bytes, nt);
+        words_to_bytes(ow, (obits + 7) / 8, *ob, nt);
     } catch (const std::bad_alloc&) {
         delete gb;
         delete ob;
@@ -542,8 +543,8 @@ int grass_compress(const grass_csr* csr, int format, int nthreads, grass_blob** 
 
 int grass_blob_data(const grass_blob* blob, const uint8_t** data, uint64_t* size) {
     if (!blob) return GRASS_EINVAL;
-    if (data) *data = blob->bytes.data();
-    if (size) *size = blob->bytes.size();
+    if (data) *data = blob->data();
+    if (size) *size = blob->size();
     return GRASS_OK;
 }
 
@@ -557,8 +558,8 @@ int grass_write_files(const char* prefix, const grass_blob* graph, const grass_b
     for (int i = 0; i < 2; ++i) {
         FILE* f = fopen((p + ext[i]).c_str(), "wb");
         if (!f) return GRASS_EIO;
-        size_t sz = blobs[i]->bytes.size();
-        bool ok = fwrite(blobs[i]->bytes.data(), 1, sz, f) == sz;
+        size_t sz = blobs[i]->size();
+        bool ok = fwrite(blobs[i]->data(), 1, sz, f) == sz;
         ok = (fclose(f) == 0) && ok;
         if (!ok) return GRASS_EIO;
     }

@@ -193,10 +193,6 @@ __global__ __launch_bounds__(kExpBlock) void gt_expand_kernel(DevGraph g, const 
     }
 }
 
-namespace {
-
-constexpr uint64_t kChunk = 64ull << 20;  // pinned staging chunk
-
 // Host threads that copy a chunk into pinned memory: GRASS_LOAD_THREADS in [1, 64], else up to 8.
 unsigned load_threads() {
     if (const char* e = getenv("GRASS_LOAD_THREADS")) {
@@ -228,6 +224,10 @@ void parallel_copy(void* dst, const uint8_t* src, uint64_t len, unsigned threads
     for (uint64_t lo : mine) memcpy((uint8_t*)dst + lo, src + lo, std::min(step, len - lo));
     for (std::thread& th : pool) th.join();
 }
+
+namespace {
+
+constexpr uint64_t kChunk = 64ull << 20;  // pinned staging chunk
 
 // Streams `src` into device memory through two pinned chunks so that the copy of chunk i+1
 // overlaps the DMA of chunk i.  A source held in host memory is copied with several threads.

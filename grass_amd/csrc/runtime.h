@@ -205,6 +205,9 @@ struct ByteSource {
     virtual bool read(void* dst, uint64_t len) = 0;
 };
 int load_graph(grass_graph* g, ByteSource& graph_src, ByteSource& offset_src);
+// Host threads for copies between pageable and pinned memory (GRASS_LOAD_THREADS, default <= 8).
+unsigned load_threads();
+void parallel_copy(void* dst, const uint8_t* src, uint64_t len, unsigned threads);
 
 // decode.hip
 int decode_csr(grass_graph* g, uint64_t* rowptr, uint32_t* col, hipStream_t stream);
