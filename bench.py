@@ -185,8 +185,40 @@ def main():
     csr = ingest.graph_sort(src, dst, threads=args.threads)
     del src, dst
     log(f"graph_sort: n={csr.n} m={csr.m} at {time.time() - t0:.1f} s")
-    gb, ob = csr.compress(args.fmt, args.threads)
+    IL = ingest.lib()
+
+    def encoded(call):
+        """(graph bytes, offset bytes, seconds of the C ABI call alone) of one compressor call."""
+        hg, ho = grass_amd.C.c_void_p(), grass_amd.C.c_void_p()
+        t1 = time.time()
+        grass_amd._check(call(grass_amd.C.byref(hg), grass_amd.C.byref(ho)), "compress")
+        dt = time.time() - t1
+        try:
+            return ingest._blob_bytes(hg), ingest._blob_bytes(ho), dt
+        finally:
+            IL.grass_blob_free(hg)
+            IL.grass_blob_free(ho)
+
+    fmt_code = 0 if args.fmt == "le" else 1
+    gb, ob, t_host_enc = encoded(lambda hg, ho: IL.grass_compress(csr._h, fmt_code, args.threads, hg, ho))
     t_ingest = time.time() - t0
+    # the same images from the device encoder (CSR uploaded first); its time includes the images'
+    # copy back to host memory
+    rp_h, col_h = csr.arrays()
+    rp_d = torch.from_numpy(rp_h.astype(np.int64)).to(dev)
+    col_d = torch.from_numpy(col_h.view(np.int32)).to(dev)
+    del rp_h, col_h
+    torch.cuda.synchronize()
+    t_dev_enc = None
+    for _ in range(2):  # the first call also loads the kernels and allocates the pinned chunks
+        gb_d, ob_d, t_dev_enc = encoded(lambda hg, ho: IL.grass_compress_device(
+            grass_amd.C.c_void_p(rp_d.data_ptr()), grass_amd.C.c_void_p(col_d.data_ptr()), csr.n, fmt_code,
+            0, None, hg, ho))
+    encode = {"host_ms": round(t_host_enc * 1e3, 1), "host_threads": args.threads,
+              "device_ms": round(t_dev_enc * 1e3, 1), "identical": gb_d == gb and ob_d == ob}
+    del gb_d, ob_d, rp_d, col_d
+    torch.cuda.empty_cache()
+    log(f"encode: {encode}")
     log(f"ingest s{args.scale} {args.fmt}: n={csr.n} m={csr.m} graph={len(gb)/1e6:.1f} MB "
         f"offset={len(ob)/1e6:.1f} MB in {t_ingest:.1f} s")
     t0 = time.time()
@@ -288,6 +320,26 @@ def main():
         assert torch.equal(ref0, outs[0]), "batched and plain walk outputs differ"
     g.sync()
     log(f"2-hop batch: {R2} requests, {sum(len(s) for s in kseeds)} seeds, {khop_edges} edges")
+    export = None
+    if R2:
+        # mini-batch blocks of the 2-hop batch (local numbering per request), timed on the stream;
+        # the batch runs again first, as the plain requests above reused its output buffers
+        khop_batched()
+        ncap = sum(len(x) for x in kseeds) + khop_edges
+        bn = torch.empty(max(ncap, 1), dtype=torch.int32, device=dev)
+        bo = torch.empty(R2 + 1, dtype=torch.int64, device=dev)
+        bls = torch.empty(max(khop_edges, 1), dtype=torch.int32, device=dev)
+        bld = torch.empty(max(khop_edges, 1), dtype=torch.int32, device=dev)
+        blk = grass_amd.KhopBlocks(bn.data_ptr(), ncap, bo.data_ptr(), bls.data_ptr(), bld.data_ptr(), 0)
+
+        def blocks():
+            grass_amd._check(L.grass_khop_blocks_build(g.handle, karr, R2, C.byref(kout), C.byref(blk),
+                                                       C.c_void_p(sptr)), "khop blocks")
+        _, ev_bl = timed(blocks, 5, 1, events=True)
+        export = {"khop_blocks_ms": round(float(np.mean(ev_bl)), 3), "nodes": blk.nodes,
+                  "edges": khop_edges}
+        del bn, bo, bls, bld
+        log(f"export: {export}")
 
     # the two request kinds are timed separately (HIP events on the launching stream), then together
     _, ev_rw = timed(lambda: L.grass_random_walk_batch(g.handle, arr, len(reqs), seed, C.c_void_p(sptr)),
@@ -488,7 +540,8 @@ def main():
         "plain_value": plain, "batched_over_plain": (value / plain) if plain else None,
         "p99_ms": p99,
         "roofline": roof, "cpu_baseline": cpu,
-        "ingest_s": round(t_ingest, 2), "load_s": round(t_load, 3), "tuning": tuning,
+        "ingest_s": round(t_ingest, 2), "load_s": round(t_load, 3), "encode": encode, "export": export,
+        "tuning": tuning,
         "gt_checkpoint": g.gt_checkpoint,
     }
     line.update(extra)

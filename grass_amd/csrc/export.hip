@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kExBlock) void rel_insert_kernel(RelSeq q, uint64_t
             h = (h + 1) & t.mask;
         }
         if (placed) atomicMin(t.val(h), (unsigned long long)p);
-        else atomicCAS(err, kErrNone, kErrBadNeighbor);  // cannot happen: slots > 2 positions
+        else atomicOr(err, 1u);  // cannot happen: slots > 2 positions
     }
 }
 
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kExBlock) void rel_emit_kernel(RelSeq q, uint64_t P
         const uint64_t fd = flag[p] ? p : t.first(rel_key(r, v));
         const uint64_t fs = t.first(rel_key(r, q.src[e]));
         if (fd >= P || fs >= P) {  // a source that is neither a seed nor an earlier destination
-            atomicCAS(err, kErrNone, kErrBadNeighbor);
+            atomicOr(err, 1u);
             ldst[e] = lsrc[e] = kNoId;
             continue;
         }
@@ -228,7 +228,7 @@ int khop_blocks(grass_graph* g, const grass_khop_request* reqs, uint32_t R, cons
     uint32_t* d_flag = cv.take<uint32_t>(P + 1);
     uint64_t* d_F = cv.take<uint64_t>(P + 1);
     uint64_t* d_part = cv.take<uint64_t>(nb + 2);
-    uint64_t* d_sc = cv.take<uint64_t>(8);  // [0] sequence total, [1] node total
+    uint64_t* d_sc = cv.take<uint64_t>(8);  // [0] sequence total, [1] node total, [2] error flag
     RelTab t;
     t.slot = (unsigned long long*)ws.ht.ptr;
     t.mask = slots - 1;
@@ -245,9 +245,11 @@ int khop_blocks(grass_graph* g, const grass_khop_request* reqs, uint32_t R, cons
     if (e != hipSuccess) return status_of(e);
     RelSeq q{d_req, b->offsets_dev, b->src_dev, b->dst_dev, d_Q, R, H};
     const unsigned gP = (unsigned)std::min<uint64_t>((P + kExBlock - 1) / kExBlock + 1, kExGrid);
+    uint32_t* d_bad = (uint32_t*)(d_sc + 2);
+    hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(1), dim3(64), 0, s, d_bad, (uint64_t)1, 0u);
     hipLaunchKernelGGL(rel_clear_kernel, dim3(kExGrid), dim3(256), 0, s, t);
     if (P) {
-        hipLaunchKernelGGL(rel_insert_kernel, dim3(gP), dim3(kExBlock), 0, s, q, P, t, g->d_err);
+        hipLaunchKernelGGL(rel_insert_kernel, dim3(gP), dim3(kExBlock), 0, s, q, P, t, d_bad);
         hipLaunchKernelGGL(rel_mark_kernel, dim3(gP), dim3(kExBlock), 0, s, q, P, t, d_flag);
     }
     e = exclusive_scan(FlagGen{d_flag}, Count{P, nullptr}, P, d_F, d_part, d_sc + 1, s);
@@ -255,15 +257,19 @@ int khop_blocks(grass_graph* g, const grass_khop_request* reqs, uint32_t R, cons
     if (P)
         hipLaunchKernelGGL(rel_emit_kernel, dim3(gP), dim3(kExBlock), 0, s, q, P, t, (const uint32_t*)d_flag,
                            (const uint64_t*)d_F, out->nodes_dev, out->node_capacity, out->lsrc_dev, out->ldst_dev,
-                           g->d_err);
+                           d_bad);
     hipLaunchKernelGGL(rel_node_off_kernel, dim3(gR), dim3(256), 0, s, (const uint64_t*)d_Q, (const uint64_t*)d_F, R,
                        out->node_off_dev);
     uint64_t* hres = ws.result_slot();
+    hres[2] = 0;
     e = hipMemcpyAsync(hres, d_Q + R, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(hres + 1, d_F + P, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(hres + 2, d_bad, 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return status_of(e);
-    if (hres[0] != P) return GRASS_EINVAL;  // the offsets are not the batch of these requests
+    // the offsets are not the batch of these requests (sizes differ, or an edge's source is not
+    // among the request's earlier vertices)
+    if (hres[0] != P || (uint32_t)hres[2] != 0) return GRASS_EINVAL;
     out->nodes = hres[1];
     return out->nodes > out->node_capacity ? GRASS_ECAPACITY : GRASS_OK;
 }

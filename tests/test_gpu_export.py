@@ -134,3 +134,40 @@ def test_mdrw_dlpack_and_local_ids(ga, fmt):
     again = g.mdrw_tensors(sizes, target=400, seed=9, local=False)
     np.testing.assert_array_equal(_np(again["ind_dst"]).view(np.uint32), host["ind_dst"])
     g.close()
+
+
+def test_khop_blocks_reject_offsets_of_another_batch(ga):
+    """Offsets that do not belong to the requests (request 1 claims request 0's hop-0 edges; every
+    read stays inside the batch's arrays) give GRASS_EINVAL, and the graph stays usable."""
+    import ctypes as C
+
+    import torch
+    print("[export khop blocks] mismatched offsets", flush=True)
+    c = load_case("rmat12")
+    g = ga.Graph.from_bytes(c["gt_graph"].tobytes(), c["gt_offset"].tobytes(), "gt")
+    from conftest import csr_of
+    rowptr, _ = csr_of(c)
+    busy = np.flatnonzero(np.diff(rowptr.astype(np.int64)) >= 2).astype(np.uint32)
+    seeds = [busy[:3], busy[-2:]]
+    arr, keep = g.khop_request_array(seeds, [[5, 5], [5, 5]])
+    cap = g.khop_bound(arr, 2)
+    src = torch.empty(cap, dtype=torch.int32, device="cuda")
+    dst = torch.empty(cap, dtype=torch.int32, device="cuda")
+    off = torch.zeros(6, dtype=torch.int64, device="cuda")
+    out = ga.KhopOutput(src.data_ptr(), dst.data_ptr(), cap, off.data_ptr(), 0, 0)
+    L = ga.lib()
+    assert L.grass_khop_batch(g.handle, arr, 2, 3, C.byref(out), None) == ga.OK
+    g.sync()
+    o = _np(off)
+    assert o[1] > o[0] and o[2] > o[1]
+    off[1] = off[0]
+    nodes = torch.empty(cap + 5, dtype=torch.int32, device="cuda")
+    noff = torch.empty(3, dtype=torch.int64, device="cuda")
+    ls = torch.empty(cap, dtype=torch.int32, device="cuda")
+    ld = torch.empty(cap, dtype=torch.int32, device="cuda")
+    blk = ga.KhopBlocks(nodes.data_ptr(), cap + 5, noff.data_ptr(), ls.data_ptr(), ld.data_ptr(), 0)
+    assert L.grass_khop_blocks_build(g.handle, arr, 2, C.byref(out), C.byref(blk), None) == ga.EINVAL
+    g.sync()  # no error left behind on the graph
+    res = g.khop_minibatch(seeds, [[5, 5], [5, 5]], seed=3)
+    _check_blocks(res, seeds)
+    g.close()
