@@ -123,16 +123,6 @@ __device__ __forceinline__ void put_field(uint32_t* B, uint64_t w0, uint64_t at,
     }
 }
 
-// Writes the buffer to stream words [w0, w0 + nwords): the first and last words, which the
-// neighbouring passes may share, are ORed in (`out` starts zeroed); the others are stored.
-__device__ __forceinline__ void flush_pass(uint32_t* out, const uint32_t* B, uint64_t w0, uint32_t nwords,
-                                           int lane) {
-    for (uint32_t j = lane; j < nwords; j += kWave) {
-        if (j == 0 || j == nwords - 1) atomicOr(&out[w0 + j], B[j]);
-        else out[w0 + j] = B[j];
-    }
-}
-
 // p[0..count) = value.  Used instead of hipMemsetAsync for anything a caller might capture in a
 // HIP graph.
 template <typename T>

@@ -5,12 +5,13 @@
 // 1. enc_plan_kernel, one wave per vertex: the record's field widths (LE rock_bit, sea_bit,
 //    first_pos; GT head_bit, body_bit, first_pos) as maxima over the list, and its length.
 // 2. An exclusive scan of the lengths: the bit where each record starts.
-// 3. enc_write_kernel, one wave per vertex: lane 0 writes the header's gamma codes; the fields
-//    sit at offsets that follow from the widths (LE: rocks, then signed seas; GT: per block the
-//    head, then the bodies), so 64 fields at a time are assembled in LDS and written out, the words
-//    shared with the neighbouring passes or records ORed in.
+// 3. enc_write_kernel, one wave per vertex: the fields sit at offsets that follow from the widths
+//    (LE: rocks, then signed seas; GT: per block the head, then the bodies), so the header's gamma
+//    codes and 64 fields at a time are assembled in LDS and stored; only the words a record shares
+//    with its neighbours are ORed into memory.
 // 4. enc_offset_kernel: the .offset image, byte 0 = ubl then n length fields of ubl bits.
-// The words are MSB-first and big-endian-bit, so a byte swap gives the file bytes.
+// The stream is MSB-first in big-endian-bit words; .graph words are stored byte-swapped, so that
+// memory holds the file's bytes, and the .offset words are swapped after assembly.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -149,38 +150,69 @@ __global__ __launch_bounds__(kEncBlock) void enc_plan_kernel(const uint64_t* __r
     if (lane == 0 && most) atomicMax(maxlen, most);
 }
 
-// `wdt` <= 32 low bits of val at stream bit `at`, ORed straight into `out`.
-__device__ __forceinline__ void put_global(uint32_t* out, uint64_t at, uint32_t wdt, uint32_t val) {
-    const uint64_t w = at >> 5;
-    const uint32_t s = (uint32_t)(at & 31);
-    if (s + wdt <= 32) {
-        atomicOr(&out[w], val << (32 - s - wdt));
-    } else {
-        atomicOr(&out[w], val >> (s + wdt - 32));
-        atomicOr(&out[w + 1], val << (64 - s - wdt));
-    }
-}
+// A record is written pass by pass.  Each pass assembles its words in LDS (B[0] = the word holding
+// the pass's first bit) and stores them; only the record's first word and its last, partial word
+// are shared with the neighbouring records and ORed into memory (`out` starts zeroed).  A partial
+// last word of a pass that does not end the record is carried into the next pass instead of
+// being written twice.
+constexpr uint32_t kRecWords = kEncWords + 6;  // the first pass also holds the header (<= 96 bits)
 
-// Elias-gamma code of x + 1 (x + 1 < 2^32): L zero bits, then x + 1 in L + 1 bits.
-__device__ __forceinline__ void put_gamma(uint32_t* out, uint64_t& at, uint32_t x) {
+struct RecState {   // wave-uniform
+    uint32_t pend;  // the carried partial word
+    uint64_t pend_w;  // its index, ~0 when nothing is carried
+    bool pend_first;  // it is the record's first word
+};
+
+// Elias-gamma code of x + 1 (x + 1 < 2^32) at stream bit `at` of the pass buffer: L zero bits,
+// then x + 1 in L + 1 bits.
+__device__ __forceinline__ void put_gamma(uint32_t* B, uint64_t w0, uint64_t& at, uint32_t x) {
     const uint32_t y = x + 1;
     const uint32_t L = 31u - (uint32_t)__clz(y);
-    put_global(out, at + L, L + 1, y);
+    put_field(B, w0, at + L, L + 1, y);
     at += 2 * L + 1;
 }
 
-// One pass of up to 64 fields, lane l holding field l (act: the lane has one); the fields are
-// ordered by lane and `last` is the highest lane that has one.
-__device__ __forceinline__ void enc_pass(uint32_t* out, uint32_t* B, int lane, bool act, uint64_t at,
-                                         uint32_t wdt, uint32_t val, uint32_t last) {
-    const uint64_t w0 = __shfl(at, 0, kWave) >> 5;
-    const uint64_t end = __shfl(at + wdt, (int)last, kWave);
-    const uint32_t nwords = (uint32_t)(((end - 1) >> 5) - w0 + 1);
-    for (uint32_t j = lane; j < kEncWords; j += kWave) B[j] = 0;
+// One pass over stream bits [from, ...): the header codes hv[0..nh) at `from` (first pass only,
+// written by lane 0), then up to 64 fields, lane l holding field l at `at` (act: the lane has one,
+// `last` the highest lane that has one; nf = 0: no fields).
+__device__ __forceinline__ void rec_pass(uint32_t* out, uint32_t* B, int lane, uint64_t from, bool first, bool final,
+                                         const uint32_t* hv, int nh, bool nf, bool act, uint64_t at, uint32_t wdt,
+                                         uint32_t val, uint32_t last, RecState& st) {
+    const uint64_t w0 = from >> 5;
+    const bool carried = st.pend_w == w0;
+    for (uint32_t j = lane; j < kRecWords; j += kWave) B[j] = (j == 0 && carried) ? st.pend : 0u;
     wave_sync();
-    if (act) put_field(B, w0, at, wdt, wdt < 32 ? val & ((1u << wdt) - 1) : val);
+    uint64_t to = from;
+    if (first) {
+        for (int i = 0; i < nh; ++i) {
+            if (lane == 0) {
+                uint64_t h = to;
+                put_gamma(B, w0, h, hv[i]);
+            }
+            to += gamma_bits(hv[i]);
+        }
+    }
+    if (nf) {
+        if (act) put_field(B, w0, at, wdt, wdt < 32 ? val & ((1u << wdt) - 1) : val);
+        to = __shfl(at + wdt, (int)last, kWave);
+    }
     wave_sync();
-    flush_pass(out, B, w0, nwords, lane);
+    const uint32_t nwords = to > from ? (uint32_t)(((to - 1) >> 5) - w0 + 1) : 0u;
+    const bool open = (to & 31) != 0;
+    const bool shared0 = first || (carried && st.pend_first);
+    for (uint32_t j = lane; j < nwords; j += kWave) {
+        const bool tail = j == nwords - 1 && open;
+        if (tail && !final) continue;  // carried into the next pass
+        // stored byte-swapped: the words become the file's bytes (a swap commutes with the OR)
+        const uint32_t x = __builtin_bswap32(B[j]);
+        if ((j == 0 && shared0) || tail) atomicOr(&out[w0 + j], x);
+        else out[w0 + j] = x;
+    }
+    if (nwords) {
+        st.pend = B[nwords - 1];
+        st.pend_w = open && !final ? w0 + nwords - 1 : ~0ull;
+        st.pend_first = nwords == 1 && shared0;
+    }
     wave_sync();
 }
 
@@ -190,7 +222,7 @@ __global__ __launch_bounds__(kEncBlock) void enc_write_kernel(const uint64_t* __
                                                               const uint64_t* __restrict__ start,
                                                               const uint2* __restrict__ widths,
                                                               uint32_t* __restrict__ out) {
-    __shared__ uint32_t buf[kEncWaves][kEncWords];
+    __shared__ uint32_t buf[kEncWaves][kRecWords];
     const int lane = lane_id();
     uint32_t* B = buf[threadIdx.x / kWave];
     const uint64_t nw = (uint64_t)gridDim.x * kEncWaves;
@@ -201,27 +233,26 @@ __global__ __launch_bounds__(kEncBlock) void enc_write_kernel(const uint64_t* __
         const int64_t vv = (int64_t)v;
         const uint2 wd = widths[v];
         const uint32_t f1 = wd.x & 0xFFu, f2 = wd.x >> 8, fp = wd.y;
+        const uint64_t at0 = start[v];
+        RecState st{0u, ~0ull, false};
         // header: gamma codes (LE: d [rock_bit | the neighbour] [sea_bit first_pos]; GT: d
         // head_bit body_bit first_pos)
-        uint64_t pay = start[v];
-        const bool le_multi = FMT == kLE && d >= 2;
-        if (lane == 0) {
-            uint64_t at = pay;
-            put_gamma(out, at, d);
-            if (FMT == kLE && d == 1) put_gamma(out, at, N[0]);
-            if (FMT == kGT || le_multi) {
-                put_gamma(out, at, f1);
-                put_gamma(out, at, f2);
-                put_gamma(out, at, fp);
-            }
+        uint32_t hv[4] = {d, f1, f2, fp};
+        int nh = 4;
+        if (FMT == kLE && d <= 1) {
+            hv[1] = d ? N[0] : 0u;
+            nh = d ? 2 : 1;
         }
-        pay += gamma_bits(d);
-        if (FMT == kLE && d == 1) pay += gamma_bits(N[0]);
-        if (FMT == kGT || le_multi) pay += gamma_bits(f1) + gamma_bits(f2) + gamma_bits(fp);
+        uint64_t pay = at0;
+        for (int i = 0; i < nh; ++i) pay += gamma_bits(hv[i]);
         if (FMT == kLE) {
-            if (!le_multi) continue;
+            if (d <= 1) {
+                rec_pass(out, B, lane, at0, true, true, hv, nh, false, false, 0, 0, 0, 0, st);
+                continue;
+            }
             uint32_t C, q, R;
             le_geometry(d, C, q, R);
+            const uint32_t S = d - 1 - C, bigS = R * q;
             // rocks 0..C: |N - v| in rock_bit bits
             for (uint32_t c0 = 0; c0 <= C; c0 += kWave) {
                 const uint32_t c = c0 + lane;
@@ -229,11 +260,11 @@ __global__ __launch_bounds__(kEncBlock) void enc_write_kernel(const uint64_t* __
                 const uint32_t cc = act ? c : C;
                 const uint32_t k = cc * q + min(cc, R);
                 const int64_t x = (int64_t)N[k] - vv;
-                enc_pass(out, B, lane, act, pay + (uint64_t)cc * f1, f1, (uint32_t)(x < 0 ? -x : x),
-                         min((uint32_t)kWave - 1, C - c0));
+                const uint64_t at = pay + (uint64_t)cc * f1;
+                rec_pass(out, B, lane, c0 ? pay + (uint64_t)c0 * f1 : at0, c0 == 0, S == 0 && c0 + kWave > C, hv, nh,
+                         true, act, at, f1, (uint32_t)(x < 0 ? -x : x), min((uint32_t)kWave - 1, C - c0), st);
             }
             // seas, chunk by chunk: |residual| in sea_bit bits, then its sign
-            const uint32_t S = d - 1 - C, bigS = R * q;
             const uint64_t seas = pay + (uint64_t)(C + 1) * f1;
             for (uint32_t j0 = 0; j0 < S; j0 += kWave) {
                 const uint32_t j = j0 + lane;
@@ -250,11 +281,15 @@ __global__ __launch_bounds__(kEncBlock) void enc_write_kernel(const uint64_t* __
                 const uint32_t k = c * q + min(c, R) + i;
                 const int64_t res = le_residual(N, k, i, q + (c < R));
                 const uint32_t mag = (uint32_t)(res < 0 ? -res : res);
-                enc_pass(out, B, lane, act, seas + (uint64_t)jj * (f2 + 1), f2 + 1, (mag << 1) | (res < 0 ? 1u : 0u),
-                         min((uint32_t)kWave - 1, S - 1 - j0));
+                rec_pass(out, B, lane, seas + (uint64_t)j0 * (f2 + 1), false, j0 + kWave >= S, hv, nh, true, act,
+                         seas + (uint64_t)jj * (f2 + 1), f2 + 1, (mag << 1) | (res < 0 ? 1u : 0u),
+                         min((uint32_t)kWave - 1, S - 1 - j0), st);
             }
         } else {
-            if (d == 0) continue;
+            if (d == 0) {
+                rec_pass(out, B, lane, at0, true, true, hv, nh, false, false, 0, 0, 0, 0, st);
+                continue;
+            }
             uint32_t C, q, R;
             gt_geometry(d, C, q, R);
             for (uint32_t k0 = 0; k0 < d; k0 += kWave) {
@@ -271,8 +306,10 @@ __global__ __launch_bounds__(kEncBlock) void enc_write_kernel(const uint64_t* __
                 } else {
                     val = (uint32_t)((int64_t)N[kk] - (int64_t)N[kk - 1] - 1);
                 }
-                enc_pass(out, B, lane, act, i ? bo + f1 + (uint64_t)(i - 1) * f2 : bo, i ? f2 : f1, val,
-                         min((uint32_t)kWave - 1, d - 1 - k0));
+                const uint64_t at = i ? bo + f1 + (uint64_t)(i - 1) * f2 : bo;
+                const uint64_t from = k0 ? __shfl(at, 0, kWave) : at0;
+                rec_pass(out, B, lane, from, k0 == 0, k0 + kWave >= d, hv, nh, true, act, at, i ? f2 : f1, val,
+                         min((uint32_t)kWave - 1, d - 1 - k0), st);
             }
         }
     }
@@ -293,11 +330,6 @@ __global__ __launch_bounds__(256) void enc_offset_kernel(const uint64_t* __restr
             atomicOr(&ow[w + 1], (unsigned long long)(x << (128 - s - ubl)));
         }
     }
-}
-
-__global__ void bswap32_kernel(uint32_t* w, uint64_t count) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
-        w[i] = __builtin_bswap32(w[i]);
 }
 
 __global__ void bswap64_kernel(unsigned long long* w, uint64_t count) {
@@ -396,7 +428,6 @@ int compress_device(const uint64_t* rowptr, const uint32_t* col, uint64_t n, int
     else
         hipLaunchKernelGGL(enc_write_kernel<kGT>, dim3(grid), dim3(kEncBlock), 0, s, rowptr, col, n,
                            (const uint64_t*)d_start, (const uint2*)d_w, d_g);
-    hipLaunchKernelGGL(bswap32_kernel, dim3(2048), dim3(256), 0, s, d_g, gwords);
     // .offset: ubl = get_bit_num(longest record), then the n lengths
     uint32_t ubl = 1;
     while (ubl < 64 && (maxlen >> ubl)) ++ubl;

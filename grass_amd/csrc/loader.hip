@@ -126,9 +126,10 @@ constexpr uint32_t kExpWords = kWave + 2;  // words that one pass of 64 fields (
 // One wave per GT vertex: decodes the .graph record 64 positions at a time (heads absolute,
 // bodies as gaps, a segmented prefix sum carrying the running id from pass to pass) and writes
 // the vertex's payload in the device stream at word woff[v]: heads and bodies copied, the 32-bit
-// id at every checkpoint position.  A pass assembles its fields in LDS, stores the words that lie
-// inside it and ORs in its first and last word (shared with the neighbouring passes; `out` starts
-// zeroed).  Finally the header's payload bit is moved to the stream.
+// id at every checkpoint position.  A pass assembles its fields in LDS and stores its complete
+// words; a last word that the pass leaves partial is carried into the next pass.  A vertex's
+// stream words are its own (its payload starts on a word), so every word is stored once and
+// nothing is ORed into memory.  Finally the header's payload bit is moved to the stream.
 __global__ __launch_bounds__(kExpBlock) void gt_expand_kernel(DevGraph g, const uint64_t* __restrict__ woff,
                                                               uint4* __restrict__ hdr, uint32_t* __restrict__ out) {
     __shared__ uint32_t buf[kExpBlock / kWave][kExpWords];
@@ -143,6 +144,8 @@ __global__ __launch_bounds__(kExpBlock) void gt_expand_kernel(DevGraph g, const 
         Hdr hx = h;
         hx.pay = woff[v] << 5;  // payload bit in the stream
         uint32_t carry = 0;
+        uint32_t pend = 0;        // the partial last word of the previous pass
+        uint64_t pend_w = ~0ull;  // its index, ~0 when that pass ended on a word boundary
         for (uint32_t base = 0; base < h.d; base += kWave) {
             const uint32_t k = base + lane;
             const bool act = k < h.d;
@@ -178,13 +181,18 @@ __global__ __launch_bounds__(kExpBlock) void gt_expand_kernel(DevGraph g, const 
             const uint64_t w0 = __shfl(at, 0, kWave) >> 5;
             const uint64_t end = __shfl(at + wdt, (int)la, kWave);
             const uint32_t nwords = (uint32_t)(((end - 1) >> 5) - w0 + 1);
-            for (uint32_t j = lane; j < kExpWords; j += kWave) B[j] = 0;
+            for (uint32_t j = lane; j < kExpWords; j += kWave) B[j] = (j == 0 && pend_w == w0) ? pend : 0u;
             wave_sync();
             if (act) put_field(B, w0, at, wdt, raw);
             wave_sync();
-            flush_pass(out, B, w0, nwords, lane);
+            const bool open = (end & 31) != 0;
+            const uint32_t full = open ? nwords - 1 : nwords;
+            for (uint32_t j = lane; j < full; j += kWave) out[w0 + j] = B[j];
+            pend = B[nwords - 1];
+            pend_w = open ? w0 + nwords - 1 : ~0ull;
             wave_sync();
         }
+        if (pend_w != ~0ull && lane == 0) out[pend_w] = pend;
         if (lane == 0) {
             e.x = (uint32_t)hx.pay;
             e.y = (e.y & ~0xFFu) | (uint32_t)(hx.pay >> 32);
