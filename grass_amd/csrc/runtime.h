@@ -70,6 +70,10 @@ struct DevMem {
         if (p) (void)hipFree(p);
     }
     hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes > 256 ? bytes : 256); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
     template <typename T>
     T* as() const { return (T*)p; }
 };

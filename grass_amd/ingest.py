@@ -1,5 +1,5 @@
 """Ingest through libgrass_amd.so (include/grass_ingest.h): R-MAT edge lists, graph_sort
-preprocessing and the LE / GT compressors.  Only compress_device needs a GPU."""
+preprocessing and the LE / GT compressors, on the host and (the *_device calls) on a GPU."""
 import ctypes as C
 import os
 
@@ -24,6 +24,9 @@ def lib():
             "grass_csr_free": (None, [P]),
             "grass_compress": (I32, [P, I32, I32, P, P]),
             "grass_compress_device": (I32, [P, P, U64, I32, I32, P, P, P]),
+            "grass_rmat_edges_device": (I32, [U32, U32, C.c_double, C.c_double, C.c_double, U64, I32, P, P,
+                                              P]),
+            "grass_graph_sort_device": (I32, [P, P, U64, I32, I32, P, P, U64, P, U64, P, P]),
             "grass_blob_data": (I32, [P, P, P]),
             "grass_blob_free": (None, [P]),
             "grass_write_files": (I32, [C.c_char_p, P, P]),
@@ -102,6 +105,37 @@ def compress_device(rowptr, col, fmt="le", device=0, stream=None):
     finally:
         lib().grass_blob_free(g)
         lib().grass_blob_free(o)
+
+
+def rmat_edges_device(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, device=0, stream=None):
+    """grass_rmat_edges on the GPU: (src, dst) int32 device tensors of edge_factor << scale ids."""
+    from . import _stream_ptr, _torch
+    torch = _torch()
+    m = edge_factor << scale
+    src = torch.empty(m, dtype=torch.int32, device=f"cuda:{device}")
+    dst = torch.empty(m, dtype=torch.int32, device=f"cuda:{device}")
+    _check(lib().grass_rmat_edges_device(scale, edge_factor, a, b, c, seed, device,
+                                         C.c_void_p(_stream_ptr(stream)), C.c_void_p(src.data_ptr()),
+                                         C.c_void_p(dst.data_ptr())), "grass_rmat_edges_device")
+    return src, dst
+
+
+def graph_sort_device(src, dst, directed=False, device=0, stream=None):
+    """grass_graph_sort on the GPU over device tensors of ids: (rowptr int64 [n+1], col int32 [m'])
+    device tensors."""
+    from . import _stream_ptr, _torch
+    torch = _torch()
+    m = src.numel()
+    n, mo = C.c_uint64(), C.c_uint64()
+    cap_r = (int(torch.maximum(src.max(), dst.max()).item()) + 2) if m else 1
+    cap_c = max(m if directed else 2 * m, 1)
+    rowptr = torch.empty(cap_r, dtype=torch.int64, device=src.device)
+    col = torch.empty(cap_c, dtype=torch.int32, device=src.device)
+    _check(lib().grass_graph_sort_device(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), m,
+                                         int(directed), device, C.c_void_p(_stream_ptr(stream)),
+                                         C.c_void_p(rowptr.data_ptr()), cap_r, C.c_void_p(col.data_ptr()),
+                                         cap_c, C.byref(n), C.byref(mo)), "grass_graph_sort_device")
+    return rowptr[: n.value + 1], col[: mo.value]
 
 
 def _blob_bytes(h):

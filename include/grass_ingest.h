@@ -55,6 +55,23 @@ int grass_compress_device(const uint64_t *rowptr_dev, const uint32_t *col_dev, u
                           int format, int device, void *stream, grass_blob **graph,
                           grass_blob **offset);
 
+/* grass_rmat_edges on a GPU: the same m = edge_factor << scale edges, written to src_dev and
+ * dst_dev (device memory of `device`, m entries each) in order on `stream`.  The reference has no
+ * generator; this is the device form of the synthetic inputs above. */
+int grass_rmat_edges_device(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                            uint64_t seed, int device, void *stream, uint32_t *src_dev,
+                            uint32_t *dst_dev);
+
+/* grass_graph_sort (graph_sort.cpp:22-82) on a GPU, over an edge list in device memory: the same
+ * CSR, written to rowptr_dev (n + 1 entries) and col_dev (*m_out entries).  *n and *m_out receive
+ * the sizes; GRASS_ECAPACITY (sizes set, CSR not written) when rowptr_capacity < n + 1 or
+ * col_capacity < *m_out.  n <= max id + 1 and *m_out <= m (directed) or 2m (undirected).  Device
+ * working memory is about 20 bytes per (directed) edge.  Synchronises on `stream`. */
+int grass_graph_sort_device(const uint32_t *src_dev, const uint32_t *dst_dev, uint64_t m,
+                            int directed, int device, void *stream, uint64_t *rowptr_dev,
+                            uint64_t rowptr_capacity, uint32_t *col_dev, uint64_t col_capacity,
+                            uint64_t *n, uint64_t *m_out);
+
 int grass_blob_data(const grass_blob *blob, const uint8_t **data, uint64_t *size);
 void grass_blob_free(grass_blob *blob);
 
