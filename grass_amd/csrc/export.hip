@@ -143,8 +143,12 @@ __global__ __launch_bounds__(kExBlock) void rel_insert_kernel(RelSeq q, uint64_t
             }
             h = (h + 1) & t.mask;
         }
-        if (placed) atomicMin(t.val(h), (unsigned long long)p);
-        else atomicOr(err, 1u);  // cannot happen: slots > 2 positions
+        if (!placed) {
+            atomicOr(err, 1u);  // cannot happen: slots > 2 positions
+        } else if (__atomic_load_n(t.val(h), __ATOMIC_RELAXED) > p) {
+            // read first: a repeat after the first occurrence needs no atomic
+            atomicMin(t.val(h), (unsigned long long)p);
+        }
     }
 }
 

@@ -113,7 +113,10 @@ __global__ __launch_bounds__(kKhBlock) void dedup_insert_kernel(DedupIn in, Coun
                 break;
             }
         }
-        atomicMin(t.val(h), ((unsigned long long)(0xFFFFu - gen) << 32) | (uint32_t)p);
+        // min of (generation, position): read first, since most repeats come after the first
+        // occurrence and then need no atomic (an atomic costs a write to memory)
+        const unsigned long long mine = ((unsigned long long)(0xFFFFu - gen) << 32) | (uint32_t)p;
+        if (__atomic_load_n(t.val(h), __ATOMIC_RELAXED) > mine) atomicMin(t.val(h), mine);
     }
 }
 
