@@ -41,7 +41,8 @@ struct FileSource : ByteSource {
         void* p = mmap(nullptr, sz, PROT_READ, MAP_PRIVATE, fd, 0);
         if (p != MAP_FAILED) {
             map = (const uint8_t*)p;
-            (void)madvise(p, sz, MADV_SEQUENTIAL | MADV_WILLNEED);
+            (void)madvise(p, sz, MADV_SEQUENTIAL);  // advice values are not flags: one call each
+            (void)madvise(p, sz, MADV_WILLNEED);
         }
     }
     ~FileSource() override {
