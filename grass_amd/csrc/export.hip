@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kExBlock) void rel_insert_kernel(RelSeq q, uint64_t
             h = (h + 1) & t.mask;
         }
         if (!placed) {
-            atomicOr(err, 1u);  // cannot happen: slots > 2 positions
+            atomicOr(err, 1u);  // cannot happen: slots > positions
         } else if (__atomic_load_n(t.val(h), __ATOMIC_RELAXED) > p) {
             // read first: a repeat after the first occurrence needs no atomic
             atomicMin(t.val(h), (unsigned long long)p);
@@ -217,7 +217,7 @@ int khop_blocks(grass_graph* g, const grass_khop_request* reqs, uint32_t R, cons
     const uint64_t P = S + b->total;
     if (P && (!out->nodes_dev && out->node_capacity)) return GRASS_EINVAL;
     uint64_t slots = 1024;
-    while (slots < 2 * P + 2) slots <<= 1;
+    while (slots < P + P / 2 + 2) slots <<= 1;  // load <= 2/3: distinct keys <= positions
     const uint64_t nb = scan_blocks(std::max<uint64_t>(std::max<uint64_t>(P, R), 1));
     const size_t need = Carve::need(R * sizeof(RelReq)) + Carve::need(R * 8) + Carve::need((R + 1) * 8) +
                         Carve::need(P * 4 + 4) + Carve::need((P + 1) * 8) + Carve::need((nb + 2) * 8) +
