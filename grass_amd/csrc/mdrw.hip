@@ -275,6 +275,24 @@ __global__ __launch_bounds__(256) void mdrw_dense_kernel(const uint64_t* __restr
     }
 }
 
+// The dense ids again as a bit vector with counts: word w holds one bit per vertex 64w..64w+63
+// that the slice holds (x) and the dense id of the first of them (y), so that the dense id of y is
+// y's word's count plus the set bits below y in the word.  The induced pass looks up every
+// adjacency entry here: 16 bytes per 64 vertices (2.2 MB at scale 24) against dmap's 4 bytes per
+// vertex (35 MB).
+__global__ __launch_bounds__(256) void mdrw_dbits_kernel(const uint32_t* __restrict__ dmap,
+                                                         const uint64_t* __restrict__ did, uint32_t n,
+                                                         ulonglong2* __restrict__ dbits) {
+    const uint64_t nwords = ((uint64_t)n + kWave - 1) / kWave;
+    const int lane = lane_id();
+    for (uint64_t w = ((uint64_t)blockIdx.x * 256 + threadIdx.x) / kWave; w < nwords;
+         w += (uint64_t)gridDim.x * 256 / kWave) {
+        const uint64_t x = w * kWave + lane;
+        const unsigned long long bits = __ballot(x < n && dmap[x] != kNoRow);
+        if (lane == 0) dbits[w] = make_ulonglong2(bits, did[w * kWave]);
+    }
+}
+
 // rows[0, *nd * W) = 0
 __global__ __launch_bounds__(256) void mdrw_zero_rows_kernel(unsigned long long* __restrict__ rows,
                                                              const uint64_t* __restrict__ nd, uint32_t W) {
@@ -357,7 +375,7 @@ __global__ __launch_bounds__(256) void mdrw_uexpand_kernel(const uint64_t* __res
 // word w in registers, and per matched instance one ballot ranks the wave's hits in list order.
 template <int FMT, bool WRITE>
 __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, const unsigned long long* __restrict__ rows,
-                                                                 uint32_t W, const uint32_t* __restrict__ dmap,
+                                                                 uint32_t W, const ulonglong2* __restrict__ dbits,
                                                                  const uint32_t* __restrict__ dvert,
                                                                  const uint64_t* __restrict__ nchunks,
                                                                  const uint32_t* __restrict__ ux,
@@ -423,7 +441,12 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
                 carry = __shfl(y, kWave - 1, kWave);
             }
             // row of y: none when no sample of the slice holds y (or the entry is not this chunk's)
-            const uint32_t dy = (act && k >= kb && y < g.n) ? dmap[y] : kNoRow;
+            uint32_t dy = kNoRow;
+            if (act && k >= kb && y < g.n) {
+                const ulonglong2 bw = dbits[y >> 6];
+                const unsigned long long bit = 1ull << (y & 63);
+                if (bw.x & bit) dy = (uint32_t)bw.y + (uint32_t)__popcll(bw.x & (bit - 1));
+            }
             // Stage the 64 rows in LDS.  With W >= 2 the W / 2 lanes of a row each load 16 bytes
             // of it, so a row is one memory access instead of one per mask word.
             if (W == 1) {
@@ -665,6 +688,7 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
     uint64_t *d_did = nullptr, *d_pbase = nullptr, *d_uoff = nullptr, *d_nu = nullptr, *d_partN = nullptr,
              *d_ccnt = nullptr, *d_cpos = nullptr, *d_part3 = nullptr;
     uint32_t *d_dmap = nullptr, *d_dvert = nullptr, *d_xpair = nullptr, *d_ux = nullptr, *d_uk = nullptr;
+    ulonglong2* d_dbits = nullptr;
     auto plan3 = [&](Carve& c) {
         d_rows = c.take<unsigned long long>(n * W);  // the slice's distinct vertices use a prefix
         d_did = c.take<uint64_t>(n + 1);
@@ -673,6 +697,7 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
         d_nu = c.take<uint64_t>(1);
         d_partN = c.take<uint64_t>(nbN + 2);
         d_dmap = c.take<uint32_t>(n);
+        d_dbits = c.take<ulonglong2>((n + kWave - 1) / kWave);
         d_dvert = c.take<uint32_t>(n);
         d_xpair = c.take<uint32_t>(NV);
         d_ux = c.take<uint32_t>(NC);
@@ -701,6 +726,8 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(mdrw_dense_kernel, dim3(sgrid), dim3(256), 0, s, (const uint64_t*)d_did, (uint32_t)n,
                            d_dmap, d_dvert);
+        hipLaunchKernelGGL(mdrw_dbits_kernel, dim3(sgrid), dim3(256), 0, s, (const uint32_t*)d_dmap,
+                           (const uint64_t*)d_did, (uint32_t)n, d_dbits);
         hipLaunchKernelGGL(mdrw_zero_rows_kernel, dim3(sgrid), dim3(256), 0, s, d_rows, d_nd, W);
         hipLaunchKernelGGL(mdrw_mask_kernel, dim3(sgrid), dim3(256), 0, s, verts, voff, i0, i1, W,
                            (const uint32_t*)d_dmap, d_rows);
@@ -717,23 +744,23 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
     };
     auto induced = [&](bool write, uint32_t* idst) {
         const unsigned long long* rows = d_rows;
-        const uint32_t* dmap = d_dmap;
+        const ulonglong2* dbits = d_dbits;
         const uint32_t* dvert = d_dvert;
         if (le && !write)
             hipLaunchKernelGGL((mdrw_induced_kernel<kLE, false>), dim3(sgrid), dim3(kIndBlock), 0, s, dg, rows, W,
-                               dmap, dvert, (const uint64_t*)d_nu, d_ux, d_uk, d_pbase, d_xpair, d_coff, d_ccnt,
+                               dbits, dvert, (const uint64_t*)d_nu, d_ux, d_uk, d_pbase, d_xpair, d_coff, d_ccnt,
                                d_cpos, idst);
         else if (le)
             hipLaunchKernelGGL((mdrw_induced_kernel<kLE, true>), dim3(sgrid), dim3(kIndBlock), 0, s, dg, rows, W,
-                               dmap, dvert, (const uint64_t*)d_nu, d_ux, d_uk, d_pbase, d_xpair, d_coff, d_ccnt,
+                               dbits, dvert, (const uint64_t*)d_nu, d_ux, d_uk, d_pbase, d_xpair, d_coff, d_ccnt,
                                d_cpos, idst);
         else if (!write)
             hipLaunchKernelGGL((mdrw_induced_kernel<kGT, false>), dim3(sgrid), dim3(kIndBlock), 0, s, dg, rows, W,
-                               dmap, dvert, (const uint64_t*)d_nu, d_ux, d_uk, d_pbase, d_xpair, d_coff, d_ccnt,
+                               dbits, dvert, (const uint64_t*)d_nu, d_ux, d_uk, d_pbase, d_xpair, d_coff, d_ccnt,
                                d_cpos, idst);
         else
             hipLaunchKernelGGL((mdrw_induced_kernel<kGT, true>), dim3(sgrid), dim3(kIndBlock), 0, s, dg, rows, W,
-                               dmap, dvert, (const uint64_t*)d_nu, d_ux, d_uk, d_pbase, d_xpair, d_coff, d_ccnt,
+                               dbits, dvert, (const uint64_t*)d_nu, d_ux, d_uk, d_pbase, d_xpair, d_coff, d_ccnt,
                                d_cpos, idst);
         return hipGetLastError();
     };
