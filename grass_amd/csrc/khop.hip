@@ -241,14 +241,18 @@ __global__ __launch_bounds__(kKhBlock) void sample_kernel(DevGraph g, const uint
                                                           uint32_t* __restrict__ osrc,
                                                           uint32_t* __restrict__ odst, uint64_t cap,
                                                           uint32_t* err) {
+    // kWave / G groups of G lanes per wave; when G does not divide 64 the last lanes are idle
+    constexpr int GPW = kWave / G;
     const int lane = lane_id();
-    const int gl = lane % G;
-    const int gbase = lane - gl;
+    const int grp = lane / G;
+    const int gl = lane - grp * G;
+    const int gbase = grp * G;
     const uint64_t I = *icount;
     const uint64_t hbase = hb[h];
-    const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / G;
-    for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;; i += groups) {
-        const bool has = i < I;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / kWave * GPW;
+    for (uint64_t i = wave * GPW + grp;; i += groups) {
+        const bool has = grp < GPW && i < I;
         if (!__any(has)) break;  // wave-uniform exit
         uint32_t v = 0, c = 0, d = 0, inst = 0;
         uint64_t ebase = 0;
@@ -346,8 +350,9 @@ static unsigned sample_grid() {
     return grid;
 }
 
-// Hop h with G lanes per frontier vertex: the smallest of 8, 16, 32, 64 that covers the hop's
-// largest fanout, so that a fanout-10 hop does not leave two thirds of a 32-lane group idle.
+// Hop h with G lanes per frontier vertex: the smallest of 8, 10, 12, 16, 21, 32, 64 that covers
+// the hop's largest fanout (times the tuned scale), so that a fanout-10 hop runs six vertices per
+// wave instead of four in 16-lane groups.
 template <int FMT>
 static void launch_sample(grass_graph* g, uint32_t G, hipStream_t s, const uint32_t* fv, const uint32_t* fr,
                           const uint32_t* fd, const uint64_t* ic, const KhReq* reqs, uint32_t h, uint64_t seed, const uint64_t* eoff,
@@ -356,7 +361,10 @@ static void launch_sample(grass_graph* g, uint32_t G, hipStream_t s, const uint3
     hipLaunchKernelGGL((sample_kernel<FMT, GG>), dim3(sample_grid<FMT, GG>()), dim3(kKhBlock), 0, s, g->dev(), \
                        fv, fr, fd, ic, reqs, h, seed, eoff, hb, osrc, odst, cap, g->d_err)
     if (G <= 8) GRASS_SAMPLE(8);
+    else if (G <= 10) GRASS_SAMPLE(10);
+    else if (G <= 12) GRASS_SAMPLE(12);
     else if (G <= 16) GRASS_SAMPLE(16);
+    else if (G <= 21) GRASS_SAMPLE(21);
     else if (G <= 32) GRASS_SAMPLE(32);
     else GRASS_SAMPLE(64);
 #undef GRASS_SAMPLE
@@ -492,9 +500,14 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
         uint32_t f = 0;
         for (uint32_t r = 0; r < R; ++r)
             if (h < reqs[r].hops) f = std::max(f, reqs[r].fanout[h]);
-        uint32_t G = 8;
-        while (G < f) G <<= 1;
-        lanes[h] = std::min<uint32_t>(64, G * g->tune.khop_lane_scale.load(std::memory_order_relaxed));
+        const uint32_t need = f * g->tune.khop_lane_scale.load(std::memory_order_relaxed);
+        uint32_t G = 64;
+        for (uint32_t c : {8u, 10u, 12u, 16u, 21u, 32u})
+            if (c >= need) {
+                G = c;
+                break;
+            }
+        lanes[h] = G;
     }
     for (uint32_t h = 0; h < H; ++h) {
         const int cur = h & 1, nxt = cur ^ 1;
