@@ -153,6 +153,8 @@ def main():
                     help="run one 256 MiB device copy (a known-size kernel that scales rocprof "
                          "FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
+    if args.rw_requests < 1:
+        ap.error("--rw-requests must be at least 1: the roofline is taken on the walk kernel")
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): requests shard "
