@@ -398,7 +398,9 @@ def main():
             "traffic": None, "bytes_per_launch": bytes_launch,
             "bytes_per_edge": round(bytes_launch / rw_edges, 3), "launch_ms": round(kern_ms, 4)}
 
-    traffic, tcc_hit = profile_figures(args.scale, args.fmt)
+    # the committed summary is of the default request mix only
+    default_mix = args.rw_requests == 256 and args.khop_requests == 256
+    traffic, tcc_hit = profile_figures(args.scale, args.fmt) if default_mix else (None, None)
     if tcc_hit is not None:
         roof["tcc_hit_rate"] = tcc_hit
     if traffic is not None:
