@@ -49,3 +49,15 @@ def test_export_argument_errors():
     assert L.grass_khop_blocks_build(None, None, 0, None, None, None) == grass_amd.EINVAL
     assert L.grass_mdrw_local_edges(None, None, 0, None, None, None) == grass_amd.EINVAL
     assert L.grass_compress_device(None, None, 0, 0, 0, None, None, None) == grass_amd.EINVAL
+
+
+def test_device_ingest_argument_errors():
+    """Argument checks that come before any device work."""
+    from grass_amd import ingest
+    import grass_amd
+    L = ingest.lib()
+    assert L.grass_rmat_edges_device(0, 16, 0.57, 0.19, 0.19, 1, 0, None, None, None) == grass_amd.EINVAL
+    assert L.grass_rmat_edges_device(10, 16, 0.6, 0.3, 0.3, 1, 0, None, None, None) == grass_amd.EINVAL
+    n, m = C.c_uint64(), C.c_uint64()
+    assert L.grass_graph_sort_device(None, None, 5, 0, 0, None, None, 0, None, 0, C.byref(n),
+                                     C.byref(m)) == grass_amd.EINVAL

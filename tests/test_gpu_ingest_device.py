@@ -93,3 +93,27 @@ def test_device_pipeline_equals_host_files(env, fmt):
     hs, hd = ingest.rmat_edges(20, 16, seed=11, threads=8)
     hg, ho = ingest.graph_sort(hs, hd, threads=8).compress(fmt, threads=8)
     assert gb == hg and ob == ho
+
+
+def test_graph_sort_device_capacity(env):
+    """Too small output buffers: GRASS_ECAPACITY with the sizes reported, nothing written."""
+    import ctypes as C
+    torch, ingest = env
+    import grass_amd
+    src = np.array([0, 1, 2, 3], np.uint32)
+    dst = np.array([1, 2, 3, 0], np.uint32)
+    s = torch.from_numpy(src.view(np.int32)).cuda()
+    d = torch.from_numpy(dst.view(np.int32)).cuda()
+    rp = torch.zeros(8, dtype=torch.int64, device="cuda")
+    col = torch.zeros(16, dtype=torch.int32, device="cuda")
+    n, m = C.c_uint64(), C.c_uint64()
+    L = ingest.lib()
+    rc = L.grass_graph_sort_device(C.c_void_p(s.data_ptr()), C.c_void_p(d.data_ptr()), 4, 0, 0, None,
+                                   C.c_void_p(rp.data_ptr()), 2, C.c_void_p(col.data_ptr()), 16, C.byref(n),
+                                   C.byref(m))
+    assert rc == grass_amd.ECAPACITY and n.value == 4
+    rc = L.grass_graph_sort_device(C.c_void_p(s.data_ptr()), C.c_void_p(d.data_ptr()), 4, 0, 0, None,
+                                   C.c_void_p(rp.data_ptr()), 8, C.c_void_p(col.data_ptr()), 3, C.byref(n),
+                                   C.byref(m))
+    assert rc == grass_amd.ECAPACITY and n.value == 4 and m.value == 8
+    assert int(col.abs().sum().item()) == 0  # nothing written
