@@ -169,7 +169,7 @@ int graph_sort_device(const uint32_t* src, const uint32_t* dst, uint64_t m, int 
     DevMem d_first, d_rank, d_fkey, d_fid, d_newid, d_part;
     const uint64_t nbI = scan_blocks(std::max<uint64_t>(nid, 1)), nbP = scan_blocks(std::max<uint64_t>(P, 1));
     if (d_first.alloc(std::max<uint64_t>(nid, 1) * 8) != hipSuccess ||
-        d_rank.alloc((std::max<uint64_t>(nid, P) + 1) * 8) != hipSuccess ||
+        d_rank.alloc((nid + 1) * 8) != hipSuccess ||
         d_part.alloc((std::max(nbI, nbP) + 2) * 8) != hipSuccess)
         return GRASS_ENOMEM;
     unsigned long long* first = d_first.as<unsigned long long>();
@@ -211,14 +211,17 @@ int graph_sort_device(const uint32_t* src, const uint32_t* dst, uint64_t m, int 
     d_first.reset();
     d_fkey.reset();
     d_fid.reset();
+    d_rank.reset();
 
     // 3-4: keys, sorted over the bits a key of the renumbered ids can set (~0 still sorts last)
+    // two buffers of P + 1 keys: the sort's double buffer, and after it the free one holds the
+    // ranks of the distinct keys (P + 1 entries)
     DevMem d_keys;
-    if (d_keys.alloc(std::max<uint64_t>(P, 1) * 16) != hipSuccess) return GRASS_ENOMEM;
+    if (d_keys.alloc((P + 1) * 16) != hipSuccess) return GRASS_ENOMEM;
     unsigned long long* keys = d_keys.as<unsigned long long>();
     if (m) hipLaunchKernelGGL(edge_keys_kernel, dim3(gm), dim3(kIgBlock), 0, s, src, dst, m,
                               (const uint32_t*)d_newid.as<uint32_t>(), per, keys);
-    rocprim::double_buffer<unsigned long long> kb(keys, keys + std::max<uint64_t>(P, 1));
+    rocprim::double_buffer<unsigned long long> kb(keys, keys + P + 1);
     {
         size_t tb = 0;
         const unsigned eb = 32u + (unsigned)std::max(1, bits_for(n ? n - 1 : 0));
@@ -230,15 +233,17 @@ int graph_sort_device(const uint32_t* src, const uint32_t* dst, uint64_t m, int 
         if (e != hipSuccess) return status_of(e);
     }
     const unsigned long long* sk = kb.current();
-    e = exclusive_scan(UniqGen{sk}, Count{P, nullptr}, P, rank, d_part.as<uint64_t>(), d_tot, s);
+    uint64_t* krank = (uint64_t*)kb.alternate();
+    e = exclusive_scan(UniqGen{sk}, Count{P, nullptr}, P, krank, d_part.as<uint64_t>(), d_tot, s);
     uint64_t M = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&M, rank + P, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&M, krank + P, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return status_of(e);
     *m_out = M;
     if (col_cap < M || (M && !col)) return GRASS_ECAPACITY;
     const unsigned gp = (unsigned)std::min<uint64_t>((P + kIgBlock) / kIgBlock + 1, kIgGrid);
-    hipLaunchKernelGGL(csr_kernel, dim3(gp), dim3(kIgBlock), 0, s, sk, P, (const uint64_t*)rank, n, rowptr, col);
+    hipLaunchKernelGGL(csr_kernel, dim3(gp), dim3(kIgBlock), 0, s, sk, P, (const uint64_t*)krank, n, rowptr,
+                       col);
     e = hipStreamSynchronize(s);
     return status_of(e);
 }

@@ -1,8 +1,9 @@
 /*
- * grass_ingest.h -- host-side ingest of libgrass_amd.so: synthetic R-MAT edge lists, the
- * reference's graph_sort preprocessing, and the LE / GT compressors that write the
- * .graph/.offset format.  All parallel (std::thread), streaming over flat arrays, and
- * byte-identical to the reference tools on the same input.
+ * grass_ingest.h -- ingest of libgrass_amd.so: synthetic R-MAT edge lists, the reference's
+ * graph_sort preprocessing, and the LE / GT compressors that write the .graph/.offset format.
+ * The host calls are parallel (std::thread) over flat arrays; the *_device calls do the same
+ * on a GPU over arrays in its memory.  All of them are byte-identical to the reference tools on
+ * the same input.
  */
 #ifndef GRASS_INGEST_H
 #define GRASS_INGEST_H
@@ -66,7 +67,7 @@ int grass_rmat_edges_device(uint32_t scale, uint32_t edge_factor, double a, doub
  * CSR, written to rowptr_dev (n + 1 entries) and col_dev (*m_out entries).  *n and *m_out receive
  * the sizes; GRASS_ECAPACITY (sizes set, CSR not written) when rowptr_capacity < n + 1 or
  * col_capacity < *m_out.  n <= max id + 1 and *m_out <= m (directed) or 2m (undirected).  Device
- * working memory is about 20 bytes per (directed) edge.  Synchronises on `stream`. */
+ * working memory is about 16 bytes per directed edge.  Synchronises on `stream`. */
 int grass_graph_sort_device(const uint32_t *src_dev, const uint32_t *dst_dev, uint64_t m,
                             int directed, int device, void *stream, uint64_t *rowptr_dev,
                             uint64_t rowptr_capacity, uint32_t *col_dev, uint64_t col_capacity,
