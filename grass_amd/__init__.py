@@ -126,6 +126,8 @@ def lib():
             "grass_strerror": (C.c_char_p, [I32]),
             "grass_open": (I32, [C.c_char_p, I32, I32, P]),
             "grass_open_memory": (I32, [P, U64, P, U64, I32, I32, P]),
+            "grass_open_device_images": (I32, [P, U64, P, U64, I32, I32, P]),
+            "grass_compress_device_open": (I32, [P, P, U64, I32, I32, P, P]),
             "grass_close": (I32, [P]),
             "grass_info": (I32, [P, P]),
             "grass_device_arrays": (I32, [P, P, P, P]),
@@ -219,6 +221,26 @@ class Graph:
     def open(cls, prefix, fmt="le", device=0):
         h = C.c_void_p()
         _check(lib().grass_open(os.fsencode(prefix), _FMT[fmt], device, C.byref(h)), "grass_open")
+        return cls(h, device)
+
+    @classmethod
+    def from_csr_device(cls, rowptr, col, fmt="le", device=0, stream=None):
+        """Encodes a CSR held in device tensors (rowptr int64 [n+1], col int32 [m], rows sorted
+        and free of duplicates) and loads the result, all on the GPU (grass_compress_device_open)."""
+        h = C.c_void_p()
+        _check(lib().grass_compress_device_open(C.c_void_p(rowptr.data_ptr()), C.c_void_p(col.data_ptr()),
+                                                rowptr.numel() - 1, _FMT[fmt], device,
+                                                C.c_void_p(_stream_ptr(stream)), C.byref(h)),
+               "grass_compress_device_open")
+        return cls(h, device)
+
+    @classmethod
+    def from_device_images(cls, graph_t, offset_t, fmt="le", device=0):
+        """Loads .graph/.offset images held in device tensors (uint8), as from_bytes does."""
+        h = C.c_void_p()
+        _check(lib().grass_open_device_images(C.c_void_p(graph_t.data_ptr()), graph_t.numel(),
+                                              C.c_void_p(offset_t.data_ptr()), offset_t.numel(), _FMT[fmt],
+                                              device, C.byref(h)), "grass_open_device_images")
         return cls(h, device)
 
     @classmethod

@@ -108,6 +108,11 @@ void free_graph(grass_graph* g) {
     delete g;
 }
 
+
+hipStream_t pick(grass_graph* g, void* stream) { return stream ? (hipStream_t)stream : g->stream; }
+
+}  // namespace
+
 int open_sources(ByteSource& gs, ByteSource& os, int format, int device, grass_graph** out) {
     if (!out) return GRASS_EINVAL;
     *out = nullptr;
@@ -140,9 +145,6 @@ int open_sources(ByteSource& gs, ByteSource& os, int format, int device, grass_g
     return GRASS_OK;
 }
 
-hipStream_t pick(grass_graph* g, void* stream) { return stream ? (hipStream_t)stream : g->stream; }
-
-}  // namespace
 }  // namespace grass
 
 using namespace grass;
@@ -173,6 +175,13 @@ int grass_open(const char* prefix, int format, int device, grass_graph** out) {
         *out = nullptr;
         return GRASS_EIO;
     }
+    return open_sources(gs, os, format, device, out);
+}
+
+int grass_open_device_images(const void* graph_dev, uint64_t graph_bytes, const void* offset_dev,
+                             uint64_t offset_bytes, int format, int device, grass_graph** out) {
+    if (!graph_dev || !offset_dev || !out) return GRASS_EINVAL;
+    DeviceSource gs(graph_dev, graph_bytes), os(offset_dev, offset_bytes);
     return open_sources(gs, os, format, device, out);
 }
 

@@ -377,8 +377,14 @@ int fetch_to_host(uint8_t* dst, const void* src, uint64_t len, hipStream_t s) {
     return rc;
 }
 
-int compress_device(const uint64_t* rowptr, const uint32_t* col, uint64_t n, int format, hipStream_t s,
-                    grass_blob* gb, grass_blob* ob) {
+// The .graph and .offset images in device memory (the files' bytes), complete when this returns.
+struct Images {
+    DevMem g, o;
+    uint64_t gbytes = 0, obytes = 0;
+};
+
+int encode_images(const uint64_t* rowptr, const uint32_t* col, uint64_t n, int format, hipStream_t s,
+                  Images& im) {
     const bool le = format == GRASS_FMT_LE;
     const uint64_t nb = scan_blocks(n);
     DevMem tmp;  // len[n], start[n + 1], widths[n], partials, total, maxlen
@@ -417,7 +423,7 @@ int compress_device(const uint64_t* rowptr, const uint32_t* col, uint64_t n, int
     if (total >= (1ull << 40)) return GRASS_EINVAL;  // past what a reader's 40-bit offsets address
     // .graph
     const uint64_t gwords = (total + 31) / 32 + 2;
-    DevMem gbuf;
+    DevMem& gbuf = im.g;
     if (gbuf.alloc(gwords * 4) != hipSuccess) return GRASS_ENOMEM;
     uint32_t* d_g = (uint32_t*)gbuf.p;
     e = hipMemsetAsync(d_g, 0, gwords * 4, s);
@@ -433,7 +439,7 @@ int compress_device(const uint64_t* rowptr, const uint32_t* col, uint64_t n, int
     while (ubl < 64 && (maxlen >> ubl)) ++ubl;
     const uint64_t obits = 8 + n * (uint64_t)ubl;
     const uint64_t owords = (obits + 63) / 64 + 1;
-    DevMem obuf;
+    DevMem& obuf = im.o;
     if (obuf.alloc(owords * 8) != hipSuccess) return GRASS_ENOMEM;
     unsigned long long* d_o = (unsigned long long*)obuf.p;
     e = hipMemsetAsync(d_o, 0, owords * 8, s);
@@ -444,10 +450,21 @@ int compress_device(const uint64_t* rowptr, const uint32_t* col, uint64_t n, int
                        (const uint64_t*)d_len, n, ubl, d_o);
     hipLaunchKernelGGL(bswap64_kernel, dim3(256), dim3(256), 0, s, d_o, owords);
     e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return status_of(e);
-    if (!gb->resize((total + 7) / 8) || !ob->resize((obits + 7) / 8)) return GRASS_ENOMEM;
-    int rc = fetch_to_host(gb->data(), d_g, gb->size(), s);
-    return rc ? rc : fetch_to_host(ob->data(), d_o, ob->size(), s);
+    im.gbytes = (total + 7) / 8;
+    im.obytes = (obits + 7) / 8;
+    return GRASS_OK;
+}
+
+int compress_device(const uint64_t* rowptr, const uint32_t* col, uint64_t n, int format, hipStream_t s,
+                    grass_blob* gb, grass_blob* ob) {
+    Images im;
+    int rc = encode_images(rowptr, col, n, format, s, im);
+    if (rc) return rc;
+    if (!gb->resize(im.gbytes) || !ob->resize(im.obytes)) return GRASS_ENOMEM;
+    rc = fetch_to_host(gb->data(), im.g.p, gb->size(), s);
+    return rc ? rc : fetch_to_host(ob->data(), im.o.p, ob->size(), s);
 }
 
 }  // namespace
@@ -477,6 +494,27 @@ int grass_compress_device(const uint64_t* rowptr_dev, const uint32_t* col_dev, u
     *graph = gb;
     *offset = ob;
     return GRASS_OK;
+}
+
+int grass_compress_device_open(const uint64_t* rowptr_dev, const uint32_t* col_dev, uint64_t n, int format,
+                               int device, void* stream, grass_graph** out) {
+    if (!rowptr_dev || !out || n == 0 || n >= 0x80000000ull || (format != GRASS_FMT_LE && format != GRASS_FMT_GT))
+        return GRASS_EINVAL;
+    *out = nullptr;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    if (hipSetDevice(device) != hipSuccess) return GRASS_EDEVICE;
+    int rc;
+    {
+        grass::Images im;
+        rc = grass::encode_images(rowptr_dev, col_dev, n, format, (hipStream_t)stream, im);
+        if (!rc) {
+            grass::DeviceSource gs(im.g.p, im.gbytes), os(im.o.p, im.obytes);
+            rc = grass::open_sources(gs, os, format, device, out);
+        }
+    }
+    (void)hipSetDevice(prev);
+    return rc;
 }
 
 }  // extern "C"

@@ -243,6 +243,15 @@ int stage_to_device(ByteSource& src, char* dst, hipStream_t stream, uint64_t* ta
                     unsigned tail_len, uint8_t* tail_out) {
     const uint64_t total = src.size();
     if (total == 0) return GRASS_OK;
+    if (const uint8_t* dev = src.device_data()) {  // already in device memory: one copy, no staging
+        const uint64_t tl = std::min<uint64_t>(tail_len, total);
+        hipError_t e = hipMemcpyAsync(dst, dev, total, hipMemcpyDeviceToDevice, stream);
+        if (e == hipSuccess && tail_out && tl)
+            e = hipMemcpyAsync(tail_out, dev + total - tl, tl, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (tail) *tail = tl;
+        return e == hipSuccess ? GRASS_OK : GRASS_EDEVICE;
+    }
     const uint64_t chunk = std::min<uint64_t>(kChunk, total);
     void* pin[2] = {nullptr, nullptr};
     hipEvent_t ev[2] = {nullptr, nullptr};

@@ -205,10 +205,23 @@ struct ByteSource {
     // The whole stream in host memory (a mapping or a caller's buffer), or nullptr when it can
     // only be read in order.
     virtual const uint8_t* data() const { return nullptr; }
+    // The whole stream in device memory of the graph's device (then data() and read() are unused).
+    virtual const uint8_t* device_data() const { return nullptr; }
     // Reads len bytes at the current position; false on a short read.
     virtual bool read(void* dst, uint64_t len) = 0;
 };
 int load_graph(grass_graph* g, ByteSource& graph_src, ByteSource& offset_src);
+// capi.cpp: a graph on `device` from two sources (grass_open and its variants)
+int open_sources(ByteSource& graph_src, ByteSource& offset_src, int format, int device, grass_graph** out);
+// Images already in device memory.
+struct DeviceSource : ByteSource {
+    const uint8_t* p;
+    uint64_t sz;
+    DeviceSource(const void* ptr, uint64_t n) : p((const uint8_t*)ptr), sz(n) {}
+    uint64_t size() const override { return sz; }
+    const uint8_t* device_data() const override { return p; }
+    bool read(void*, uint64_t) override { return false; }
+};
 // Host threads for copies between pageable and pinned memory (GRASS_LOAD_THREADS, default <= 8).
 unsigned load_threads();
 void parallel_copy(void* dst, const uint8_t* src, uint64_t len, unsigned threads);

@@ -69,6 +69,17 @@ int grass_open(const char *prefix, int format, int device, grass_graph **out);
 int grass_open_memory(const void *graph, uint64_t graph_bytes, const void *offset,
                       uint64_t offset_bytes, int format, int device, grass_graph **out);
 
+/* Same as grass_open_memory from file images already in device memory of `device` (copied on the
+ * device, no host staging).  No reference counterpart: the reference reads files only. */
+int grass_open_device_images(const void *graph_dev, uint64_t graph_bytes, const void *offset_dev,
+                             uint64_t offset_bytes, int format, int device, grass_graph **out);
+
+/* Encodes a CSR in device memory with the LE or GT compressor (as grass_compress_device in
+ * grass_ingest.h) and loads the images it made as grass_open would load their files, without
+ * either leaving the GPU.  No reference counterpart; for a graph built on the GPU. */
+int grass_compress_device_open(const uint64_t *rowptr_dev, const uint32_t *col_dev, uint64_t n,
+                               int format, int device, void *stream, grass_graph **out);
+
 /* Frees every device and host resource of the graph.  NULL is accepted. */
 int grass_close(grass_graph *g);
 

@@ -117,3 +117,29 @@ def test_graph_sort_device_capacity(env):
                                    C.byref(m))
     assert rc == grass_amd.ECAPACITY and n.value == 4 and m.value == 8
     assert int(col.abs().sum().item()) == 0  # nothing written
+
+
+@pytest.mark.parametrize("fmt", ["le", "gt"])
+def test_compress_device_open_and_device_images(env, fmt):
+    """A graph encoded on the GPU and loaded from its device images (no host round trip) is the
+    graph its host files load to: same sizes, same CSR, same walks."""
+    print(f"[ingest device] compress + open on the device fmt={fmt}", flush=True)
+    torch, ingest = env
+    import grass_amd
+    s, d = ingest.rmat_edges_device(16, 16, seed=5)
+    rp, col = ingest.graph_sort_device(s, d)
+    gb, ob = ingest.compress_device(rp, col, fmt)
+    ref = grass_amd.Graph.from_bytes(gb, ob, fmt)
+    gd = grass_amd.Graph.from_csr_device(rp, col, fmt)
+    gi = grass_amd.Graph.from_device_images(torch.frombuffer(bytearray(gb), dtype=torch.uint8).cuda(),
+                                            torch.frombuffer(bytearray(ob), dtype=torch.uint8).cuda(), fmt)
+    erp, ecol = rp.cpu().numpy().astype(np.uint64), col.cpu().numpy().view(np.uint32)
+    wref = grass_amd.to_numpy_u32(ref.random_walk(4096, 30, seed=3))
+    for g in (gd, gi):
+        assert g.info()["graph_bytes"] == len(gb) and g.info()["offset_bytes"] == len(ob)
+        r2, c2 = g.decode_csr()
+        np.testing.assert_array_equal(r2, erp)
+        np.testing.assert_array_equal(c2, ecol)
+        np.testing.assert_array_equal(grass_amd.to_numpy_u32(g.random_walk(4096, 30, seed=3)), wref)
+        g.close()
+    ref.close()
