@@ -58,9 +58,10 @@ def serving_mix(rng, n_req, n_vertices, max_seeds=4096):
     return reqs
 
 
-def serving_bench(g, seed, rng, seconds=1.5, burst=6000):
+def serving_bench(g, seed, rng, seconds=3.0, burst=6000):
     """Saturation QPS from a burst replay (all requests due at once), then p50/p99 latency of an
-    open-loop Poisson replay at 70 % of it."""
+    open-loop Poisson replay at 70 % of it for `seconds` (long enough that a single host stall of a
+    few milliseconds moves p99 little)."""
     from grass_amd import serve
     with serve.Session(g, memory_budget=24 << 30, max_batch=512, seed=seed) as s:
         warm = serving_mix(rng, 500, g.n)

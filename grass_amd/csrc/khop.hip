@@ -350,9 +350,9 @@ static unsigned sample_grid() {
     return grid;
 }
 
-// Hop h with G lanes per frontier vertex: the smallest of 8, 10, 12, 16, 21, 32, 64 that covers
-// the hop's largest fanout (times the tuned scale), so that a fanout-10 hop runs six vertices per
-// wave instead of four in 16-lane groups.
+// Hop h with G lanes per frontier vertex: the smallest of 4, 6, 8, 10, 12, 16, 21, 32, 64 that
+// covers the hop's largest fanout (times the tuned scale), so that a fanout-10 hop runs six
+// vertices per wave instead of four in 16-lane groups.
 template <int FMT>
 static void launch_sample(grass_graph* g, uint32_t G, hipStream_t s, const uint32_t* fv, const uint32_t* fr,
                           const uint32_t* fd, const uint64_t* ic, const KhReq* reqs, uint32_t h, uint64_t seed, const uint64_t* eoff,
@@ -360,7 +360,9 @@ static void launch_sample(grass_graph* g, uint32_t G, hipStream_t s, const uint3
 #define GRASS_SAMPLE(GG)                                                                                   \
     hipLaunchKernelGGL((sample_kernel<FMT, GG>), dim3(sample_grid<FMT, GG>()), dim3(kKhBlock), 0, s, g->dev(), \
                        fv, fr, fd, ic, reqs, h, seed, eoff, hb, osrc, odst, cap, g->d_err)
-    if (G <= 8) GRASS_SAMPLE(8);
+    if (G <= 4) GRASS_SAMPLE(4);
+    else if (G <= 6) GRASS_SAMPLE(6);
+    else if (G <= 8) GRASS_SAMPLE(8);
     else if (G <= 10) GRASS_SAMPLE(10);
     else if (G <= 12) GRASS_SAMPLE(12);
     else if (G <= 16) GRASS_SAMPLE(16);
@@ -502,7 +504,7 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
             if (h < reqs[r].hops) f = std::max(f, reqs[r].fanout[h]);
         const uint32_t need = f * g->tune.khop_lane_scale.load(std::memory_order_relaxed);
         uint32_t G = 64;
-        for (uint32_t c : {8u, 10u, 12u, 16u, 21u, 32u})
+        for (uint32_t c : {4u, 6u, 8u, 10u, 12u, 16u, 21u, 32u})
             if (c >= need) {
                 G = c;
                 break;
