@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by the
-cpu_baseline leg of bench.py, as the checker.  The product (grass_amd) never imports it.
+cpu_baseline leg of bench.py, to check results against.  The product (grass_amd) never
+imports it.
 The C restatement and its reference citations are in grass_oracle.c.
 """
 import ctypes as C

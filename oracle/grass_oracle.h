@@ -2,8 +2,8 @@
  * grass_oracle.h -- CPU oracle for the GraSS compressed-graph sampling path.
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference's decoder and of
- * the sampling semantics, used as the checker by tests/, by __graft_entry__.smoke() and by the
- * cpu_baseline leg of bench.py.  No product code links, loads or calls it.
+ * the sampling semantics, which tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py check results against.  No product code links, loads or calls it.
  *
  * Reference anchors (paths relative to the reference checkout):
  *   .graph word view  LE/src/grass.cu:1036-1050      .offset fields  LE/src/bcg.cuh:284-329
