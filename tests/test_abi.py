@@ -61,3 +61,11 @@ def test_device_ingest_argument_errors():
     n, m = C.c_uint64(), C.c_uint64()
     assert L.grass_graph_sort_device(None, None, 5, 0, 0, None, None, 0, None, 0, C.byref(n),
                                      C.byref(m)) == grass_amd.EINVAL
+
+
+def test_device_open_argument_errors():
+    import grass_amd
+    L = grass_amd.lib()
+    h = C.c_void_p()
+    assert L.grass_open_device_images(None, 0, None, 0, 0, 0, C.byref(h)) == grass_amd.EINVAL
+    assert L.grass_compress_device_open(None, None, 0, 0, 0, None, C.byref(h)) == grass_amd.EINVAL
