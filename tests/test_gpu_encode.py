@@ -122,3 +122,26 @@ def test_encode_rejects_empty(env):
     cl = torch.zeros(1, dtype=torch.int32, device="cuda")
     with pytest.raises(Exception):
         ingest.compress_device(rp, cl, "le")
+
+
+@pytest.mark.parametrize("fmt", ["le", "gt"])
+def test_encode_one_huge_list(env, fmt):
+    """A vertex with 300,000 neighbours (thousands of 64-field passes, wide rocks and seas) among
+    short lists: the same bytes as the host compressor."""
+    print(f"[encode huge list] fmt={fmt}", flush=True)
+    torch, ingest = env
+    rng = np.random.default_rng(77)
+    n = 400000
+    short = np.repeat(np.arange(n, dtype=np.int64), rng.integers(0, 4, size=n))
+    short = short[short != 1234]
+    keys = [short * n + rng.integers(0, n, size=len(short)),
+            1234 * n + rng.choice(n, size=300000, replace=False).astype(np.int64),
+            np.array([(n - 1) * n + (n - 1)], np.int64)]
+    keys = np.unique(np.concatenate(keys))  # sorted (row, column) pairs, no repeats
+    src, col = (keys // n).astype(np.uint32), (keys % n).astype(np.uint32)
+    rowptr = np.zeros(n + 1, np.uint64)
+    rowptr[1:] = np.cumsum(np.bincount(src, minlength=n))
+    assert int(np.diff(rowptr.astype(np.int64))[1234]) == 300000
+    csr = ingest.csr_from_sorted_edges(src, col)
+    assert csr.n == n
+    _check(torch, ingest, rowptr, col, fmt, csr.compress(fmt, threads=8))
