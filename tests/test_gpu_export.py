@@ -171,3 +171,35 @@ def test_khop_blocks_reject_offsets_of_another_batch(ga):
     res = g.khop_minibatch(seeds, [[5, 5], [5, 5]], seed=3)
     _check_blocks(res, seeds)
     g.close()
+
+
+def test_mdrw_dlpack_memory_goes_with_the_last_tensor(ga):
+    """The result's device memory stays while any exported tensor lives (after grass_mdrw_free)
+    and is returned when the last one goes."""
+    import torch
+
+    from grass_amd import ingest
+    print("[export mdrw dlpack] memory lifetime", flush=True)
+    s, d = ingest.rmat_edges_device(18, 16, seed=8)
+    rp, col = ingest.graph_sort_device(s, d)
+    del s, d
+    g = ga.Graph.from_csr_device(rp, col, "gt")
+    del rp, col
+    sizes = [4000] * 48
+    g.mdrw_tensors(sizes, target=16384, seed=1, local=False)  # the graph's workspaces grow here
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    t = g.mdrw_tensors(sizes, target=16384, seed=1, local=False)
+    keep = t["ind_dst"]
+    held = sum(x.numel() * x.element_size() for x in t.values())
+    assert held > 64 << 20
+    del t
+    torch.cuda.synchronize()
+    free1 = torch.cuda.mem_get_info()[0]
+    assert free0 - free1 >= held // 2  # one tensor still holds the whole result
+    assert int(keep[-1].item()) >= 0
+    del keep
+    torch.cuda.synchronize()
+    free2 = torch.cuda.mem_get_info()[0]
+    assert free2 - free1 >= held // 2  # returned with the last tensor
+    g.close()
