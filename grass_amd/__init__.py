@@ -113,13 +113,27 @@ MDRW_MAX_TARGET = 16384
 _lib = None
 
 
+def _bind_torch_runtime():
+    """Imports torch (when installed) before the library is loaded.  torch's wheel carries its own
+    HIP runtime and loads it by file name (libamdhip64.so); the library asks for the soname
+    libamdhip64.so.7, which the dynamic loader then resolves to torch's copy.  Loaded the other
+    way round, the library binds the system runtime and torch loads a second HIP and HSA runtime
+    beside it, and kernels launched through one runtime on memory allocated by the other fail."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
-    """Loads libgrass_amd.so; raises if it has not been built."""
+    """Loads libgrass_amd.so (after torch, so that both use one HIP runtime); raises if it has not
+    been built."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C grass_amd` "
                                "(or __graft_entry__.build())")
+        _bind_torch_runtime()
         L = C.CDLL(LIB_PATH)
         P, U64, U32, I32 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int
         sig = {

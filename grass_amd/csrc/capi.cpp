@@ -85,7 +85,7 @@ struct DeviceGuard {
     bool ok = false;
     explicit DeviceGuard(int dev) {
         if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        ok = hipSetDevice(dev) == hipSuccess;
+        ok = grass::enter_device(dev);
     }
     ~DeviceGuard() {
         if (prev >= 0) (void)hipSetDevice(prev);

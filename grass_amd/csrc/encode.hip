@@ -480,7 +480,7 @@ int grass_compress_device(const uint64_t* rowptr_dev, const uint32_t* col_dev, u
     *graph = *offset = nullptr;
     int prev = 0;
     (void)hipGetDevice(&prev);
-    if (hipSetDevice(device) != hipSuccess) return GRASS_EDEVICE;
+    if (!grass::enter_device(device)) return GRASS_EDEVICE;
     grass_blob* gb = new (std::nothrow) grass_blob();
     grass_blob* ob = new (std::nothrow) grass_blob();
     int rc = gb && ob ? grass::compress_device(rowptr_dev, col_dev, n, format, (hipStream_t)stream, gb, ob)
@@ -503,7 +503,7 @@ int grass_compress_device_open(const uint64_t* rowptr_dev, const uint32_t* col_d
     *out = nullptr;
     int prev = 0;
     (void)hipGetDevice(&prev);
-    if (hipSetDevice(device) != hipSuccess) return GRASS_EDEVICE;
+    if (!grass::enter_device(device)) return GRASS_EDEVICE;
     int rc;
     {
         grass::Images im;

@@ -337,7 +337,7 @@ int grass_khop_blocks_build(grass_graph* g, const grass_khop_request* reqs, uint
     }
     int prev = 0;
     (void)hipGetDevice(&prev);
-    if (hipSetDevice(g->device) != hipSuccess) return GRASS_EDEVICE;
+    if (!grass::enter_device(g->device)) return GRASS_EDEVICE;
     hipStream_t s = stream ? (hipStream_t)stream : g->stream;
     int rc = grass::khop_blocks(g, reqs, count, batch, out, s);
     (void)hipSetDevice(prev);
@@ -354,7 +354,7 @@ int grass_mdrw_local_edges(grass_graph* g, const grass_mdrw_result* res, int whi
     if (!lsrc_dev || !ldst_dev) return GRASS_EINVAL;
     int prev = 0;
     (void)hipGetDevice(&prev);
-    if (hipSetDevice(g->device) != hipSuccess) return GRASS_EDEVICE;
+    if (!grass::enter_device(g->device)) return GRASS_EDEVICE;
     hipStream_t s = stream ? (hipStream_t)stream : g->stream;
     const unsigned grid = (unsigned)std::min<uint64_t>((E + grass::kExBlock - 1) / grass::kExBlock, grass::kExGrid);
     hipLaunchKernelGGL(grass::mdrw_local_kernel, dim3(grid), dim3(grass::kExBlock), 0, s,

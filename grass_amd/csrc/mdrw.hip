@@ -806,7 +806,7 @@ int grass_mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t count, u
     if (!g) return GRASS_EINVAL;
     int prev = 0;
     (void)hipGetDevice(&prev);
-    if (hipSetDevice(g->device) != hipSuccess) return GRASS_EDEVICE;
+    if (!grass::enter_device(g->device)) return GRASS_EDEVICE;
     hipStream_t s = stream ? (hipStream_t)stream : g->stream;
     int rc = grass::mdrw(g, reqs, count, target, seed, s, out);
     (void)hipSetDevice(prev);

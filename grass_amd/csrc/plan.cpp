@@ -72,7 +72,7 @@ int grass_plan_create(grass_graph* g, const grass_rw_request* rw, uint32_t nrw, 
     *out = nullptr;
     int prev = 0;
     (void)hipGetDevice(&prev);
-    if (hipSetDevice(g->device) != hipSuccess) return GRASS_EDEVICE;
+    if (!grass::enter_device(g->device)) return GRASS_EDEVICE;
     grass_plan* p = new (std::nothrow) grass_plan();
     int rc = p ? GRASS_OK : GRASS_ENOMEM;
     if (!rc) {
@@ -118,7 +118,7 @@ int grass_plan_launch(grass_plan* p, void* stream) {
     if (!p) return GRASS_EINVAL;
     int prev = 0;
     (void)hipGetDevice(&prev);
-    if (hipSetDevice(p->g->device) != hipSuccess) return GRASS_EDEVICE;
+    if (!grass::enter_device(p->g->device)) return GRASS_EDEVICE;
     hipStream_t s = stream ? (hipStream_t)stream : p->g->stream;
     hipError_t e = hipGraphLaunch(p->exec, s);
     (void)hipSetDevice(prev);

@@ -187,6 +187,16 @@ namespace grass {
 
 int status_of(hipError_t e);
 
+// Makes `dev` current for an entry point.  HIP keeps the thread's last failed runtime call as its
+// "last error", and the launch checks here read that state after each launch; a failure the caller
+// made and ignored before entering the library must not be reported as one of this call's
+// launches, so the state is cleared on entry.
+inline bool enter_device(int dev) {
+    if (hipSetDevice(dev) != hipSuccess) return false;
+    (void)hipGetLastError();
+    return true;
+}
+
 // Workgroups that fill the current device once at the kernel's occupancy (persistent grids).
 inline unsigned persistent_grid(const void* kernel, int block) {
     int dev = 0, cus = 0, per = 0;

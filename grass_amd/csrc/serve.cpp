@@ -553,7 +553,7 @@ int grass_session_create(grass_graph* g, const grass_session_config* cfg, grass_
     s->cfg.max_inflight = std::min<uint32_t>(s->cfg.max_inflight, 64);
     int prev = 0;
     (void)hipGetDevice(&prev);
-    if (hipSetDevice(g->device) != hipSuccess) {
+    if (!grass::enter_device(g->device)) {
         delete s;
         return GRASS_EDEVICE;
     }

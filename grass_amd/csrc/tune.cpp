@@ -94,7 +94,7 @@ int grass_autotune(grass_graph* g, uint32_t what, grass_tuning* chosen, grass_tu
     memset(&r, 0, sizeof(r));
     int prev = 0;
     (void)hipGetDevice(&prev);
-    if (hipSetDevice(g->device) != hipSuccess) return GRASS_EDEVICE;
+    if (!grass::enter_device(g->device)) return GRASS_EDEVICE;
     hipStream_t s = g->stream;
     grass_tuning best = current(g);
     int rc = GRASS_OK;

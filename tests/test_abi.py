@@ -69,3 +69,19 @@ def test_device_open_argument_errors():
     h = C.c_void_p()
     assert L.grass_open_device_images(None, 0, None, 0, 0, 0, C.byref(h)) == grass_amd.EINVAL
     assert L.grass_compress_device_open(None, None, 0, 0, 0, None, C.byref(h)) == grass_amd.EINVAL
+
+
+def test_one_hip_runtime_when_the_library_loads_before_torch():
+    """A process whose first grass_amd call is a host one (no torch imported yet) still ends up
+    with a single HIP runtime: the library binds the copy torch loads."""
+    import subprocess
+    import sys
+    code = ("from grass_amd import ingest\n"
+            "ingest.rmat_edges(4, 2, seed=1, threads=1)\n"
+            "import torch\n"
+            "maps = open('/proc/self/maps').read().split('\\n')\n"
+            "print(sorted({l.split()[-1] for l in maps if 'libamdhip64' in l}))\n")
+    p = subprocess.run([sys.executable, "-c", code], cwd=REPO, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    paths = eval(p.stdout.strip().splitlines()[-1])
+    assert len(paths) == 1, paths

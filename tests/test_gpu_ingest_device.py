@@ -36,6 +36,25 @@ def test_rmat_device_equals_host(env, scale, abc, seed):
     np.testing.assert_array_equal(_u32(dd), hd)
 
 
+def test_device_call_after_failed_runtime_call(env):
+    """A failed HIP call left in the thread (here hipSetDevice on a device that does not exist,
+    made by a rejected call) is not reported by the next call's launch check."""
+    print("[ingest device] rmat after a failed hipSetDevice", flush=True)
+    torch, ingest = env
+    import ctypes as C
+    src = torch.empty(16 << 10, dtype=torch.int32, device="cuda:0")
+    dst = torch.empty_like(src)
+    args = (10, 16, 0.57, 0.19, 0.19, 42)
+    ptrs = (C.c_void_p(0), C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()))
+    L = ingest.lib()
+    assert L.grass_rmat_edges_device(*args, 4096, *ptrs) == 5  # GRASS_EDEVICE
+    assert L.grass_rmat_edges_device(*args, 0, *ptrs) == 0
+    torch.cuda.synchronize()
+    hs, hd = ingest.rmat_edges(10, 16, 0.57, 0.19, 0.19, seed=42, threads=4)
+    np.testing.assert_array_equal(_u32(src), hs)
+    np.testing.assert_array_equal(_u32(dst), hd)
+
+
 def _check_sort(torch, ingest, src, dst, directed, expect_rowptr, expect_col):
     s = torch.from_numpy(np.ascontiguousarray(src, dtype=np.uint32).view(np.int32)).cuda()
     d = torch.from_numpy(np.ascontiguousarray(dst, dtype=np.uint32).view(np.int32)).cuda()
