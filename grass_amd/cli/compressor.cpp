@@ -1,10 +1,12 @@
 // compressor: the reference's chunk_compress tool (LE/chunk_compress/compress.cpp,
 // GT/chunk_compress/compress.cpp) with the same command line, `<input_path> <output_path>
 // [<encode_type>]`, writing <output_path>.graph and <output_path>.offset byte-identical to it.
-// The reference builds one binary per codec; here `--format le|gt` (default le) selects it.
+// The reference builds one binary per codec; here `--format le|gt` (default le) selects it, and
+// `--device N` encodes on GPU N (the same bytes).
 // Only the "normal" encoding exists (the GT "head_bit" mode writes files its own decoder cannot
 // read).
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "../../include/grass.h"
@@ -12,6 +14,7 @@
 
 int main(int argc, char* argv[]) {
     int fmt = GRASS_FMT_LE;
+    int device = -1;
     const char* pos[3] = {nullptr, nullptr, nullptr};
     int np = 0;
     for (int i = 1; i < argc; ++i) {
@@ -20,6 +23,8 @@ int main(int argc, char* argv[]) {
             if (!strcmp(f, "le")) fmt = GRASS_FMT_LE;
             else if (!strcmp(f, "gt")) fmt = GRASS_FMT_GT;
             else np = 99;
+        } else if (!strcmp(argv[i], "--device") && i + 1 < argc) {
+            device = atoi(argv[++i]);
         } else if (np < 3) {
             pos[np++] = argv[i];
         } else {
@@ -28,7 +33,7 @@ int main(int argc, char* argv[]) {
     }
     if (np != 2 && np != 3) {
         printf("incorrect arguments.\n");
-        printf("<input_path> <output_path> [<encode_type>]   (--format le|gt)\n");
+        printf("<input_path> <output_path> [<encode_type>]   (--format le|gt, --device N)\n");
         return 1;
     }
     if (np == 3 && strcmp(pos[2], "normal") != 0) {
@@ -53,7 +58,8 @@ int main(int argc, char* argv[]) {
     }
     printf("%s graph loaded.\n", pos[0]);
     grass_blob *gb = nullptr, *ob = nullptr;
-    rc = grass_compress(csr, fmt, 0, &gb, &ob);
+    rc = device >= 0 ? grass_compress_on_device(csr, fmt, device, &gb, &ob)
+                     : grass_compress(csr, fmt, 0, &gb, &ob);
     if (!rc) rc = grass_write_files(pos[1], gb, ob);
     grass_blob_free(gb);
     grass_blob_free(ob);

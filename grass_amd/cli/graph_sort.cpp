@@ -1,14 +1,24 @@
 // graph_sort: the reference's preprocessing tool (LE/chunk_compress/graph_sort.cpp:22-82) with the
 // same command line and output: drops self loops, numbers vertices by first appearance, for `u`
 // (undirected) makes every pair (min, max) and adds its mirror, sorts, removes duplicates, and
-// writes "u v" lines.  Runs on the host's threads (grass_graph_sort).
+// writes "u v" lines.  Runs on the host's threads (grass_graph_sort), or with `--device N` on GPU N
+// (grass_graph_sort_on_device; the same output).
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "../../include/grass.h"
 #include "../../include/grass_ingest.h"
 
 int main(int argc, char* argv[]) {
+    int device = -1;
+    for (int i = 1; i + 1 < argc; ++i)  // `--device N` anywhere; the rest is the reference's command line
+        if (!strcmp(argv[i], "--device")) {
+            device = atoi(argv[i + 1]);
+            for (int j = i; j + 2 < argc; ++j) argv[j] = argv[j + 2];
+            argc -= 2;
+            break;
+        }
     if (argc < 4) {
         printf("incorrect arguments.\n");
         printf("<input_path> <output_path> <u/d> [<skip_line>]\n");
@@ -27,7 +37,8 @@ int main(int argc, char* argv[]) {
     uint64_t m;
     grass_edges_data(e, &src, &dst, &m);
     grass_csr* csr = nullptr;
-    rc = grass_graph_sort(src, dst, m, directed, 0, &csr);
+    rc = device >= 0 ? grass_graph_sort_on_device(src, dst, m, directed, device, &csr)
+                     : grass_graph_sort(src, dst, m, directed, 0, &csr);
     grass_edges_free(e);
     if (!rc) rc = grass_write_edge_list(argv[2], csr, 0);
     if (rc) {

@@ -1,9 +1,17 @@
-// Byte image handed out through the ingest ABI (include/grass_ingest.h: grass_blob_data).
+// Host objects handed out through the ingest ABI (include/grass_ingest.h): byte images
+// (grass_blob_data) and CSRs (grass_csr_arrays).
 #pragma once
 #include <stdint.h>
 
 #include <memory>
 #include <new>
+#include <vector>
+
+struct grass_csr {
+    uint64_t n = 0, m = 0;
+    std::vector<uint64_t> rowptr;
+    std::vector<uint32_t> col;
+};
 
 struct grass_blob {
     std::unique_ptr<uint8_t[]> buf;

@@ -30,12 +30,6 @@
 #include "codec.h"
 #include "philox.h"
 
-struct grass_csr {
-    uint64_t n = 0, m = 0;
-    std::vector<uint64_t> rowptr;
-    std::vector<uint32_t> col;
-};
-
 struct grass_edges {
     std::vector<uint32_t> src, dst;
 };

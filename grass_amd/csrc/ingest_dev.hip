@@ -18,6 +18,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "../../include/grass_ingest.h"
+#include "blob.h"
 #include "devutil.h"
 #include "philox.h"
 #include "runtime.h"
@@ -263,7 +264,7 @@ int grass_rmat_edges_device(uint32_t scale, uint32_t edge_factor, double a, doub
                    tABC = (uint64_t)((a + b + c) * two32);
     int prev = 0;
     (void)hipGetDevice(&prev);
-    if (hipSetDevice(device) != hipSuccess) return GRASS_EDEVICE;
+    if (!grass::enter_device(device)) return GRASS_EDEVICE;
     const unsigned grid = (unsigned)std::min<uint64_t>((m + grass::kIgBlock - 1) / grass::kIgBlock, grass::kIgGrid);
     hipLaunchKernelGGL(grass::rmat_kernel, dim3(grid), dim3(grass::kIgBlock), 0, (hipStream_t)stream, scale, m, tA,
                        tAB, tABC, seed, src_dev, dst_dev);
@@ -279,11 +280,84 @@ int grass_graph_sort_device(const uint32_t* src_dev, const uint32_t* dst_dev, ui
     *n = *m_out = 0;
     int prev = 0;
     (void)hipGetDevice(&prev);
-    if (hipSetDevice(device) != hipSuccess) return GRASS_EDEVICE;
+    if (!grass::enter_device(device)) return GRASS_EDEVICE;
     const int rc = grass::graph_sort_device(src_dev, dst_dev, m, directed, (hipStream_t)stream, rowptr_dev,
                                             rowptr_capacity, col_dev, col_capacity, n, m_out);
     (void)hipSetDevice(prev);
     return rc;
+}
+
+int grass_compress_on_device(const grass_csr* csr, int format, int device, grass_blob** graph, grass_blob** offset) {
+    if (!csr || !graph || !offset || csr->n == 0) return GRASS_EINVAL;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    if (!grass::enter_device(device)) return GRASS_EDEVICE;
+    int rc = GRASS_OK;
+    {
+        grass::DevMem rp, cl;
+        if (rp.alloc((csr->n + 1) * 8) != hipSuccess || cl.alloc(std::max<uint64_t>(csr->m, 1) * 4) != hipSuccess)
+            rc = GRASS_ENOMEM;
+        hipError_t e = hipSuccess;
+        if (!rc) e = hipMemcpy(rp.p, csr->rowptr.data(), (csr->n + 1) * 8, hipMemcpyHostToDevice);
+        if (!rc && e == hipSuccess && csr->m) e = hipMemcpy(cl.p, csr->col.data(), csr->m * 4, hipMemcpyHostToDevice);
+        if (!rc && e != hipSuccess) rc = grass::status_of(e);
+        if (!rc)
+            rc = grass_compress_device(rp.as<uint64_t>(), cl.as<uint32_t>(), csr->n, format, device, nullptr, graph,
+                                       offset);
+    }
+    (void)hipSetDevice(prev);
+    return rc;
+}
+
+int grass_graph_sort_on_device(const uint32_t* src, const uint32_t* dst, uint64_t m, int directed, int device,
+                               grass_csr** out) {
+    if (!out || (m && (!src || !dst)) || m >= (1ull << 34)) return GRASS_EINVAL;
+    *out = nullptr;
+    uint32_t maxid = 0;
+    for (uint64_t i = 0; i < m; ++i) maxid = std::max(maxid, std::max(src[i], dst[i]));
+    if (maxid >= 0x80000000u) return GRASS_EINVAL;
+    const uint64_t rcap = (uint64_t)maxid + 2, ccap = std::max<uint64_t>(directed ? m : 2 * m, 1);
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    if (!grass::enter_device(device)) return GRASS_EDEVICE;
+    int rc = GRASS_OK;
+    grass_csr* csr = new (std::nothrow) grass_csr();
+    if (!csr) rc = GRASS_ENOMEM;
+    {
+        grass::DevMem ds, dd, rp, cl;
+        if (!rc && (ds.alloc(std::max<uint64_t>(m, 1) * 4) != hipSuccess ||
+                    dd.alloc(std::max<uint64_t>(m, 1) * 4) != hipSuccess || rp.alloc(rcap * 8) != hipSuccess ||
+                    cl.alloc(ccap * 4) != hipSuccess))
+            rc = GRASS_ENOMEM;
+        hipError_t e = hipSuccess;
+        if (!rc && m) e = hipMemcpy(ds.p, src, m * 4, hipMemcpyHostToDevice);
+        if (!rc && m && e == hipSuccess) e = hipMemcpy(dd.p, dst, m * 4, hipMemcpyHostToDevice);
+        if (!rc && e != hipSuccess) rc = grass::status_of(e);
+        uint64_t n = 0, mo = 0;
+        if (!rc)
+            rc = grass_graph_sort_device(ds.as<uint32_t>(), dd.as<uint32_t>(), m, directed, device, nullptr,
+                                         rp.as<uint64_t>(), rcap, cl.as<uint32_t>(), ccap, &n, &mo);
+        if (!rc) {
+            try {
+                csr->n = n;
+                csr->m = mo;
+                csr->rowptr.resize(n + 1);
+                csr->col.resize(mo);
+            } catch (const std::bad_alloc&) {
+                rc = GRASS_ENOMEM;
+            }
+        }
+        if (!rc) e = hipMemcpy(csr->rowptr.data(), rp.p, (n + 1) * 8, hipMemcpyDeviceToHost);
+        if (!rc && e == hipSuccess && mo) e = hipMemcpy(csr->col.data(), cl.p, mo * 4, hipMemcpyDeviceToHost);
+        if (!rc && e != hipSuccess) rc = grass::status_of(e);
+    }
+    (void)hipSetDevice(prev);
+    if (rc) {
+        delete csr;
+        return rc;
+    }
+    *out = csr;
+    return GRASS_OK;
 }
 
 }  // extern "C"

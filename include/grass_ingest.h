@@ -73,6 +73,14 @@ int grass_graph_sort_device(const uint32_t *src_dev, const uint32_t *dst_dev, ui
                             uint64_t rowptr_capacity, uint32_t *col_dev, uint64_t col_capacity,
                             uint64_t *n, uint64_t *m_out);
 
+/* The two device calls above for callers that hold their data on the host (the graph_sort and
+ * compressor tools' --device mode): the CSR or the edge list is copied to `device`, processed
+ * there, and the result comes back as the host calls return it -- the same bytes. */
+int grass_compress_on_device(const grass_csr *csr, int format, int device, grass_blob **graph,
+                             grass_blob **offset);
+int grass_graph_sort_on_device(const uint32_t *src, const uint32_t *dst, uint64_t m, int directed,
+                               int device, grass_csr **out);
+
 int grass_blob_data(const grass_blob *blob, const uint8_t **data, uint64_t *size);
 void grass_blob_free(grass_blob *blob);
 

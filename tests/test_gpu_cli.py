@@ -61,3 +61,27 @@ def test_mrw_tool(tmp_path, fmt):
     out = json.loads(_tool("mrw", "-g", prefix, "--format", fmt, "--samples", "300", "--json").stdout)
     # rmat12 has no vertex without edges, so every instance runs all 100 steps
     assert out["sampled_edges"] == 300 * 100
+
+
+@pytest.mark.parametrize("directed", ["u", "d"])
+def test_graph_sort_and_compressor_device_mode(tmp_path, directed):
+    """`--device 0` on the reference-compatible graph_sort and compressor tools writes the same
+    files as their host mode."""
+    from grass_amd import ingest
+    src, dst = ingest.rmat_edges(14, 16, seed=14, threads=4)
+    raw = tmp_path / "raw.txt"
+    with open(raw, "w") as f:
+        f.write("# header line\n")
+        f.write("\n".join(f"{u} {v}" for u, v in zip(src.tolist(), dst.tolist())))
+    outs = {}
+    for mode in ("host", "dev"):
+        extra = ["--device", "0"] if mode == "dev" else []
+        srt = tmp_path / f"sorted_{mode}.txt"
+        _tool("graph_sort", str(raw), str(srt), directed, "1", *extra)
+        outs[mode] = srt.read_bytes()
+        for fmt in ("le", "gt"):
+            _tool("compressor", "--format", fmt, *extra, str(srt), str(tmp_path / f"{fmt}_{mode}"))
+    assert outs["host"] == outs["dev"]
+    for fmt in ("le", "gt"):
+        for ext in (".graph", ".offset"):
+            assert (tmp_path / f"{fmt}_host{ext}").read_bytes() == (tmp_path / f"{fmt}_dev{ext}").read_bytes()
