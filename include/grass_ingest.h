@@ -66,8 +66,9 @@ int grass_rmat_edges_device(uint32_t scale, uint32_t edge_factor, double a, doub
 /* grass_graph_sort (graph_sort.cpp:22-82) on a GPU, over an edge list in device memory: the same
  * CSR, written to rowptr_dev (n + 1 entries) and col_dev (*m_out entries).  *n and *m_out receive
  * the sizes; GRASS_ECAPACITY (sizes set, CSR not written) when rowptr_capacity < n + 1 or
- * col_capacity < *m_out.  n <= max id + 1 and *m_out <= m (directed) or 2m (undirected).  Device
- * working memory is about 16 bytes per directed edge.  Synchronises on `stream`. */
+ * col_capacity < *m_out.  n <= max id + 1 and *m_out <= m (directed) or 2m (undirected).  Ids
+ * must be below 2^31 and m below 2^34 (GRASS_EINVAL otherwise; the host call takes any id below
+ * 2^32 - 1).  Device working memory is about 16 bytes per directed edge.  Synchronises on `stream`. */
 int grass_graph_sort_device(const uint32_t *src_dev, const uint32_t *dst_dev, uint64_t m,
                             int directed, int device, void *stream, uint64_t *rowptr_dev,
                             uint64_t rowptr_capacity, uint32_t *col_dev, uint64_t col_capacity,
@@ -75,7 +76,8 @@ int grass_graph_sort_device(const uint32_t *src_dev, const uint32_t *dst_dev, ui
 
 /* The two device calls above for callers that hold their data on the host (the graph_sort and
  * compressor tools' --device mode): the CSR or the edge list is copied to `device`, processed
- * there, and the result comes back as the host calls return it -- the same bytes. */
+ * there, and the result comes back as the host calls return it -- the same bytes.  The device
+ * calls' limits apply (ids below 2^31, n below 2^31). */
 int grass_compress_on_device(const grass_csr *csr, int format, int device, grass_blob **graph,
                              grass_blob **offset);
 int grass_graph_sort_on_device(const uint32_t *src, const uint32_t *dst, uint64_t m, int directed,
