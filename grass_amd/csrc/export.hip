@@ -72,12 +72,12 @@ __device__ __forceinline__ unsigned long long rel_key(uint32_t r, uint32_t v) {
     return (((unsigned long long)r << 32) | v) + 1;
 }
 
+// One 16-byte store per slot: key 0 (empty), first position ~0.
 __global__ void rel_clear_kernel(RelTab t) {
+    ulonglong2* slot = reinterpret_cast<ulonglong2*>(t.slot);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= t.mask;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        *t.key(i) = 0ull;
-        *t.val(i) = ~0ull;
-    }
+         i += (uint64_t)gridDim.x * blockDim.x)
+        slot[i] = make_ulonglong2(0ull, ~0ull);
 }
 
 // The batch as one position space.
