@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <new>
 
 #include "../../include/grass_export.h"
@@ -45,21 +47,51 @@ __device__ __forceinline__ uint32_t req_of(const uint64_t* __restrict__ Q, uint3
     return lo;
 }
 
-// Open-addressing table of {key, first position} pairs, 16 bytes per slot; key 0 = empty.
-struct RelTab {
+// Open-addressing tables from (request r, vertex v) to the first position of v in r's sequence.
+// Both have the same interface; the kernels below take either.  Linear probing from a
+// multiplicative hash; a slot, once it holds a key, keeps it, and its position only decreases, so
+// every slot ends at the minimum position whatever the order of the inserts.
+
+// 16 bytes per slot: {key, first position}, key = ((r << 32) | v) + 1, key 0 = empty.  Takes any
+// request count, vertex id and position; two atomics per new key (claim, then position).
+struct RelTab16 {
     unsigned long long* slot;  // 2 * (mask + 1) words
     uint64_t mask;
     uint32_t shift;
-    __device__ __forceinline__ unsigned long long* key(uint64_t h) const { return slot + 2 * h; }
-    __device__ __forceinline__ unsigned long long* val(uint64_t h) const { return slot + 2 * h + 1; }
+    __device__ __forceinline__ static unsigned long long key(uint32_t r, uint32_t v) {
+        return (((unsigned long long)r << 32) | v) + 1;
+    }
+    __device__ __forceinline__ unsigned long long* kp(uint64_t h) const { return slot + 2 * h; }
     __device__ __forceinline__ uint64_t home(unsigned long long k) const {
         return ((k * 0x9E3779B97F4A7C15ull) >> shift) & mask;
     }
-    // First position of key k; ~0 when absent.
-    __device__ __forceinline__ uint64_t first(unsigned long long k) const {
+    __device__ __forceinline__ void clear(uint64_t i) const {
+        reinterpret_cast<ulonglong2*>(slot)[i] = make_ulonglong2(0ull, ~0ull);
+    }
+    __device__ __forceinline__ void insert(uint32_t r, uint32_t v, uint64_t p, uint32_t* err) const {
+        const unsigned long long k = key(r, v);
         uint64_t h = home(k);
         for (uint64_t probes = 0; probes <= mask; ++probes) {
-            const ulonglong2 kv = *(const ulonglong2*)key(h);
+            unsigned long long cur = __atomic_load_n(kp(h), __ATOMIC_RELAXED);
+            if (cur == 0) {
+                cur = atomicCAS(kp(h), 0ull, k);
+                if (cur == 0) cur = k;
+            }
+            if (cur == k) {
+                // read first: a repeat after the first occurrence needs no atomic
+                if (__atomic_load_n(kp(h) + 1, __ATOMIC_RELAXED) > p) atomicMin(kp(h) + 1, (unsigned long long)p);
+                return;
+            }
+            h = (h + 1) & mask;
+        }
+        atomicOr(err, 1u);  // cannot happen: slots > positions
+    }
+    // First position of v in request r; ~0 when absent.
+    __device__ __forceinline__ uint64_t first(uint32_t r, uint32_t v) const {
+        const unsigned long long k = key(r, v);
+        uint64_t h = home(k);
+        for (uint64_t probes = 0; probes <= mask; ++probes) {
+            const ulonglong2 kv = *(const ulonglong2*)kp(h);
             if (kv.x == k) return kv.y;
             if (kv.x == 0) break;
             h = (h + 1) & mask;
@@ -68,16 +100,59 @@ struct RelTab {
     }
 };
 
-__device__ __forceinline__ unsigned long long rel_key(uint32_t r, uint32_t v) {
-    return (((unsigned long long)r << 32) | v) + 1;
-}
+// 8 bytes per slot when the request, the vertex and the position fit in one word:
+// word = (((r << vb) | v) << pb) | position, all ones = empty (positions < 2^pb - 1 keep the low
+// bits from being all ones).  A CAS claims an empty slot with the key and the position at once, and
+// atomicMin on a word of the same key lowers only its position bits: one atomic per new key.
+struct RelTab8 {
+    unsigned long long* slot;  // mask + 1 words
+    uint64_t mask;
+    uint32_t shift, vb, pb;
+    __device__ __forceinline__ uint64_t home(unsigned long long k) const {
+        return (((k + 1) * 0x9E3779B97F4A7C15ull) >> shift) & mask;
+    }
+    __device__ __forceinline__ void clear(uint64_t i) const { slot[i] = ~0ull; }
+    __device__ __forceinline__ void insert(uint32_t r, uint32_t v, uint64_t p, uint32_t* err) const {
+        if (vb < 32 && (v >> vb) != 0) {  // not a vertex of the graph
+            atomicOr(err, 1u);
+            return;
+        }
+        const unsigned long long k = ((unsigned long long)r << vb) | v;
+        const unsigned long long w = (k << pb) | p;
+        uint64_t h = home(k);
+        for (uint64_t probes = 0; probes <= mask; ++probes) {
+            unsigned long long cur = __atomic_load_n(slot + h, __ATOMIC_RELAXED);
+            if (cur == ~0ull) {
+                cur = atomicCAS(slot + h, ~0ull, w);
+                if (cur == ~0ull) return;
+            }
+            if ((cur >> pb) == k) {
+                if (cur > w) atomicMin(slot + h, w);
+                return;
+            }
+            h = (h + 1) & mask;
+        }
+        atomicOr(err, 1u);  // cannot happen: slots > positions
+    }
+    __device__ __forceinline__ uint64_t first(uint32_t r, uint32_t v) const {
+        if (vb < 32 && (v >> vb) != 0) return ~0ull;
+        const unsigned long long k = ((unsigned long long)r << vb) | v;
+        uint64_t h = home(k);
+        for (uint64_t probes = 0; probes <= mask; ++probes) {
+            const unsigned long long cur = slot[h];
+            if (cur == ~0ull) break;
+            if ((cur >> pb) == k) return cur & ((1ull << pb) - 1);
+            h = (h + 1) & mask;
+        }
+        return ~0ull;
+    }
+};
 
-// One 16-byte store per slot: key 0 (empty), first position ~0.
-__global__ void rel_clear_kernel(RelTab t) {
-    ulonglong2* slot = reinterpret_cast<ulonglong2*>(t.slot);
+template <class Tab>
+__global__ void rel_clear_kernel(Tab t) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= t.mask;
          i += (uint64_t)gridDim.x * blockDim.x)
-        slot[i] = make_ulonglong2(0ull, ~0ull);
+        t.clear(i);
 }
 
 // The batch as one position space.
@@ -123,42 +198,24 @@ struct LenGen {
     __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return len[i]; }
 };
 
-__global__ __launch_bounds__(kExBlock) void rel_insert_kernel(RelSeq q, uint64_t P, RelTab t, uint32_t* err) {
+template <class Tab>
+__global__ __launch_bounds__(kExBlock) void rel_insert_kernel(RelSeq q, uint64_t P, Tab t, uint32_t* err) {
     for (uint64_t p = (uint64_t)blockIdx.x * kExBlock + threadIdx.x; p < P; p += (uint64_t)gridDim.x * kExBlock) {
         uint32_t r;
         uint64_t e;
         const uint32_t v = q.at(p, r, e);
-        const unsigned long long k = rel_key(r, v);
-        uint64_t h = t.home(k);
-        bool placed = false;
-        for (uint64_t probes = 0; probes <= t.mask; ++probes) {
-            unsigned long long cur = __atomic_load_n(t.key(h), __ATOMIC_RELAXED);
-            if (cur == 0) {
-                cur = atomicCAS(t.key(h), 0ull, k);
-                if (cur == 0) cur = k;
-            }
-            if (cur == k) {
-                placed = true;
-                break;
-            }
-            h = (h + 1) & t.mask;
-        }
-        if (!placed) {
-            atomicOr(err, 1u);  // cannot happen: slots > positions
-        } else if (__atomic_load_n(t.val(h), __ATOMIC_RELAXED) > p) {
-            // read first: a repeat after the first occurrence needs no atomic
-            atomicMin(t.val(h), (unsigned long long)p);
-        }
+        t.insert(r, v, p, err);
     }
 }
 
-__global__ __launch_bounds__(kExBlock) void rel_mark_kernel(RelSeq q, uint64_t P, RelTab t,
+template <class Tab>
+__global__ __launch_bounds__(kExBlock) void rel_mark_kernel(RelSeq q, uint64_t P, Tab t,
                                                             uint32_t* __restrict__ flag) {
     for (uint64_t p = (uint64_t)blockIdx.x * kExBlock + threadIdx.x; p < P; p += (uint64_t)gridDim.x * kExBlock) {
         uint32_t r;
         uint64_t e;
         const uint32_t v = q.at(p, r, e);
-        flag[p] = t.first(rel_key(r, v)) == p;
+        flag[p] = t.first(r, v) == p;
     }
 }
 
@@ -167,7 +224,8 @@ struct FlagGen {
     __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return flag[i]; }
 };
 
-__global__ __launch_bounds__(kExBlock) void rel_emit_kernel(RelSeq q, uint64_t P, RelTab t,
+template <class Tab>
+__global__ __launch_bounds__(kExBlock) void rel_emit_kernel(RelSeq q, uint64_t P, Tab t,
                                                             const uint32_t* __restrict__ flag,
                                                             const uint64_t* __restrict__ F,
                                                             uint32_t* __restrict__ nodes, uint64_t cap,
@@ -180,8 +238,8 @@ __global__ __launch_bounds__(kExBlock) void rel_emit_kernel(RelSeq q, uint64_t P
         if (flag[p] && F[p] < cap) nodes[F[p]] = v;
         if (e == ~0ull) continue;
         const uint64_t base = F[q.Q[r]];
-        const uint64_t fd = flag[p] ? p : t.first(rel_key(r, v));
-        const uint64_t fs = t.first(rel_key(r, q.src[e]));
+        const uint64_t fd = flag[p] ? p : t.first(r, v);
+        const uint64_t fs = t.first(r, q.src[e]);
         if (fd >= P || fs >= P) {  // a source that is neither a seed nor an earlier destination
             atomicOr(err, 1u);
             ldst[e] = lsrc[e] = kNoId;
@@ -195,6 +253,25 @@ __global__ __launch_bounds__(kExBlock) void rel_emit_kernel(RelSeq q, uint64_t P
 __global__ void rel_node_off_kernel(const uint64_t* __restrict__ Q, const uint64_t* __restrict__ F, uint32_t R,
                                     uint64_t* __restrict__ node_off) {
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r <= R; r += gridDim.x * blockDim.x) node_off[r] = F[Q[r]];
+}
+
+// Clears the table, inserts every position, marks the first appearances, numbers them (scan) and
+// writes the sampled vertices and the local ids of every edge.
+template <class Tab>
+hipError_t rel_build(const Tab& t, const RelSeq& q, uint64_t P, uint32_t* d_flag, uint64_t* d_F, uint64_t* d_part,
+                     uint64_t* d_total, uint32_t* d_bad, grass_khop_blocks* out, hipStream_t s) {
+    const unsigned gP = (unsigned)std::min<uint64_t>((P + kExBlock - 1) / kExBlock + 1, kExGrid);
+    hipLaunchKernelGGL(rel_clear_kernel<Tab>, dim3(kExGrid), dim3(256), 0, s, t);
+    if (P) {
+        hipLaunchKernelGGL(rel_insert_kernel<Tab>, dim3(gP), dim3(kExBlock), 0, s, q, P, t, d_bad);
+        hipLaunchKernelGGL(rel_mark_kernel<Tab>, dim3(gP), dim3(kExBlock), 0, s, q, P, t, d_flag);
+    }
+    hipError_t e = exclusive_scan(FlagGen{d_flag}, Count{P, nullptr}, P, d_F, d_part, d_total, s);
+    if (e != hipSuccess || !P) return e;
+    hipLaunchKernelGGL(rel_emit_kernel<Tab>, dim3(gP), dim3(kExBlock), 0, s, q, P, t, (const uint32_t*)d_flag,
+                       (const uint64_t*)d_F, out->nodes_dev, out->node_capacity, out->lsrc_dev, out->ldst_dev,
+                       d_bad);
+    return hipGetLastError();
 }
 
 int khop_blocks(grass_graph* g, const grass_khop_request* reqs, uint32_t R, const grass_khop_output* b,
@@ -222,7 +299,19 @@ int khop_blocks(grass_graph* g, const grass_khop_request* reqs, uint32_t R, cons
     const size_t need = Carve::need(R * sizeof(RelReq)) + Carve::need(R * 8) + Carve::need((R + 1) * 8) +
                         Carve::need(P * 4 + 4) + Carve::need((P + 1) * 8) + Carve::need((nb + 2) * 8) +
                         Carve::need(64);
-    if (ws.scratch.reserve(need) != hipSuccess || ws.ht.reserve(slots * 16) != hipSuccess) return GRASS_ENOMEM;
+    // Packed 8-byte slots when the request, vertex and position bits fit one word; 16-byte slots
+    // otherwise, or when GRASS_EXPORT_SLOT=16 asks for them.
+    auto bits = [](uint64_t x) {
+        uint32_t k = 0;
+        while (k < 64 && (x >> k) != 0) ++k;
+        return k;
+    };
+    const uint32_t rb = bits(R - 1), vb = std::max<uint32_t>(bits(g->n ? g->n - 1 : 0), 1u),
+                   pb = std::max<uint32_t>(bits(P), 1u);  // positions < 2^pb - 1
+    const char* force = getenv("GRASS_EXPORT_SLOT");
+    const bool wide = rb + vb + pb > 64 || (force && strcmp(force, "16") == 0);
+    if (ws.scratch.reserve(need) != hipSuccess || ws.ht.reserve(slots * (wide ? 16 : 8)) != hipSuccess)
+        return GRASS_ENOMEM;
     // this overwrites the k-hop dedup table's memory: the next k-hop batch clears it first
     ws.ht_gen = 0xFFFF;
     Carve cv(ws.scratch.ptr);
@@ -233,12 +322,7 @@ int khop_blocks(grass_graph* g, const grass_khop_request* reqs, uint32_t R, cons
     uint64_t* d_F = cv.take<uint64_t>(P + 1);
     uint64_t* d_part = cv.take<uint64_t>(nb + 2);
     uint64_t* d_sc = cv.take<uint64_t>(8);  // [0] sequence total, [1] node total, [2] error flag
-    RelTab t;
-    t.slot = (unsigned long long*)ws.ht.ptr;
-    t.mask = slots - 1;
-    uint32_t lg = 0;
-    while ((1ull << lg) < slots) ++lg;
-    t.shift = 64 - lg;
+    const uint32_t lg = (uint32_t)__builtin_ctzll(slots);
 
     hipError_t e = hipMemcpyAsync(d_req, hreq, R * sizeof(RelReq), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return status_of(e);
@@ -248,20 +332,16 @@ int khop_blocks(grass_graph* g, const grass_khop_request* reqs, uint32_t R, cons
     e = exclusive_scan(LenGen{d_len}, Count{R, nullptr}, R, d_Q, d_part, d_sc, s);
     if (e != hipSuccess) return status_of(e);
     RelSeq q{d_req, b->offsets_dev, b->src_dev, b->dst_dev, d_Q, R, H};
-    const unsigned gP = (unsigned)std::min<uint64_t>((P + kExBlock - 1) / kExBlock + 1, kExGrid);
     uint32_t* d_bad = (uint32_t*)(d_sc + 2);
     hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(1), dim3(64), 0, s, d_bad, (uint64_t)1, 0u);
-    hipLaunchKernelGGL(rel_clear_kernel, dim3(kExGrid), dim3(256), 0, s, t);
-    if (P) {
-        hipLaunchKernelGGL(rel_insert_kernel, dim3(gP), dim3(kExBlock), 0, s, q, P, t, d_bad);
-        hipLaunchKernelGGL(rel_mark_kernel, dim3(gP), dim3(kExBlock), 0, s, q, P, t, d_flag);
+    if (wide) {
+        RelTab16 t{(unsigned long long*)ws.ht.ptr, slots - 1, 64 - lg};
+        e = rel_build(t, q, P, d_flag, d_F, d_part, d_sc + 1, d_bad, out, s);
+    } else {
+        RelTab8 t{(unsigned long long*)ws.ht.ptr, slots - 1, 64 - lg, vb, pb};
+        e = rel_build(t, q, P, d_flag, d_F, d_part, d_sc + 1, d_bad, out, s);
     }
-    e = exclusive_scan(FlagGen{d_flag}, Count{P, nullptr}, P, d_F, d_part, d_sc + 1, s);
     if (e != hipSuccess) return status_of(e);
-    if (P)
-        hipLaunchKernelGGL(rel_emit_kernel, dim3(gP), dim3(kExBlock), 0, s, q, P, t, (const uint32_t*)d_flag,
-                           (const uint64_t*)d_F, out->nodes_dev, out->node_capacity, out->lsrc_dev, out->ldst_dev,
-                           d_bad);
     hipLaunchKernelGGL(rel_node_off_kernel, dim3(gR), dim3(256), 0, s, (const uint64_t*)d_Q, (const uint64_t*)d_F, R,
                        out->node_off_dev);
     uint64_t* hres = ws.result_slot();

@@ -19,6 +19,17 @@ def ga():
     return grass_amd
 
 
+@pytest.fixture(params=["8", "16"])
+def slot(request, monkeypatch):
+    """The table layout of the k-hop blocks: packed 8-byte slots (chosen whenever the request,
+    vertex and position bits fit one word, as in every test here) or 16-byte slots (forced)."""
+    if request.param == "16":
+        monkeypatch.setenv("GRASS_EXPORT_SLOT", "16")
+    else:
+        monkeypatch.delenv("GRASS_EXPORT_SLOT", raising=False)
+    return request.param
+
+
 def _np(t):
     return t.cpu().numpy()
 
@@ -55,8 +66,8 @@ def _check_blocks(res, seeds_list):
 
 
 @pytest.mark.parametrize("fmt", ["le", "gt"])
-def test_khop_blocks_small(ga, oracle_mod, fmt):
-    print(f"[export khop blocks] rmat12 fmt={fmt}", flush=True)
+def test_khop_blocks_small(ga, oracle_mod, fmt, slot):
+    print(f"[export khop blocks] rmat12 fmt={fmt} slot={slot}", flush=True)
     c = load_case("rmat12")
     gb, ob = c[f"{fmt}_graph"].tobytes(), c[f"{fmt}_offset"].tobytes()
     g = ga.Graph.from_bytes(gb, ob, fmt)
@@ -77,7 +88,7 @@ def test_khop_blocks_small(ga, oracle_mod, fmt):
     g.close()
 
 
-def test_khop_blocks_rmat18(ga):
+def test_khop_blocks_rmat18(ga, slot):
     """A serving-sized batch (64 requests of 200 seeds, [25, 10]) on R-MAT-18, checked against
     the dict numbering and the invariants nodes[local] == global id, one local id per vertex."""
     from grass_amd import ingest
@@ -91,7 +102,8 @@ def test_khop_blocks_rmat18(ga):
     rng = np.random.default_rng(18)
     seeds = [rng.integers(0, g.n, size=200).astype(np.uint32) for _ in range(64)]
     res = g.khop_minibatch(seeds, [[25, 10]] * 64, seed=7)
-    print(f"[export khop blocks] rmat18 edges={res['total']} nodes={res['nodes'].numel()}", flush=True)
+    print(f"[export khop blocks] rmat18 slot={slot} edges={res['total']} nodes={res['nodes'].numel()}",
+          flush=True)
     _check_blocks(res, seeds)
     nodes = _np(res["nodes"]).view(np.uint32)
     n0 = _np(res["node_off"])
@@ -136,7 +148,7 @@ def test_mdrw_dlpack_and_local_ids(ga, fmt):
     g.close()
 
 
-def test_khop_blocks_reject_offsets_of_another_batch(ga):
+def test_khop_blocks_reject_offsets_of_another_batch(ga, slot):
     """Offsets that do not belong to the requests (request 1 claims request 0's hop-0 edges; every
     read stays inside the batch's arrays) give GRASS_EINVAL, and the graph stays usable."""
     import ctypes as C
