@@ -5,7 +5,8 @@
 // 1); with no option at all, or without -g, the usage text is printed and the exit status is 0.
 // Added options: --format le|gt (the reference picks the codec by which binary is built),
 // --seed, --samples (alias --walkers; default: App::numSamples(n)), --batch (samples per library
-// call), --device, --json (one JSON line instead of the timing lines), --quiet.
+// call), --device, --json (one JSON line instead of the timing lines), --quiet, --out FILE (the
+// samples, in FileSink's format).
 #pragma once
 #include <stdio.h>
 #include <stdlib.h>
@@ -27,6 +28,7 @@ inline void print_usage(const char* prog) {
     printf("   --device D          HIP device ordinal (default 0)\n");
     printf("   --json              Print one JSON line\n");
     printf("   --quiet             Omit the reference's timing lines\n");
+    printf("   --out FILE          Write the samples to FILE (binary, one record per sample)\n");
     (void)prog;
 }
 
@@ -34,6 +36,7 @@ template <class App>
 int appMain(int argc, char* argv[]) {
     RunOptions opt;
     const char* graph = nullptr;
+    const char* out_path = nullptr;
     int nruns = 1;
     if (argc < 2) {
         print_usage(argv[0]);
@@ -75,6 +78,8 @@ int appMain(int argc, char* argv[]) {
             opt.json = true;
         } else if (!strcmp(a, "--quiet")) {
             opt.quiet = true;
+        } else if (!strcmp(a, "--out")) {
+            out_path = value(a);
         } else {
             fprintf(stderr, "unknown option %s\n", a);
             print_usage(argv[0]);
@@ -88,6 +93,15 @@ int appMain(int argc, char* argv[]) {
     if (nruns < 1) nruns = 1;
     // like the reference, a graph that cannot be loaded is reported but not an exit failure
     // (LE/src/main.cu:38-40 ignores nextdoor's result); a sampling error is
+    std::unique_ptr<FileSink> sink;
+    if (out_path) {
+        sink.reset(new FileSink(out_path));
+        if (!sink->ok()) {
+            fprintf(stderr, "cannot write %s\n", out_path);
+            return 1;
+        }
+        opt.sink = sink.get();
+    }
     RunReport rep;
     nextdoor<App>(graph, nruns, opt, &rep);
     return (rep.status == GRASS_OK || rep.status == GRASS_EIO) ? 0 : 1;

@@ -34,6 +34,7 @@ constexpr uint32_t kNoId = 0xFFFFFFFFu;  // local id of a vertex that is not in 
 struct RelReq {
     const uint32_t* seeds;
     uint64_t num;
+    uint32_t hops;
 };
 
 // Largest r in [0, R) with Q[r] <= p (Q ascending).
@@ -155,7 +156,9 @@ __global__ void rel_clear_kernel(Tab t) {
         t.clear(i);
 }
 
-// The batch as one position space.
+// The batch as one position space.  Every offset row has been checked (rel_check_kernel) to be
+// ascending and to end at or below the batch total before any pass reads through it, so every
+// edge index e below is < total.
 struct RelSeq {
     const RelReq* req;
     const uint64_t* off;  // batch offsets, hop-major [H][R + 1]
@@ -163,6 +166,7 @@ struct RelSeq {
     const uint32_t* dst;
     const uint64_t* Q;  // R + 1 request starts
     uint32_t R, H;
+    uint32_t n;  // vertices of the graph: an id >= n is not a vertex
     // Vertex at position p < Q[R]; r = its request, e = its batch edge (~0 for a seed).
     __device__ __forceinline__ uint32_t at(uint64_t p, uint32_t& r, uint64_t& e) const {
         r = req_of(Q, R, p);
@@ -180,17 +184,46 @@ struct RelSeq {
             }
             j -= c;
         }
-        return 0;  // past the request's edges: Q and the offsets disagree (reported by the host)
+        return n;  // past the request's edges (the checks reject such offsets first)
     }
 };
 
+// Checks the batch offsets against the requests before any pass reads through them: every row
+// ascending and ending at or below `total`, no edges of request r at hops >= its hop count, and
+// the sequence lengths summing to P.  Any failure sets *invalid, and the passes then touch
+// nothing.
+__global__ void rel_check_kernel(const uint64_t* __restrict__ off, const RelReq* __restrict__ req,
+                                 uint32_t R, uint32_t H, uint64_t total, uint32_t* invalid) {
+    const uint64_t cells = (uint64_t)H * (R + 1);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t h = (uint32_t)(i / (R + 1)), r = (uint32_t)(i % (R + 1));
+        const uint64_t a = off[i];
+        bool ok = a <= total;
+        if (r < R) {
+            const uint64_t b = off[i + 1];
+            ok = ok && a <= b && (h < req[r].hops || a == b);
+        }
+        if (!ok) atomicOr(invalid, 1u);
+    }
+}
+
+// Sequence lengths: seeds plus the request's edges at every hop (only after rel_check_kernel).
 __global__ void rel_len_kernel(const RelReq* __restrict__ req, const uint64_t* __restrict__ off, uint32_t R,
-                               uint32_t H, uint64_t* __restrict__ len) {
+                               uint32_t H, const uint32_t* __restrict__ invalid, uint64_t* __restrict__ len) {
+    const bool bad = *invalid != 0;
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < R; r += gridDim.x * blockDim.x) {
         uint64_t L = req[r].num;
-        for (uint32_t h = 0; h < H; ++h) L += off[(uint64_t)h * (R + 1) + r + 1] - off[(uint64_t)h * (R + 1) + r];
+        if (!bad)
+            for (uint32_t h = 0; h < H; ++h) L += off[(uint64_t)h * (R + 1) + r + 1] - off[(uint64_t)h * (R + 1) + r];
         len[r] = L;
     }
+}
+
+// The sequence total must be the seeds plus the batch's edges: rows of different hops that
+// overlap would make it larger.
+__global__ void rel_total_kernel(const uint64_t* __restrict__ Q, uint32_t R, uint64_t P, uint32_t* invalid) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && Q[R] != P) atomicOr(invalid, 1u);
 }
 
 struct LenGen {
@@ -199,23 +232,35 @@ struct LenGen {
 };
 
 template <class Tab>
-__global__ __launch_bounds__(kExBlock) void rel_insert_kernel(RelSeq q, uint64_t P, Tab t, uint32_t* err) {
+__global__ __launch_bounds__(kExBlock) void rel_insert_kernel(RelSeq q, uint64_t P, Tab t,
+                                                              const uint32_t* __restrict__ invalid, uint32_t* err) {
+    if (*invalid) return;
     for (uint64_t p = (uint64_t)blockIdx.x * kExBlock + threadIdx.x; p < P; p += (uint64_t)gridDim.x * kExBlock) {
         uint32_t r;
         uint64_t e;
         const uint32_t v = q.at(p, r, e);
+        if (v >= q.n) {  // a seed or destination that is not a vertex of the graph
+            atomicOr(err, 1u);
+            continue;
+        }
         t.insert(r, v, p, err);
     }
 }
 
 template <class Tab>
 __global__ __launch_bounds__(kExBlock) void rel_mark_kernel(RelSeq q, uint64_t P, Tab t,
+                                                            const uint32_t* __restrict__ invalid,
                                                             uint32_t* __restrict__ flag) {
+    const bool bad = *invalid != 0;
     for (uint64_t p = (uint64_t)blockIdx.x * kExBlock + threadIdx.x; p < P; p += (uint64_t)gridDim.x * kExBlock) {
         uint32_t r;
         uint64_t e;
+        if (bad) {
+            flag[p] = 0;
+            continue;
+        }
         const uint32_t v = q.at(p, r, e);
-        flag[p] = t.first(r, v) == p;
+        flag[p] = v < q.n && t.first(r, v) == p;
     }
 }
 
@@ -230,7 +275,9 @@ __global__ __launch_bounds__(kExBlock) void rel_emit_kernel(RelSeq q, uint64_t P
                                                             const uint64_t* __restrict__ F,
                                                             uint32_t* __restrict__ nodes, uint64_t cap,
                                                             uint32_t* __restrict__ lsrc,
-                                                            uint32_t* __restrict__ ldst, uint32_t* err) {
+                                                            uint32_t* __restrict__ ldst,
+                                                            const uint32_t* __restrict__ invalid, uint32_t* err) {
+    if (*invalid) return;
     for (uint64_t p = (uint64_t)blockIdx.x * kExBlock + threadIdx.x; p < P; p += (uint64_t)gridDim.x * kExBlock) {
         uint32_t r;
         uint64_t e;
@@ -238,7 +285,7 @@ __global__ __launch_bounds__(kExBlock) void rel_emit_kernel(RelSeq q, uint64_t P
         if (flag[p] && F[p] < cap) nodes[F[p]] = v;
         if (e == ~0ull) continue;
         const uint64_t base = F[q.Q[r]];
-        const uint64_t fd = flag[p] ? p : t.first(r, v);
+        const uint64_t fd = flag[p] ? p : v < q.n ? t.first(r, v) : ~0ull;
         const uint64_t fs = t.first(r, q.src[e]);
         if (fd >= P || fs >= P) {  // a source that is neither a seed nor an earlier destination
             atomicOr(err, 1u);
@@ -259,18 +306,19 @@ __global__ void rel_node_off_kernel(const uint64_t* __restrict__ Q, const uint64
 // writes the sampled vertices and the local ids of every edge.
 template <class Tab>
 hipError_t rel_build(const Tab& t, const RelSeq& q, uint64_t P, uint32_t* d_flag, uint64_t* d_F, uint64_t* d_part,
-                     uint64_t* d_total, uint32_t* d_bad, grass_khop_blocks* out, hipStream_t s) {
+                     uint64_t* d_total, const uint32_t* d_invalid, uint32_t* d_bad, grass_khop_blocks* out,
+                     hipStream_t s) {
     const unsigned gP = (unsigned)std::min<uint64_t>((P + kExBlock - 1) / kExBlock + 1, kExGrid);
     hipLaunchKernelGGL(rel_clear_kernel<Tab>, dim3(kExGrid), dim3(256), 0, s, t);
     if (P) {
-        hipLaunchKernelGGL(rel_insert_kernel<Tab>, dim3(gP), dim3(kExBlock), 0, s, q, P, t, d_bad);
-        hipLaunchKernelGGL(rel_mark_kernel<Tab>, dim3(gP), dim3(kExBlock), 0, s, q, P, t, d_flag);
+        hipLaunchKernelGGL(rel_insert_kernel<Tab>, dim3(gP), dim3(kExBlock), 0, s, q, P, t, d_invalid, d_bad);
+        hipLaunchKernelGGL(rel_mark_kernel<Tab>, dim3(gP), dim3(kExBlock), 0, s, q, P, t, d_invalid, d_flag);
     }
     hipError_t e = exclusive_scan(FlagGen{d_flag}, Count{P, nullptr}, P, d_F, d_part, d_total, s);
     if (e != hipSuccess || !P) return e;
     hipLaunchKernelGGL(rel_emit_kernel<Tab>, dim3(gP), dim3(kExBlock), 0, s, q, P, t, (const uint32_t*)d_flag,
                        (const uint64_t*)d_F, out->nodes_dev, out->node_capacity, out->lsrc_dev, out->ldst_dev,
-                       d_bad);
+                       d_invalid, d_bad);
     return hipGetLastError();
 }
 
@@ -288,7 +336,7 @@ int khop_blocks(grass_graph* g, const grass_khop_request* reqs, uint32_t R, cons
     uint64_t S = 0;
     for (uint32_t r = 0; r < R; ++r) {
         if (reqs[r].hops > H || (reqs[r].num_seeds && !reqs[r].seeds_dev)) return GRASS_EINVAL;
-        hreq[r] = RelReq{reqs[r].seeds_dev, reqs[r].num_seeds};
+        hreq[r] = RelReq{reqs[r].seeds_dev, reqs[r].num_seeds, reqs[r].hops};
         S += reqs[r].num_seeds;
     }
     const uint64_t P = S + b->total;
@@ -321,25 +369,34 @@ int khop_blocks(grass_graph* g, const grass_khop_request* reqs, uint32_t R, cons
     uint32_t* d_flag = cv.take<uint32_t>(P + 1);
     uint64_t* d_F = cv.take<uint64_t>(P + 1);
     uint64_t* d_part = cv.take<uint64_t>(nb + 2);
-    uint64_t* d_sc = cv.take<uint64_t>(8);  // [0] sequence total, [1] node total, [2] error flag
+    uint64_t* d_sc = cv.take<uint64_t>(8);  // [0] sequence total, [1] node total, [2] error flags
     const uint32_t lg = (uint32_t)__builtin_ctzll(slots);
 
     hipError_t e = hipMemcpyAsync(d_req, hreq, R * sizeof(RelReq), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return status_of(e);
+    // the two halves of d_sc[2]: read back together below
+    uint32_t* d_bad = (uint32_t*)(d_sc + 2);  // an id that is not a vertex, or an unknown source
+    uint32_t* d_invalid = d_bad + 1;          // offsets that do not fit the requests
+    hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(1), dim3(64), 0, s, d_bad, (uint64_t)2, 0u);
     const unsigned gR = (unsigned)std::min<uint64_t>((R + 255) / 256 + 1, 1024);
+    if (H) {
+        const uint64_t cells = (uint64_t)H * (R + 1);
+        const unsigned gC = (unsigned)std::min<uint64_t>((cells + 255) / 256, 1024);
+        hipLaunchKernelGGL(rel_check_kernel, dim3(gC), dim3(256), 0, s, (const uint64_t*)b->offsets_dev,
+                           (const RelReq*)d_req, R, H, b->total, d_invalid);
+    }
     hipLaunchKernelGGL(rel_len_kernel, dim3(gR), dim3(256), 0, s, (const RelReq*)d_req,
-                       (const uint64_t*)b->offsets_dev, R, H, d_len);
+                       (const uint64_t*)b->offsets_dev, R, H, (const uint32_t*)d_invalid, d_len);
     e = exclusive_scan(LenGen{d_len}, Count{R, nullptr}, R, d_Q, d_part, d_sc, s);
     if (e != hipSuccess) return status_of(e);
-    RelSeq q{d_req, b->offsets_dev, b->src_dev, b->dst_dev, d_Q, R, H};
-    uint32_t* d_bad = (uint32_t*)(d_sc + 2);
-    hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(1), dim3(64), 0, s, d_bad, (uint64_t)1, 0u);
+    hipLaunchKernelGGL(rel_total_kernel, dim3(1), dim3(64), 0, s, (const uint64_t*)d_Q, R, P, d_invalid);
+    RelSeq q{d_req, b->offsets_dev, b->src_dev, b->dst_dev, d_Q, R, H, (uint32_t)g->n};
     if (wide) {
         RelTab16 t{(unsigned long long*)ws.ht.ptr, slots - 1, 64 - lg};
-        e = rel_build(t, q, P, d_flag, d_F, d_part, d_sc + 1, d_bad, out, s);
+        e = rel_build(t, q, P, d_flag, d_F, d_part, d_sc + 1, d_invalid, d_bad, out, s);
     } else {
         RelTab8 t{(unsigned long long*)ws.ht.ptr, slots - 1, 64 - lg, vb, pb};
-        e = rel_build(t, q, P, d_flag, d_F, d_part, d_sc + 1, d_bad, out, s);
+        e = rel_build(t, q, P, d_flag, d_F, d_part, d_sc + 1, d_invalid, d_bad, out, s);
     }
     if (e != hipSuccess) return status_of(e);
     hipLaunchKernelGGL(rel_node_off_kernel, dim3(gR), dim3(256), 0, s, (const uint64_t*)d_Q, (const uint64_t*)d_F, R,
@@ -348,12 +405,13 @@ int khop_blocks(grass_graph* g, const grass_khop_request* reqs, uint32_t R, cons
     hres[2] = 0;
     e = hipMemcpyAsync(hres, d_Q + R, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(hres + 1, d_F + P, 8, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(hres + 2, d_bad, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(hres + 2, d_bad, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return status_of(e);
-    // the offsets are not the batch of these requests (sizes differ, or an edge's source is not
-    // among the request's earlier vertices)
-    if (hres[0] != P || (uint32_t)hres[2] != 0) return GRASS_EINVAL;
+    // the offsets are not the batch of these requests (rows out of order or past the total, edges
+    // at hops a request does not have, sizes that differ), an id is not a vertex, or an edge's
+    // source is not among the request's earlier vertices
+    if (hres[0] != P || hres[2] != 0) return GRASS_EINVAL;
     out->nodes = hres[1];
     return out->nodes > out->node_capacity ? GRASS_ECAPACITY : GRASS_OK;
 }

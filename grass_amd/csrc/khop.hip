@@ -356,10 +356,10 @@ static unsigned sample_grid() {
 template <int FMT>
 static void launch_sample(grass_graph* g, uint32_t G, hipStream_t s, const uint32_t* fv, const uint32_t* fr,
                           const uint32_t* fd, const uint64_t* ic, const KhReq* reqs, uint32_t h, uint64_t seed, const uint64_t* eoff,
-                          const uint64_t* hb, uint32_t* osrc, uint32_t* odst, uint64_t cap) {
+                          const uint64_t* hb, uint32_t* osrc, uint32_t* odst, uint64_t cap, uint32_t* err) {
 #define GRASS_SAMPLE(GG)                                                                                   \
     hipLaunchKernelGGL((sample_kernel<FMT, GG>), dim3(sample_grid<FMT, GG>()), dim3(kKhBlock), 0, s, g->dev(), \
-                       fv, fr, fd, ic, reqs, h, seed, eoff, hb, osrc, odst, cap, g->d_err)
+                       fv, fr, fd, ic, reqs, h, seed, eoff, hb, osrc, odst, cap, err)
     if (G <= 4) GRASS_SAMPLE(4);
     else if (G <= 6) GRASS_SAMPLE(6);
     else if (G <= 8) GRASS_SAMPLE(8);
@@ -479,15 +479,16 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
     hipLaunchKernelGGL(fill_kernel<uint64_t>, dim3(1), dim3(64), 0, s, d_sc, (uint64_t)64, (uint64_t)0);
 
     const unsigned gridE = 2048;
+    uint32_t* err = err_word(g, ws);
     // frontier 0: seeds, deduplicated
     const uint32_t n32 = (uint32_t)g->n;
     if (S) hipLaunchKernelGGL(seed_gather_kernel, dim3(gridE), dim3(kKhBlock), 0, s, d_req, d_sbeg, R, S, n32,
-                              d_seedv, g->d_err);
+                              d_seedv, err);
     {
         DedupIn in{d_seedv, nullptr, S};
         uint32_t gen = next_gen(ws, ht, s);
         hipLaunchKernelGGL(dedup_insert_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, Count{S, nullptr},
-                           d_sbeg, R, ht, gen, g->d_err);
+                           d_sbeg, R, ht, gen, err);
         hipLaunchKernelGGL(dedup_mark_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, Count{S, nullptr},
                            d_sbeg, R, ht, gen, d_flag);
         e = exclusive_scan(FlagGen{d_flag}, Count{S, nullptr}, Dmax, d_fpos, d_part, d_ftot, s);
@@ -520,16 +521,16 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
                            d_eoff_r, d_hb, h, out->offsets_dev);
         if (g->fmt == kLE)
             launch_sample<kLE>(g, lanes[h], s, d_fv[cur], d_fr[cur], d_fd[cur], d_icount[cur], d_req, h, seed, d_eoff,
-                               d_hb, out->src_dev, out->dst_dev, out->capacity);
+                               d_hb, out->src_dev, out->dst_dev, out->capacity, err);
         else
             launch_sample<kGT>(g, lanes[h], s, d_fv[cur], d_fr[cur], d_fd[cur], d_icount[cur], d_req, h, seed, d_eoff,
-                               d_hb, out->src_dev, out->dst_dev, out->capacity);
+                               d_hb, out->src_dev, out->dst_dev, out->capacity, err);
         if (h + 1 == H) break;
         DedupIn in{out->dst_dev, d_hb + h, out->capacity};
         Count E{0, d_etot};
         uint32_t gen = next_gen(ws, ht, s);
         hipLaunchKernelGGL(dedup_insert_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, E, d_eoff_r, R, ht, gen,
-                           g->d_err);
+                           err);
         hipLaunchKernelGGL(dedup_mark_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, E, d_eoff_r, R, ht, gen,
                            d_flag);
         e = exclusive_scan(FlagGen{d_flag}, E, Dmax, d_fpos, d_part, d_ftot, s);

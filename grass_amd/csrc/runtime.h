@@ -90,6 +90,9 @@ struct Workspace {
     uint64_t ht_slots = 0;
     uint32_t ht_gen = 0xFFFF;
     PinnedBuf host;
+    // Error word of the kernels this workspace's batches launch (first problem met, kErr*);
+    // nullptr: they record into the graph's shared word (the synchronous entry points).
+    uint32_t* err = nullptr;
     uint64_t* result_slot() { return (uint64_t*)host.ptr; }
     void* stage() { return (char*)host.ptr + kResultBytes; }
     hipError_t reserve_stage(size_t bytes) { return host.reserve(kResultBytes + bytes); }
@@ -97,6 +100,8 @@ struct Workspace {
         scratch.release();
         ht.release();
         host.release();
+        if (err) (void)hipFree(err);
+        err = nullptr;
         ht_slots = 0;
         ht_gen = 0xFFFF;
     }
@@ -186,6 +191,14 @@ struct grass_mdrw_result {
 namespace grass {
 
 int status_of(hipError_t e);
+
+// The error word the kernels of a batch on `ws` record into.
+inline uint32_t* err_word(grass_graph* g, const Workspace& ws) { return ws.err ? ws.err : g->d_err; }
+
+// Status of a recorded error kind (kErrNone -> GRASS_OK).
+inline int status_of_err(uint32_t kind) {
+    return kind == kErrNone ? GRASS_OK : kind == kErrBadStart ? GRASS_EINVAL : GRASS_EFORMAT;
+}
 
 // Makes `dev` current for an entry point.  HIP keeps the thread's last failed runtime call as its
 // "last error", and the launch checks here read that state after each launch; a failure the caller

@@ -232,22 +232,22 @@ int random_walk(grass_graph* g, const uint32_t* starts, uint64_t W, uint32_t L, 
 
 template <int FMT, bool COUNT, uint32_t STAGE>
 static void launch_rw_batch_stage(grass_graph* g, hipStream_t s, const RwDesc* d, uint32_t R, uint64_t gran,
-                                  uint32_t claim, unsigned long long* queue, uint64_t seed,
+                                  uint32_t claim, unsigned long long* queue, uint64_t seed, uint32_t* err,
                                   unsigned long long* bytes) {
     static unsigned grid = 0;
     if (!grid) grid = persistent_grid((const void*)rw_batch_kernel<FMT, COUNT, STAGE>, kRwBlock);
     hipLaunchKernelGGL((rw_batch_kernel<FMT, COUNT, STAGE>), dim3(grid), dim3(kRwBlock), 0, s, g->dev(), d, R,
-                       gran, claim, queue, seed, g->d_err, bytes);
+                       gran, claim, queue, seed, err, bytes);
 }
 
 template <int FMT, bool COUNT>
 static void launch_rw_batch(grass_graph* g, hipStream_t s, const RwDesc* d, uint32_t R, uint64_t gran,
-                            unsigned long long* queue, uint64_t seed, unsigned long long* bytes) {
+                            unsigned long long* queue, uint64_t seed, uint32_t* err, unsigned long long* bytes) {
     const uint32_t stage = g->tune.rw_stage.load(std::memory_order_relaxed);
     const uint32_t claim = g->tune.rw_claim.load(std::memory_order_relaxed);
-    if (stage <= 8) launch_rw_batch_stage<FMT, COUNT, 8>(g, s, d, R, gran, claim, queue, seed, bytes);
-    else if (stage <= 16) launch_rw_batch_stage<FMT, COUNT, 16>(g, s, d, R, gran, claim, queue, seed, bytes);
-    else launch_rw_batch_stage<FMT, COUNT, 32>(g, s, d, R, gran, claim, queue, seed, bytes);
+    if (stage <= 8) launch_rw_batch_stage<FMT, COUNT, 8>(g, s, d, R, gran, claim, queue, seed, err, bytes);
+    else if (stage <= 16) launch_rw_batch_stage<FMT, COUNT, 16>(g, s, d, R, gran, claim, queue, seed, err, bytes);
+    else launch_rw_batch_stage<FMT, COUNT, 32>(g, s, d, R, gran, claim, queue, seed, err, bytes);
 }
 
 int rw_batch_enqueue(grass_graph* g, const grass_rw_request* reqs, uint32_t count, uint64_t seed,
@@ -286,12 +286,13 @@ int rw_batch_enqueue(grass_graph* g, const grass_rw_request* reqs, uint32_t coun
     e = hipMemcpyAsync(d_desc, desc, R * sizeof(RwDesc), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return status_of(e);
     hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, s, d_queue, (uint64_t)1, 0ull);
+    uint32_t* err = err_word(g, ws);
     if (g->fmt == kLE) {
-        if (bytes_dev) launch_rw_batch<kLE, true>(g, s, d_desc, R, gran, d_queue, seed, bytes_dev);
-        else launch_rw_batch<kLE, false>(g, s, d_desc, R, gran, d_queue, seed, nullptr);
+        if (bytes_dev) launch_rw_batch<kLE, true>(g, s, d_desc, R, gran, d_queue, seed, err, bytes_dev);
+        else launch_rw_batch<kLE, false>(g, s, d_desc, R, gran, d_queue, seed, err, nullptr);
     } else {
-        if (bytes_dev) launch_rw_batch<kGT, true>(g, s, d_desc, R, gran, d_queue, seed, bytes_dev);
-        else launch_rw_batch<kGT, false>(g, s, d_desc, R, gran, d_queue, seed, nullptr);
+        if (bytes_dev) launch_rw_batch<kGT, true>(g, s, d_desc, R, gran, d_queue, seed, err, bytes_dev);
+        else launch_rw_batch<kGT, false>(g, s, d_desc, R, gran, d_queue, seed, err, nullptr);
     }
     return status_of(hipGetLastError());
 }

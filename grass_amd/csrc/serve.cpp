@@ -28,6 +28,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "devutil.h"
 #include "runtime.h"
 
 namespace grass {
@@ -65,9 +66,11 @@ class Arena {
         }
         return false;
     }
-    // Whether regions of the given sizes could all be allocated now.
+    // Whether regions of the given sizes could all be allocated now.  The trial allocations are
+    // undone and leave the peak as it was.
     bool fits(std::initializer_list<size_t> sizes) {
         std::vector<std::pair<size_t, size_t>> got;
+        const size_t peak0 = peak;
         bool ok = true;
         for (size_t b : sizes) {
             if (!b) continue;
@@ -79,6 +82,7 @@ class Arena {
             got.emplace_back(off, b);
         }
         for (auto& p : got) release(p.first, p.second);
+        peak = peak0;
         return ok;
     }
     void release(size_t off, size_t bytes) {
@@ -140,6 +144,7 @@ struct Slot {
     Workspace ws_rw, ws_kh;
     PinnedBuf stage;  // the group's inputs on their way to HBM
     PinnedBuf offs;   // k-hop offsets read back
+    PinnedBuf errs;   // the group's error words read back: [0] walks, [1] k-hop
     bool busy = false;
     bool rw_finished = false;  // the group's walks have been completed
     std::vector<Req*> reqs;
@@ -239,11 +244,15 @@ struct grass_session {
     void finish_kind(Slot& s, int kind) {
         const uint64_t* off = (const uint64_t*)s.offs.ptr;
         const uint32_t K = (uint32_t)s.kreqs.size();
+        // what the kind's kernels recorded in its own error word (a decoded id or a start that is
+        // not a vertex) fails the kind's requests of this group, and no other call's
+        int status = s.launch_status;
+        if (status == GRASS_OK) status = status_of_err(((const uint32_t*)s.errs.ptr)[kind == GRASS_REQ_RW ? 0 : 1]);
         for (Req*& slot_req : s.reqs) {
             Req* r = slot_req;
             if (!r || r->q.kind != kind) continue;
             slot_req = nullptr;  // finish() may delete a cancelled request
-            if (kind == GRASS_REQ_KHOP && s.launch_status == GRASS_OK) {
+            if (kind == GRASS_REQ_KHOP && status == GRASS_OK) {
                 r->res.hops = r->q.hops;
                 r->edges = 0;
                 for (uint32_t h = 0; h < r->q.hops && h < s.H; ++h) {
@@ -257,7 +266,7 @@ struct grass_session {
             } else if (kind == GRASS_REQ_RW) {
                 r->edges = r->q.walkers * r->q.length;
             }
-            finish(r, s.launch_status);
+            finish(r, status);
         }
     }
 
@@ -386,6 +395,8 @@ struct grass_session {
     int enqueue(Slot& s, uint64_t kcap) {
         const uint64_t seed = cfg.seed;
         uint32_t* in_dev = (uint32_t*)arena.at(s.in_off);
+        uint32_t* errs = (uint32_t*)s.errs.ptr;
+        errs[0] = errs[1] = kErrNone;
         // 1. every input of the group in one copy (on the k-hop stream; the walk stream waits)
         if (s.in_bytes) {
             if (s.stage.reserve(s.in_bytes) != hipSuccess) return GRASS_ENOMEM;
@@ -418,9 +429,13 @@ struct grass_session {
                 memcpy(k.fanout, r->q.fanout, sizeof(k.fanout));
             }
         }
-        // 2. walks on the walk stream
+        // 2. walks on the walk stream, recording into the slot's own error word
         if (!rw.empty()) {
+            hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(1), dim3(64), 0, s.s_rw, s.ws_rw.err, (uint64_t)1, 0u);
             int rc = rw_batch_enqueue(g, rw.data(), (uint32_t)rw.size(), seed, s.s_rw, s.ws_rw, nullptr);
+            if (rc == GRASS_OK &&
+                hipMemcpyAsync(errs, s.ws_rw.err, 4, hipMemcpyDeviceToHost, s.s_rw) != hipSuccess)
+                rc = GRASS_EDEVICE;
             if (hipEventRecord(s.ev_rw, s.s_rw) != hipSuccess && rc == GRASS_OK) rc = GRASS_EDEVICE;
             s.rw_pending = true;
             if (rc) return rc;
@@ -436,11 +451,14 @@ struct grass_session {
             o.dst_dev = (uint32_t*)base + kcap;
             o.capacity = kcap;
             o.offsets_dev = (uint64_t*)(base + kcap * 8);
+            hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(1), dim3(64), 0, s.s_kh, s.ws_kh.err, (uint64_t)1, 0u);
             int rc = khop_enqueue(g, kh.data(), K, seed, &o, s.s_kh, s.ws_kh);
             const size_t ob = (size_t)std::max<uint32_t>(s.H, 1) * (K + 1) * 8;
             if (rc == GRASS_OK && s.offs.reserve(ob) != hipSuccess) rc = GRASS_ENOMEM;
             if (rc == GRASS_OK &&
                 hipMemcpyAsync(s.offs.ptr, o.offsets_dev, ob, hipMemcpyDeviceToHost, s.s_kh) != hipSuccess)
+                rc = GRASS_EDEVICE;
+            if (rc == GRASS_OK && hipMemcpyAsync(errs + 1, s.ws_kh.err, 4, hipMemcpyDeviceToHost, s.s_kh) != hipSuccess)
                 rc = GRASS_EDEVICE;
             if (hipEventRecord(s.ev_kh, s.s_kh) != hipSuccess && rc == GRASS_OK) rc = GRASS_EDEVICE;
             s.kh_pending = true;
@@ -566,6 +584,12 @@ int grass_session_create(grass_graph* g, const grass_session_config* cfg, grass_
             hipEventCreateWithFlags(&sl.in_ready, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&sl.ev_rw, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&sl.ev_kh, hipEventDisableTiming) != hipSuccess)
+            rc = GRASS_EDEVICE;
+        // each slot's kernels record errors in words of their own, read back with the group
+        else if (hipMalloc((void**)&sl.ws_rw.err, 64) != hipSuccess ||
+                 hipMalloc((void**)&sl.ws_kh.err, 64) != hipSuccess || sl.errs.reserve(64) != hipSuccess)
+            rc = GRASS_ENOMEM;
+        else if (hipMemset(sl.ws_rw.err, 0, 64) != hipSuccess || hipMemset(sl.ws_kh.err, 0, 64) != hipSuccess)
             rc = GRASS_EDEVICE;
     }
     (void)hipSetDevice(prev);
@@ -732,6 +756,7 @@ int grass_session_destroy(grass_session* s) {
         sl.ws_kh.release();
         sl.stage.release();
         sl.offs.release();
+        sl.errs.release();
         for (hipEvent_t e : {sl.in_ready, sl.ev_rw, sl.ev_kh})
             if (e) (void)hipEventDestroy(e);
         if (sl.s_rw) (void)hipStreamDestroy(sl.s_rw);

@@ -185,6 +185,66 @@ def test_khop_blocks_reject_offsets_of_another_batch(ga, slot):
     g.close()
 
 
+def test_khop_blocks_reject_malformed_offsets_and_ids(ga, slot):
+    """Offsets checked before any pass reads through them: a max_hops larger than the batch wrote
+    (rows of garbage past the total), a descending row, a row that ends past the total, and a seed
+    that is not a vertex (the seed buffer rewritten after the batch) all give GRASS_EINVAL, under
+    both table layouts, and leave the graph usable."""
+    import ctypes as C
+
+    import torch
+    print(f"[export khop blocks] malformed offsets slot={slot}", flush=True)
+    c = load_case("rmat12")
+    g = ga.Graph.from_bytes(c["gt_graph"].tobytes(), c["gt_offset"].tobytes(), "gt")
+    from conftest import csr_of
+    rowptr, _ = csr_of(c)
+    busy = np.flatnonzero(np.diff(rowptr.astype(np.int64)) >= 2).astype(np.uint32)
+    seeds = [busy[:3], busy[-2:]]
+    arr, keep = g.khop_request_array(seeds, [[5, 5], [5, 5]])
+    cap = g.khop_bound(arr, 2)
+    src = torch.empty(cap, dtype=torch.int32, device="cuda")
+    dst = torch.empty(cap, dtype=torch.int32, device="cuda")
+    H4 = 4
+    off = torch.zeros(H4 * 3, dtype=torch.int64, device="cuda")
+    out = ga.KhopOutput(src.data_ptr(), dst.data_ptr(), cap, off.data_ptr(), 0, 0)
+    L = ga.lib()
+    assert L.grass_khop_batch(g.handle, arr, 2, 3, C.byref(out), None) == ga.OK
+    g.sync()
+    good = off.clone()
+    total = out.total
+    nodes = torch.empty(cap + 5, dtype=torch.int32, device="cuda")
+    noff = torch.empty(3, dtype=torch.int64, device="cuda")
+    ls = torch.empty(cap, dtype=torch.int32, device="cuda")
+    ld = torch.empty(cap, dtype=torch.int32, device="cuda")
+
+    def build(max_hops):
+        o = ga.KhopOutput(src.data_ptr(), dst.data_ptr(), cap, off.data_ptr(), total, max_hops)
+        blk = ga.KhopBlocks(nodes.data_ptr(), cap + 5, noff.data_ptr(), ls.data_ptr(), ld.data_ptr(), 0)
+        rc = L.grass_khop_blocks_build(g.handle, arr, 2, C.byref(o), C.byref(blk), None)
+        g.sync()  # no error left behind on the graph
+        return rc
+
+    assert build(2) == ga.OK
+    off[6:] = 1 << 40  # rows 2 and 3 were never written by a 2-hop batch
+    assert build(H4) == ga.EINVAL
+    off.copy_(good)
+    off[0], off[1] = 5, 2  # descending row
+    assert build(2) == ga.EINVAL
+    off.copy_(good)
+    off[5] = total + 100  # hop 1 ends past the batch total
+    assert build(2) == ga.EINVAL
+    off.copy_(good)
+    seed_dev = keep[0]
+    saved = seed_dev.clone()
+    seed_dev[0] = g.n  # not a vertex of the graph
+    assert build(2) == ga.EINVAL
+    seed_dev.copy_(saved)
+    assert build(2) == ga.OK
+    res = g.khop_minibatch(seeds, [[5, 5], [5, 5]], seed=3)
+    _check_blocks(res, seeds)
+    g.close()
+
+
 def test_mdrw_dlpack_memory_goes_with_the_last_tensor(ga):
     """The result's device memory stays while any exported tensor lives (after grass_mdrw_free)
     and is returned when the last one goes."""
