@@ -89,17 +89,67 @@ def serving_bench(g, seed, rng, seconds=3.0, burst=6000):
     return out
 
 
+def plain_four_queues(g, reqs, kseeds, kfans, seed, reps=3):
+    """The step's requests one at a time (a launch group of one request each: one walk launch, or
+    one k-hop pipeline), four groups in flight on their own streams and no host synchronisation
+    per request: a serving session with max_batch 1 and max_inflight 4 given every request at
+    once.  Isolates what batching gains over per-request launches that already overlap."""
+    from grass_amd import serve
+    reqs_s = [serve.rw_request(r["walkers"], r["length"], r["instance"]) for r in reqs]
+    reqs_s += [serve.khop_request(k, f, 10000 + i) for i, (k, f) in enumerate(zip(kseeds, kfans))]
+    with serve.Session(g, memory_budget=24 << 30, max_batch=1, max_inflight=4, seed=seed) as s:
+        s.replay(reqs_s, np.zeros(len(reqs_s)))  # warm-up: sizes every slot's workspaces
+        walls, edges = [], []
+        for _ in range(reps):
+            e0 = s.stats()["sampled_edges"]
+            lat, wall = s.replay(reqs_s, np.zeros(len(reqs_s)))
+            assert (lat >= 0).all()
+            walls.append(wall)
+            edges.append(s.stats()["sampled_edges"] - e0)
+    wall = float(np.median(walls))
+    return {"value": edges[0] / (wall * 1e-6), "ms_per_step": round(wall / 1e3, 3), "queues": 4,
+            "edges": int(edges[0])}
+
+
+def serving_rmat26(seed, rng):
+    """The serving workload's own graph: R-MAT scale 26 (edge factor 16, seed 42) built, encoded
+    (GT) and loaded on the GPU, then the same saturation + 70 % Poisson measurement."""
+    import torch
+    import grass_amd
+    from grass_amd import ingest
+    torch.cuda.synchronize()
+    t = time.time()
+    src, dst = ingest.rmat_edges_device(26, 16, seed=seed)
+    rp, col = ingest.graph_sort_device(src, dst)
+    del src, dst
+    g = grass_amd.Graph.from_csr_device(rp, col, "gt")
+    torch.cuda.synchronize()
+    build_s = time.time() - t
+    del rp, col
+    torch.cuda.empty_cache()
+    try:
+        out = serving_bench(g, seed, rng)
+        out.update({"scale": 26, "n": g.n, "m": g.m, "device_bytes": g.device_bytes,
+                    "build_s": round(build_s, 2)})
+    finally:
+        g.close()
+    return out
+
+
 def profile_figures(scale, fmt):
-    """(HBM bytes per rw_batch_kernel launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, TCC hit rate)
-    from the committed summary of this exact workload; (None, None) when there is none."""
+    """(HBM bytes per rw_batch_kernel launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, TCC hit rate,
+    the profiled average launch in ms) from the committed summary of this exact workload;
+    (None, None, None) when there is none."""
     import re
     path = os.path.join(REPO, "profiles", f"rw_batch_kernel_rmat{scale}_{fmt}.md")
     if not os.path.exists(path):
-        return None, None
+        return None, None, None
     text = open(path).read()
     t = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", text, re.M)
     h = re.search(r"^tcc_hit_rate:\s*([0-9.]+)", text, re.M)
-    return (int(t.group(1)) if t else None), (float(h.group(1)) if h else None)
+    d = re.search(r"^profiled_launch_us:\s*([0-9.]+)", text, re.M)
+    return ((int(t.group(1)) if t else None), (float(h.group(1)) if h else None),
+            (float(d.group(1)) / 1e3 if d else None))
 
 
 def mdrw_bench(g, seed, instances, sptr, target=16384):
@@ -149,7 +199,9 @@ def main():
     ap.add_argument("--no-mdrw", action="store_true", help="skip the multi-dimensional random walks")
     ap.add_argument("--mdrw-instances", type=int, default=512)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
-    ap.add_argument("--no-serve", action="store_true", help="skip the serving (p50/p99) run")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (p50/p99) runs")
+    ap.add_argument("--no-s26", action="store_true",
+                    help="skip the serving run on the R-MAT scale-26 graph (built on the GPU)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 256 MiB device copy (a known-size kernel that scales rocprof "
                          "FETCH_SIZE / WRITE_SIZE readings)")
@@ -350,6 +402,9 @@ def main():
         torch.cuda.synchronize()
         assert torch.equal(ref0, outs[0]), "batched and plain walk outputs differ"
     g.sync()
+    # sampled walk edges: entries that are not the padding a walk gets after a vertex without
+    # neighbours (none on an undirected graph), minus the start column
+    rw_edges = int(sum(int((o != g.n).sum()) - o.shape[0] for o in outs))
     log(f"2-hop batch: {R2} requests, {sum(len(s) for s in kseeds)} seeds, {khop_edges} edges")
     export = None
     if R2:
@@ -385,10 +440,13 @@ def main():
     log(f"batched: {ms_step:.3f} ms/step, {value/1e9:.3f} G edges/s "
         f"(rw {np.mean(ev_rw):.3f} ms for {rw_edges} edges, 2-hop {np.mean(ev_kh):.3f} ms for {khop_edges})")
     plain = None
+    plain4 = None
     if not args.no_plain:
         wall_p, _ = timed(plain_step, max(1, args.steps // 2), 1)
         plain = edges_step * max(1, args.steps // 2) / wall_p
         log(f"plain: {wall_p/max(1, args.steps // 2)*1e3:.3f} ms/step, {plain/1e9:.3f} G edges/s")
+        plain4 = plain_four_queues(g, reqs, kseeds, kfans, seed)
+        log(f"plain on 4 queues: {plain4['value']/1e9:.3f} G edges/s ({plain4['ms_per_step']:.3f} ms/step)")
 
     # roofline of the dominant kernel (the persistent random-walk kernel): its launch timed with
     # HIP events on the launching stream, replayed from a captured graph so that no host
@@ -408,7 +466,16 @@ def main():
 
     # the committed summary is of the default request mix only
     default_mix = args.rw_requests == 256 and args.khop_requests == 256
-    traffic, tcc_hit = profile_figures(args.scale, args.fmt) if default_mix else (None, None)
+    traffic, tcc_hit, prof_ms = profile_figures(args.scale, args.fmt) if default_mix else (None, None, None)
+    # the counters describe the kernel that was profiled: a launch time more than 5 % away from the
+    # profiled one means the kernel changed since, and the figures are left out
+    if prof_ms is None or abs(kern_ms - prof_ms) > 0.05 * prof_ms:
+        if traffic is not None:
+            roof["traffic_note"] = (f"profiled launch {prof_ms} ms differs from this run's by more than 5 %: "
+                                    "counters not reused")
+        traffic, tcc_hit = None, None
+    else:
+        roof["profiled_launch_ms"] = prof_ms
     if tcc_hit is not None:
         roof["tcc_hit_rate"] = tcc_hit
     if traffic is not None:
@@ -485,11 +552,27 @@ def main():
         del rowptr, col
         torch.cuda.empty_cache()
 
+    if not args.no_serve and not args.no_s26:
+        extra["serving_s26"] = serving_rmat26(seed, np.random.default_rng([seed, 26]))
+        log(f"serving s26: {extra['serving_s26']}")
+    csr_bytes = 8 * (g.n + 1) + 4 * g.m
+    extra["resident"] = {"device_bytes": g.device_bytes, "bytes_per_edge": round(g.device_bytes / g.m, 3),
+                         "csr_u64_u32_bytes": csr_bytes, "over_csr": round(g.device_bytes / csr_bytes, 4),
+                         "file_bytes": len(gb) + len(ob)}
+
     cpu = None
     if not args.no_cpu:
         import oracle
         og = oracle.Graph(gb, ob, args.fmt)
-        cores = min(16, os.cpu_count() or 1)
+        # the host's cores are shared between its GPUs: one GPU's share is OMP_NUM_THREADS (16 on
+        # these boxes), whatever nproc counts for the whole host
+        nproc = os.cpu_count() or 1
+        try:
+            affinity = len(os.sched_getaffinity(0))
+        except (AttributeError, OSError):
+            affinity = nproc
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        cores = max(1, min(affinity, omp or 16))
         # bounded sample of the same step: a prefix of the walk requests and a prefix of the 2-hop
         # requests, each sized to half of ~cpu-seconds of oracle work
         t = time.perf_counter()
@@ -534,7 +617,7 @@ def main():
                          f"first {nk} of the {R2} 2-hop requests ({ke} edges), the oracle's C "
                          f"restatement with OpenMP over walkers / requests; median of 5 runs after "
                          f"a warm-up, {dt:.2f} s each",
-               "cpu_model": model}
+               "cpu_model": model, "nproc": nproc, "affinity_cpus": affinity, "omp_num_threads": omp or None}
         # one core: the first walk requests that take about a second, median of 5 runs
         t = time.perf_counter()
         r1, e1 = [], 0
@@ -571,6 +654,7 @@ def main():
                    "khop_edges_per_step": khop_edges},
         "rw_ms": round(float(np.mean(ev_rw)), 4), "khop_ms": round(float(np.mean(ev_kh)), 4),
         "plain_value": plain, "batched_over_plain": (value / plain) if plain else None,
+        "plain_4q": plain4, "batched_over_plain_4q": (value / plain4["value"]) if plain4 else None,
         "p99_ms": p99,
         "roofline": roof, "cpu_baseline": cpu,
         "ingest_s": round(t_ingest, 2), "load_s": round(t_load, 3), "encode": encode, "export": export,

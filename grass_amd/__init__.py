@@ -164,6 +164,10 @@ def lib():
             "grass_plan_launch": (I32, [P, P]),
             "grass_plan_khop_total": (I32, [P, P]),
             "grass_plan_destroy": (I32, [P]),
+            "grass_plan_create_mdrw": (I32, [P, P, U32, U32, U64, P]),
+            "grass_plan_mdrw_output": (I32, [P, P]),
+            "grass_copy_to_host": (I32, [P, P, P, U64]),
+            "grass_copy_to_device": (I32, [P, P, P, U64]),
             "grass_get_tuning": (I32, [P, P]),
             "grass_set_tuning": (I32, [P, P]),
             "grass_autotune": (I32, [P, U32, P, P]),
@@ -521,6 +525,10 @@ class Graph:
         """Captures a fixed batch (see Plan); the buffers live as long as the plan."""
         return Plan(self, rw_requests, khop_seeds, khop_fanouts, seed, khop_instances)
 
+    def mdrw_plan(self, frontier_sizes, target, seed, instances=None):
+        """Captures one MDRW batch (see MdrwPlan)."""
+        return MdrwPlan(self, frontier_sizes, target, seed, instances)
+
     @staticmethod
     def khop_request_edges(res, r):
         """Request r's edges, hop by hop concatenated (request-major view of a hop-major batch)."""
@@ -612,6 +620,59 @@ class Plan:
         off = self.khop_offsets.cpu().numpy().view(np.uint64)[: self.H * (self.R + 1)]
         return {"offsets": off.reshape(self.H, self.R + 1), "src": to_numpy_u32(self.khop_src)[:n],
                 "dst": to_numpy_u32(self.khop_dst)[:n], "total": n}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().grass_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MdrwPlan:
+    """One MDRW batch captured into a HIP graph (grass_plan_create_mdrw): every launch replays it
+    into the same device outputs."""
+
+    def __init__(self, graph, frontier_sizes, target, seed, instances=None):
+        self.graph = graph
+        I = len(frontier_sizes)
+        self.I = I
+        arr = (MdrwRequest * max(I, 1))()
+        for i, m in enumerate(frontier_sizes):
+            arr[i] = MdrwRequest(int(instances[i]) if instances is not None else i, int(m))
+        h = C.c_void_p()
+        _check(lib().grass_plan_create_mdrw(graph.handle, arr, I, target, seed, C.byref(h)),
+               "grass_plan_create_mdrw")
+        self._h = h
+
+    def launch(self, stream=None):
+        _check(lib().grass_plan_launch(self._h, C.c_void_p(_stream_ptr(stream))), "grass_plan_launch")
+
+    def result(self):
+        """Host copies of the outputs in Graph.mdrw's layout (synchronous copies: call them after
+        the launch stream has passed the replay)."""
+        o = MdrwOutput()
+        _check(lib().grass_plan_mdrw_output(self._h, C.byref(o)), "grass_plan_mdrw_output")
+        I = self.I
+
+        def get(ptr, count, dtype):
+            a = np.zeros(max(count, 1), dtype=dtype)
+            if count:
+                _check(lib().grass_copy_to_host(self.graph.handle, a.ctypes.data, ptr,
+                                                count * a.itemsize), "grass_copy_to_host")
+            return a[:count]
+        return {"step_off": get(o.step_off_dev, I + 1, np.uint64),
+                "vert_off": get(o.vert_off_dev, I + 1, np.uint64),
+                "ind_off": get(o.ind_off_dev, I + 1, np.uint64),
+                "step_src": get(o.step_src_dev, o.steps, np.uint32),
+                "step_dst": get(o.step_dst_dev, o.steps, np.uint32),
+                "verts": get(o.verts_dev, o.verts, np.uint32),
+                "ind_src": get(o.ind_src_dev, o.induced, np.uint32),
+                "ind_dst": get(o.ind_dst_dev, o.induced, np.uint32)}
 
     def close(self):
         if getattr(self, "_h", None):

@@ -219,7 +219,7 @@ int grass_info(const grass_graph* g, grass_graph_info* info) {
 int grass_device_arrays(const grass_graph* g, const uint32_t** words, const uint64_t** starts,
                         const uint32_t** degree) {
     if (!g) return GRASS_EINVAL;
-    if (words) *words = g->d_words;
+    if (words) *words = g->dev().w;
     if (starts) *starts = g->d_start;
     if (degree) *degree = g->d_deg;
     return GRASS_OK;

@@ -29,7 +29,6 @@ struct DevGraph {
     // Payload words the samplers decode (big-endian word view, padded): the .graph image itself
     // for LE; for GT the checkpointed payload stream built at load (see gtx_block_bit).
     const uint32_t* __restrict__ w;
-    const uint64_t* __restrict__ start;  // bit start of record v in .graph, n + 1 entries
     const uint32_t* __restrict__ deg;    // degree of v, decoded and validated at load
     const uint4* __restrict__ hdr;       // decoded header of v (see pack_hdr), built at load
     uint32_t n;
@@ -43,7 +42,6 @@ struct Hdr {
     uint32_t d;
     uint32_t f1, f2, fp;
     uint32_t C, q, R;  // LE: chunks, q = floor((d-1)/C); GT: blocks, q = floor(d/C); R remainder
-    uint32_t hbits;    // header length in bits (of the .graph record)
 };
 
 // GT checkpoints.  The GT payload stream on the device keeps the .graph blocks and their order
@@ -56,23 +54,26 @@ struct Hdr {
 constexpr uint32_t kGtCheckShiftDefault = 2;
 
 // Decoded-header cache: one 16-byte entry per vertex, so that a sampling step reads one aligned
-// dwordx4 instead of walking four dependent gamma codes through the record.
+// dwordx4 instead of walking four dependent gamma codes through the record, and needs no
+// division to recover the chunk geometry.
 //   x = payload bit [31:0]          (LE d == 1: the neighbour itself)
-//   y = payload bit [39:32] | f1 << 8 | f2 << 14 | header bits << 20
+//   y = payload bit [37:32] | f1 << 6 | f2 << 11 | q << 16
 //   z = d
-//   w = C | first_pos << 16          (C, first_pos <= sqrt(2^31) + 2 < 2^16)
-// Payload bits must be < 2^40 (a .graph file below 128 GiB); f1, f2 <= 31; header <= 252 bits.
+//   w = C | first_pos << 16          (C, q, first_pos <= sqrt(2^31) + 2 < 2^16)
+// Payload bits must be < 2^38 (a payload below 32 GiB); f1, f2 <= 31.
+constexpr uint64_t kMaxPayloadBits = 1ull << 38;
+
 __host__ __device__ __forceinline__ uint4 pack_hdr(const Hdr& h, int fmt) {
     uint4 e;
     e.z = h.d;
     if (fmt == kLE && h.d == 1) {
         e.x = h.f1;
-        e.y = h.hbits << 20;
+        e.y = 0;
         e.w = 0;
         return e;
     }
     e.x = (uint32_t)h.pay;
-    e.y = (uint32_t)(h.pay >> 32) | (h.f1 << 8) | (h.f2 << 14) | (h.hbits << 20);
+    e.y = ((uint32_t)(h.pay >> 32) & 63u) | (h.f1 << 6) | (h.f2 << 11) | (h.q << 16);
     e.w = h.C | (h.fp << 16);
     return e;
 }
@@ -80,21 +81,32 @@ __host__ __device__ __forceinline__ uint4 pack_hdr(const Hdr& h, int fmt) {
 template <int FMT>
 __device__ __forceinline__ void unpack_hdr(uint4 e, Hdr& h) {
     h.d = e.z;
-    h.hbits = e.y >> 20;
     if (FMT == kLE && h.d == 1) {
         h.f1 = e.x;
         h.f2 = h.fp = h.C = h.q = h.R = 0;
         h.pay = 0;
         return;
     }
-    h.pay = ((uint64_t)(e.y & 0xFFu) << 32) | e.x;
-    h.f1 = (e.y >> 8) & 63u;
-    h.f2 = (e.y >> 14) & 63u;
+    h.pay = ((uint64_t)(e.y & 63u) << 32) | e.x;
+    h.f1 = (e.y >> 6) & 31u;
+    h.f2 = (e.y >> 11) & 31u;
+    h.q = e.y >> 16;
     h.C = e.w & 0xFFFFu;
     h.fp = e.w >> 16;
     const uint32_t D = FMT == kLE ? h.d - 1 : h.d;
-    h.q = h.C ? D / h.C : 0u;
     h.R = D - h.C * h.q;
+}
+
+// Header bits of a well-formed record (a field holding x is the Elias-gamma code of x + 1,
+// 2 floor(log2(x + 1)) + 1 bits): LE g(d) [g(rock_bit) [g(sea_bit) g(first_pos)]] (d = 1
+// stores the neighbour in the rock_bit field), GT all four fields.
+template <int FMT>
+__device__ __forceinline__ uint32_t header_bits(const Hdr& h) {
+    const uint32_t b = 32 - __clz(h.d + 1);  // bits of d + 1
+    uint32_t t = 2 * b - 1;
+    if (FMT == kGT || h.d >= 1) t += 2 * (32 - __clz(h.f1 + 1)) - 1;
+    if (FMT == kGT || h.d >= 2) t += 2 * (32 - __clz(h.f2 + 1)) - 1 + 2 * (32 - __clz(h.fp + 1)) - 1;
+    return t;
 }
 
 // The payload bit of a GT entry points into the device's GT payload stream once the loader

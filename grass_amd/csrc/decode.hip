@@ -53,6 +53,70 @@ __global__ __launch_bounds__(256) void wave_start_kernel(const uint64_t* __restr
     }
 }
 
+// A GT list in the device stream with its wave-uniform constants: block geometry, field
+// widths, the bits of a (q + 1)- and of a q-position block, and shared divisors, so that an
+// element costs two multiply-high divisions and one 32-bit window.
+struct GtList {
+    uint64_t pay;
+    uint32_t v, f1, f2, fp, q, R, big, F1, F0, sh;
+    UDiv d1, d0;
+    __device__ __forceinline__ void init(const Hdr& h, uint32_t vv, uint32_t shift) {
+        pay = h.pay;
+        v = vv;
+        f1 = h.f1;
+        f2 = h.f2;
+        fp = h.fp;
+        q = h.q;
+        R = h.R;
+        sh = shift;
+        big = (q + 1) * R;
+        F1 = (uint32_t)gtx_field(h, q + 1, shift);
+        F0 = (uint32_t)gtx_field(h, q, shift);
+        d1.init(q + 1);
+        d0.init(q);
+    }
+    // Element k: the absolute id at a block head or checkpoint (head = 1), else the gap + 1.
+    __device__ __forceinline__ uint32_t at(const uint32_t* __restrict__ w, uint32_t k, int& head) const {
+        uint32_t blk, i;
+        uint64_t bo;
+        if (k < big) {
+            blk = d1.div(k);
+            i = k - blk * (q + 1);
+            bo = pay + (uint64_t)blk * F1;
+        } else {
+            const uint32_t kk = k - big, b2 = d0.div(kk);
+            i = kk - b2 * q;
+            blk = b2 + R;
+            bo = pay + (uint64_t)R * F1 + (uint64_t)b2 * F0;
+        }
+        head = 1;
+        if (i == 0) {
+            const uint32_t x = read_bits(w, bo, f1);
+            return blk >= fp ? v + x + 1 : v - x - 1;
+        }
+        const uint32_t j = i - 1;
+        const uint64_t p = bo + f1 + j * f2 + (j >> sh) * (32 - f2);
+        if ((i & ((1u << sh) - 1)) == 0) return read_bits(w, p, 32);
+        head = 0;
+        return read_bits(w, p, f2) + 1;
+    }
+};
+
+// Segmented inclusive sum over runs of at most `span` lanes (a lane with head != 0 starts a
+// run): log2(span) shuffle steps instead of six.
+__device__ __forceinline__ uint32_t short_segmented_sum(uint32_t x, int head, uint32_t span) {
+    const int lane = lane_id();
+    for (uint32_t off = 1; off < span && off < (uint32_t)kWave; off <<= 1) {
+        const uint32_t px = __shfl_up(x, off, kWave);
+        const int ph = __shfl_up(head, off, kWave);
+        if (lane >= (int)off && !head) {
+            x += px;
+            head = ph;
+        }
+    }
+    return x;
+}
+
 template <int FMT>
 __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
                                                                const uint64_t* __restrict__ rowptr,
@@ -67,37 +131,45 @@ __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
     // owner of e0, and a bound on the owner of e1 - 1 (the next wave's first owner)
     const uint32_t vlo = wstart[wave];
     const uint32_t vhi = e1 < m ? wstart[wave + 1] : g.n - 1;
+    // GT: an absolute value (block head or checkpoint) starts every run of at most P positions
+    const uint32_t span = 1u << g.cp_shift;
     uint64_t eb = e0;
     if (FMT == kGT) {
         Hdr h;
         load_hdr<kGT>(g, vlo, h);
         uint32_t blk, i;
         gt_locate(h, (uint32_t)(e0 - rowptr[vlo]), blk, i);
-        eb = e0 - (i & ((1u << g.cp_shift) - 1));  // back to the block head or checkpoint
+        eb = e0 - (i & (span - 1));  // back to the block head or checkpoint
     }
     uint32_t carry = 0;
-    // owner of the iteration's first slot (a lower bound for every lane's), wave-uniform
+    // owner of the iteration's first slot (a lower bound for every lane's) and its row bounds,
+    // wave-uniform, reloaded only when it changes
     uint32_t cur = __builtin_amdgcn_readfirstlane(vlo);
-    uint32_t hv = 0xFFFFFFFFu;  // vertex whose header and divisors are cached below
+    uint64_t rc = rowptr[cur], rn = rowptr[cur + 1];
+    uint32_t hv = 0xFFFFFFFFu;  // vertex whose list constants are cached below
     Hdr uh;
     UDiv d1, d0;
+    GtList gl;
     for (uint64_t base = eb; base < e1; base += kWave) {
         const uint64_t e = base + lane;
         const bool act = e < e1;
         const uint64_t ee = act ? e : e1 - 1;
-        const uint64_t rc = rowptr[cur];
-        const uint64_t rn = rowptr[cur + 1];
         uint32_t x;
         int head = 1;
         if (base >= rc && base + (kWave - 1) < rn) {
-            // the whole wave is inside one list: one header, shared divisors, no owner search
+            // the whole wave is inside one list: one header, shared constants, no owner search
             if (cur != hv) {
                 hv = cur;
                 load_hdr<FMT>(g, hv, uh);
-                d1.init(uh.q + 1);
-                d0.init(uh.q);
+                if (FMT == kGT) {
+                    gl.init(uh, hv, g.cp_shift);
+                } else {
+                    d1.init(uh.q + 1);
+                    d0.init(uh.q);
+                }
             }
-            x = element_uniform<FMT>(g, uh, hv, (uint32_t)(ee - rc), d1, d0, head);
+            if (FMT == kGT) x = gl.at(g.w, (uint32_t)(ee - rc), head);
+            else x = element_uniform<FMT>(g, uh, hv, (uint32_t)(ee - rc), d1, d0, head);
         } else {
             // Owner of slot ee: one coalesced load of the 64 row ends after `cur`, then a 6-step
             // search over them in registers; slots more than 64 vertices past `cur` (only
@@ -121,7 +193,12 @@ __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
                 v = owner_of(rowptr, cur + kWave, vhi, ee);
                 vstart = rowptr[v];
             }
-            cur = __builtin_amdgcn_readfirstlane(__shfl(v, kWave - 1, kWave));
+            const uint32_t nc = __builtin_amdgcn_readfirstlane(__shfl(v, kWave - 1, kWave));
+            if (nc != cur) {
+                cur = nc;
+                rc = rowptr[cur];
+                rn = rowptr[cur + 1];
+            }
             const uint32_t k = (uint32_t)(ee - vstart);
             Hdr h;
             load_hdr<FMT>(g, v, h);
@@ -138,7 +215,7 @@ __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
                 x += carry;
                 head = 1;
             }
-            x = wave_segmented_sum(x, head);
+            x = short_segmented_sum(x, head, span);
             carry = __shfl(x, kWave - 1, kWave);
         }
         if (act && e >= e0) col[e] = x;

@@ -91,7 +91,6 @@ __global__ __launch_bounds__(kValBlock) void validate_kernel(const uint32_t* __r
             h.d = h.f1 = h.f2 = h.fp = h.C = 0;
             h.pay = b0;
         }
-        h.hbits = (uint32_t)(h.pay - b0);
         deg[v] = h.d;
         hdr[v] = pack_hdr(h, FMT);
         sum += h.d;
@@ -195,7 +194,7 @@ __global__ __launch_bounds__(kExpBlock) void gt_expand_kernel(DevGraph g, const 
         if (pend_w != ~0ull && lane == 0) out[pend_w] = pend;
         if (lane == 0) {
             e.x = (uint32_t)hx.pay;
-            e.y = (e.y & ~0xFFu) | (uint32_t)(hx.pay >> 32);
+            e.y = (e.y & ~63u) | (uint32_t)(hx.pay >> 32);
             hdr[v] = e;
         }
     }
@@ -316,7 +315,7 @@ int build_gt_stream(grass_graph* g, uint64_t* words);
 int load_graph(grass_graph* g, ByteSource& gsrc, ByteSource& osrc) {
     const uint64_t gbytes = gsrc.size(), obytes = osrc.size();
     if (obytes < 2 || gbytes < 1) return GRASS_EFORMAT;
-    if (gbytes >= (1ull << 37)) return GRASS_EFORMAT;  // payload bits must fit the 40-bit cache field
+    if (8 * gbytes >= kMaxPayloadBits) return GRASS_EFORMAT;  // payload bits must fit the header cache
     hipStream_t s = g->stream;
     const uint64_t gwords = (gbytes + 3) / 4 + kPadWords;
     const uint64_t owords = (obytes + 3) / 4 + kPadWords;
@@ -403,7 +402,16 @@ int load_graph(grass_graph* g, ByteSource& gsrc, ByteSource& osrc) {
         int rc2 = build_gt_stream(g, &xwords);
         if (rc2) return rc2;
     }
-    g->device_bytes = gwords * 4 + (n + 1) * 8 + n * 4 + hdr_bytes + xwords * 4;
+    // What stays resident is what the kernels read: the payload words (LE: the .graph image; GT:
+    // the payload stream, which replaces the image), the header cache and the degrees.  The record
+    // starts are folded into the header cache.
+    (void)hipFree(g->d_start);
+    g->d_start = nullptr;
+    if (g->fmt == kGT) {
+        (void)hipFree(g->d_words);
+        g->d_words = nullptr;
+    }
+    g->device_bytes = (g->fmt == kGT ? xwords : gwords) * 4 + n * 4 + hdr_bytes;
     return GRASS_OK;
 }
 
@@ -420,8 +428,8 @@ int build_gt_stream(grass_graph* g, uint64_t* words) {
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     uint32_t* out = nullptr;
     int rc = status_of(e);
-    // payload bits must fit the header cache's 40-bit field
-    if (!rc && total >= (1ull << 35)) rc = GRASS_EFORMAT;
+    // payload bits must fit the header cache
+    if (!rc && 32 * total >= kMaxPayloadBits) rc = GRASS_EFORMAT;
     if (!rc && hipMalloc(&out, (total + kPadWords) * 4) != hipSuccess) rc = GRASS_ENOMEM;
     if (!rc) {
         e = hipMemsetAsync(out, 0, (total + kPadWords) * 4, s);

@@ -36,6 +36,7 @@
 namespace grass {
 
 constexpr uint32_t kIndChunk = 2048;  // adjacency entries per chunk (32 wave iterations)
+constexpr uint32_t kMdrwHdrLdsMax = 4096;  // largest frontier whose headers the walk keeps in LDS
 constexpr int kSortBlock = 1024;
 
 struct MdDesc {
@@ -47,7 +48,10 @@ struct MdDesc {
 
 // ---------------------------------------------------------------------------- 1. walks
 
-template <int FMT>
+// HDR: the frontier entries' header-cache words sit in LDS next to their ids and degrees (20 bytes
+// per entry instead of 8), so a step's chain is two dependent loads (the picked entry's neighbour,
+// then that neighbour's header) instead of three (header, neighbour, degree).
+template <int FMT, bool HDR>
 __global__ __launch_bounds__(kWave) void mdrw_walk_kernel(DevGraph g, const MdDesc* __restrict__ desc,
                                                           uint32_t I, unsigned long long* queue,
                                                           uint64_t seed, uint32_t* __restrict__ init,
@@ -58,6 +62,9 @@ __global__ __launch_bounds__(kWave) void mdrw_walk_kernel(DevGraph g, const MdDe
     extern __shared__ uint32_t lds[];
     uint32_t* F = lds;         // frontier ids
     uint32_t* D = lds + mmax;  // their degrees
+    uint32_t* Hx = lds + 2 * mmax;  // HDR: their header words x, y, w (z is the degree)
+    uint32_t* Hy = lds + 3 * mmax;
+    uint32_t* Hw = lds + 4 * mmax;
     const int lane = lane_id();
     for (;;) {
         unsigned long long gi = 0;
@@ -69,7 +76,15 @@ __global__ __launch_bounds__(kWave) void mdrw_walk_kernel(DevGraph g, const MdDe
         for (uint32_t j = lane; j < m; j += kWave) {
             const uint32_t v = (uint32_t)uniform(seed, q.inst, j, kTagStart, 0, g.n);
             F[j] = v;
-            D[j] = g.deg[v];
+            if (HDR) {
+                const uint4 e = g.hdr[v];
+                D[j] = e.z;
+                Hx[j] = e.x;
+                Hy[j] = e.y;
+                Hw[j] = e.w;
+            } else {
+                D[j] = g.deg[v];
+            }
             init[q.init_base + j] = v;
         }
         __syncthreads();
@@ -115,7 +130,8 @@ __global__ __launch_bounds__(kWave) void mdrw_walk_kernel(DevGraph g, const MdDe
             }
             const uint32_t u = F[j], du = D[j];
             Hdr h;
-            load_hdr<FMT>(g, u, h);
+            if (HDR) unpack_hdr<FMT>(make_uint4(Hx[j], Hy[j], du, Hw[j]), h);
+            else load_hdr<FMT>(g, u, h);
             const uint32_t k = (uint32_t)mulhi64(((uint64_t)r.w << 32) | r.z, du);
             const uint32_t v = get_neighbor<FMT>(g, h, u, k, nullptr);
             if (v >= g.n) {  // the graph decodes to garbage: stop this instance
@@ -123,13 +139,20 @@ __global__ __launch_bounds__(kWave) void mdrw_walk_kernel(DevGraph g, const MdDe
                 bad = true;
                 break;
             }
-            const uint32_t dv = g.deg[v];
-            __syncthreads();  // every lane has read F[j], D[j]
+            uint4 ev = make_uint4(0, 0, 0, 0);
+            if (HDR) ev = g.hdr[v];
+            const uint32_t dv = HDR ? ev.z : g.deg[v];
+            __syncthreads();  // every lane has read F[j], D[j] (and the header words)
             if (lane == 0) {
                 ssrc[q.step_base + t] = u;
                 sdst[q.step_base + t] = v;
                 F[j] = v;
                 D[j] = dv;
+                if (HDR) {
+                    Hx[j] = ev.x;
+                    Hy[j] = ev.y;
+                    Hw[j] = ev.w;
+                }
             }
             if (lane == L) bs = bs - du + dv;
             W = W - du + dv;
@@ -547,29 +570,57 @@ static int sync_read(hipStream_t s, void* host, const void* dev, size_t bytes) {
     return status_of(e);
 }
 
-int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t target, uint64_t seed,
-         hipStream_t s, grass_mdrw_result** out) {
-    if (!out || (I && !reqs) || target > GRASS_MDRW_MAX_TARGET) return GRASS_EINVAL;
-    *out = nullptr;
-    std::vector<MdDesc> desc(I);
+// Working memory, sizes and outputs of one MDRW batch.  The eager entry point sizes everything from
+// totals it reads back between the phases and frees the working memory on return.  A captured
+// batch (grass_plan_create_mdrw) keeps the state of the eager run that created it: the requests and
+// the seed are fixed, so every replay has the same sizes, and the replay neither allocates nor
+// reads anything back.
+struct MdrwState {
+    std::vector<MdDesc> desc;
+    uint32_t I = 0, target = 0;
+    uint64_t seed = 0;
     uint64_t tot_steps = 0, tot_init = 0, tot_vb = 0;
     uint32_t mmax = 1, vmax = 1;
+    uint64_t NS = 0, NV = 0, NC = 0, NI = 0;  // steps, sampled vertices, list chunks, induced edges
+    DevMem tmp, tmp2, tmp3;
+    grass_mdrw_result* res = nullptr;
+    ~MdrwState() {
+        if (res) res->release();
+    }
+};
+
+// Checks the requests and fills the descriptors and bounds.
+static int mdrw_prepare(const grass_mdrw_request* reqs, uint32_t I, uint32_t target, uint64_t seed,
+                        MdrwState& st) {
+    if ((I && !reqs) || target > GRASS_MDRW_MAX_TARGET) return GRASS_EINVAL;
+    st.desc.resize(I);
+    st.I = I;
+    st.target = target;
+    st.seed = seed;
     for (uint32_t i = 0; i < I; ++i) {
         const uint32_t m = reqs[i].frontier;
         if (m == 0 || m > GRASS_MDRW_MAX_FRONTIER) return GRASS_EINVAL;
-        const uint32_t st = target > m ? target - m : 0;
-        desc[i] = MdDesc{tot_steps, tot_init, tot_vb, reqs[i].instance, m, st, 0};
-        tot_steps += st;
-        tot_init += m;
-        tot_vb += (uint64_t)m + st;
-        mmax = std::max(mmax, m);
-        vmax = std::max(vmax, m + st);
+        const uint32_t stp = target > m ? target - m : 0;
+        st.desc[i] = MdDesc{st.tot_steps, st.tot_init, st.tot_vb, reqs[i].instance, m, stp, 0};
+        st.tot_steps += stp;
+        st.tot_init += m;
+        st.tot_vb += (uint64_t)m + stp;
+        st.mmax = std::max(st.mmax, m);
+        st.vmax = std::max(st.vmax, m + stp);
     }
-    std::unique_ptr<grass_mdrw_result> res(new (std::nothrow) grass_mdrw_result());
-    if (!res) return GRASS_ENOMEM;
+    st.res = new (std::nothrow) grass_mdrw_result();
+    return st.res ? GRASS_OK : GRASS_ENOMEM;
+}
+
+// Enqueues the batch on s.  replay = false: reads the totals back as they become known and
+// allocates the outputs and working memory to fit (then waits for the batch); replay = true:
+// takes every size and buffer from a previous eager run on the same state and only enqueues.
+static int mdrw_run(grass_graph* g, MdrwState& st, hipStream_t s, bool replay) {
+    const uint32_t I = st.I;
+    grass_mdrw_result* res = st.res;
     res->device = g->device;
     res->out.count = I;
-    if (res->offs.alloc(3 * (uint64_t)(I + 1) * 8) != hipSuccess) return GRASS_ENOMEM;
+    if (!replay && res->offs.alloc(3 * (uint64_t)(I + 1) * 8) != hipSuccess) return GRASS_ENOMEM;
     uint64_t* step_off = res->offs.as<uint64_t>();
     uint64_t* vert_off = step_off + (I + 1);
     uint64_t* ind_off = vert_off + (I + 1);
@@ -577,82 +628,96 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
     res->out.vert_off_dev = vert_off;
     res->out.ind_off_dev = ind_off;
     if (I == 0) {
-        uint64_t z[3] = {0, 0, 0};
-        for (int k = 0; k < 3; ++k)
-            if (hipMemcpy(step_off + k, &z[k], 8, hipMemcpyHostToDevice) != hipSuccess) return GRASS_EDEVICE;
-        *out = res.release();
-        return GRASS_OK;
+        hipLaunchKernelGGL(fill_kernel<uint64_t>, dim3(1), dim3(64), 0, s, step_off, (uint64_t)3, (uint64_t)0);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess && !replay) e = hipStreamSynchronize(s);
+        return status_of(e);
     }
     const DevGraph dg = g->dev();
     const bool le = g->fmt == kLE;
+    const uint64_t seed = st.seed;
 
     // phase 1: walks and vertex sets in bound layout
     const uint64_t nbI = scan_blocks(I);
-    DevMem tmp;
-    Carve need(nullptr);
-    auto plan = [&](Carve& c) {
+    auto plan1 = [&](Carve& c) {
         c.take<MdDesc>(I);
         c.take<unsigned long long>(1);
-        c.take<uint32_t>(tot_init);
-        c.take<uint32_t>(tot_steps);
-        c.take<uint32_t>(tot_steps);
+        c.take<uint32_t>(st.tot_init);
+        c.take<uint32_t>(st.tot_steps);
+        c.take<uint32_t>(st.tot_steps);
         c.take<uint32_t>(I);
-        c.take<uint32_t>(tot_vb);
+        c.take<uint32_t>(st.tot_vb);
         c.take<uint32_t>(I);
         c.take<uint64_t>(nbI + 2);
         c.take<uint64_t>(1);
     };
-    plan(need);
-    if (tmp.alloc(need.off + 256) != hipSuccess) return GRASS_ENOMEM;
-    Carve c(tmp.p);
+    if (!replay) {
+        Carve need(nullptr);
+        plan1(need);
+        if (st.tmp.alloc(need.off + 256) != hipSuccess) return GRASS_ENOMEM;
+    }
+    Carve c(st.tmp.p);
     MdDesc* d_desc = c.take<MdDesc>(I);
     unsigned long long* d_queue = c.take<unsigned long long>(1);
-    uint32_t* d_init = c.take<uint32_t>(tot_init);
-    uint32_t* d_ssrc = c.take<uint32_t>(tot_steps);
-    uint32_t* d_sdst = c.take<uint32_t>(tot_steps);
+    uint32_t* d_init = c.take<uint32_t>(st.tot_init);
+    uint32_t* d_ssrc = c.take<uint32_t>(st.tot_steps);
+    uint32_t* d_sdst = c.take<uint32_t>(st.tot_steps);
     uint32_t* d_scnt = c.take<uint32_t>(I);
-    uint32_t* d_vb = c.take<uint32_t>(tot_vb);
+    uint32_t* d_vb = c.take<uint32_t>(st.tot_vb);
     uint32_t* d_vcnt = c.take<uint32_t>(I);
     uint64_t* d_part = c.take<uint64_t>(nbI + 2);
     uint64_t* d_tot = c.take<uint64_t>(1);
-    hipError_t e = hipMemcpyAsync(d_desc, desc.data(), I * sizeof(MdDesc), hipMemcpyHostToDevice, s);
+    // the descriptors are uploaded once (a replay reads the copy the eager run left)
+    hipError_t e = replay ? hipSuccess
+                          : hipMemcpyAsync(d_desc, st.desc.data(), I * sizeof(MdDesc), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return status_of(e);
     hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, s, d_queue, (uint64_t)1, 0ull);
     {
-        const size_t lds = 2ull * mmax * 4;
-        const void* kfn = le ? (const void*)mdrw_walk_kernel<kLE> : (const void*)mdrw_walk_kernel<kGT>;
+        // headers in LDS while two instances still fit a CU (20 bytes per frontier entry)
+        const bool hdr = st.mmax <= kMdrwHdrLdsMax;
+        const size_t lds = (hdr ? 5ull : 2ull) * st.mmax * 4;
+        const void* kfn = le ? (hdr ? (const void*)mdrw_walk_kernel<kLE, true> : (const void*)mdrw_walk_kernel<kLE, false>)
+                             : (hdr ? (const void*)mdrw_walk_kernel<kGT, true> : (const void*)mdrw_walk_kernel<kGT, false>);
+        allow_lds(kfn, lds);
         int dev = 0, cus = 0, per = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kfn, kWave, lds);
         const unsigned grid = (unsigned)std::min<uint64_t>(I, (uint64_t)std::max(cus, 1) * std::max(per, 1));
-        if (le)
-            hipLaunchKernelGGL(mdrw_walk_kernel<kLE>, dim3(grid), dim3(kWave), lds, s, dg, d_desc, I, d_queue,
-                               seed, d_init, d_ssrc, d_sdst, d_scnt, mmax, g->d_err);
-        else
-            hipLaunchKernelGGL(mdrw_walk_kernel<kGT>, dim3(grid), dim3(kWave), lds, s, dg, d_desc, I, d_queue,
-                               seed, d_init, d_ssrc, d_sdst, d_scnt, mmax, g->d_err);
+#define GRASS_MDRW_WALK(F, H)                                                                                   \
+    hipLaunchKernelGGL((mdrw_walk_kernel<F, H>), dim3(grid), dim3(kWave), lds, s, dg, d_desc, I, d_queue, seed, \
+                       d_init, d_ssrc, d_sdst, d_scnt, st.mmax, g->d_err)
+        if (le && hdr) GRASS_MDRW_WALK(kLE, true);
+        else if (le) GRASS_MDRW_WALK(kLE, false);
+        else if (hdr) GRASS_MDRW_WALK(kGT, true);
+        else GRASS_MDRW_WALK(kGT, false);
+#undef GRASS_MDRW_WALK
     }
-    const uint32_t cap = next_pow2(vmax);
+    const uint32_t cap = next_pow2(st.vmax);
     hipLaunchKernelGGL(mdrw_verts_kernel, dim3(I), dim3(kSortBlock), (size_t)cap * 4, s, d_desc, d_init,
                        d_sdst, d_scnt, cap, d_vb, d_vcnt);
     e = exclusive_scan(U32Gen{d_scnt}, Count{I, nullptr}, I, step_off, d_part, d_tot, s);
     if (e == hipSuccess) e = exclusive_scan(U32Gen{d_vcnt}, Count{I, nullptr}, I, vert_off, d_part, d_tot, s);
     if (e != hipSuccess) return status_of(e);
-    uint64_t totals[2];
-    if (hipMemcpyAsync(&totals[0], step_off + I, 8, hipMemcpyDeviceToHost, s) != hipSuccess) return GRASS_EDEVICE;
-    int rc = sync_read(s, &totals[1], vert_off + I, 8);
-    if (rc) return rc;
-    uint32_t err[2] = {0, 0};
-    if (hipMemcpy(err, g->d_err, sizeof(err), hipMemcpyDeviceToHost) != hipSuccess) return GRASS_EDEVICE;
-    if (err[0] == kErrBadNeighbor) {
-        (void)hipMemset(g->d_err, 0, sizeof(err));
-        return GRASS_EFORMAT;
+    if (!replay) {
+        uint64_t totals[2];
+        if (hipMemcpyAsync(&totals[0], step_off + I, 8, hipMemcpyDeviceToHost, s) != hipSuccess) return GRASS_EDEVICE;
+        int rc = sync_read(s, &totals[1], vert_off + I, 8);
+        if (rc) return rc;
+        uint32_t err[2] = {0, 0};
+        if (hipMemcpy(err, g->d_err, sizeof(err), hipMemcpyDeviceToHost) != hipSuccess) return GRASS_EDEVICE;
+        if (err[0] == kErrBadNeighbor) {
+            (void)hipMemset(g->d_err, 0, sizeof(err));
+            return GRASS_EFORMAT;
+        }
+        st.NS = totals[0];
+        st.NV = totals[1];
+        if (res->steps.alloc(st.NS * 8) != hipSuccess || res->verts.alloc(st.NV * 4) != hipSuccess)
+            return GRASS_ENOMEM;
     }
-    const uint64_t NS = totals[0], NV = totals[1];
+    const uint64_t NS = st.NS, NV = st.NV;
     res->out.steps = NS;
     res->out.verts = NV;
-    if (res->steps.alloc(NS * 8) != hipSuccess || res->verts.alloc(NV * 4) != hipSuccess) return GRASS_ENOMEM;
     uint32_t* o_ssrc = res->steps.as<uint32_t>();
     uint32_t* o_sdst = o_ssrc + NS;
     uint32_t* o_verts = res->verts.as<uint32_t>();
@@ -665,16 +730,17 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
 
     // phase 2: chunks of the sampled vertices' lists
     const uint64_t nbV = scan_blocks(NV);
-    DevMem tmp2;
-    if (tmp2.alloc((NV + 1) * 8 + (nbV + 2) * 8 + 512) != hipSuccess) return GRASS_ENOMEM;
-    Carve c2(tmp2.p);
+    if (!replay && st.tmp2.alloc((NV + 1) * 8 + (nbV + 2) * 8 + 512) != hipSuccess) return GRASS_ENOMEM;
+    Carve c2(st.tmp2.p);
     uint64_t* d_coff = c2.take<uint64_t>(NV + 1);
     uint64_t* d_part2 = c2.take<uint64_t>(nbV + 2);
     e = exclusive_scan(ChunkGen{o_verts, dg.deg}, Count{NV, nullptr}, NV, d_coff, d_part2, d_part2 + nbV + 1, s);
     if (e != hipSuccess) return status_of(e);
-    uint64_t NC = 0;
-    rc = sync_read(s, &NC, d_coff + NV, 8);
-    if (rc) return rc;
+    if (!replay) {
+        int rc = sync_read(s, &st.NC, d_coff + NV, 8);
+        if (rc) return rc;
+    }
+    const uint64_t NC = st.NC;
 
     // phase 3: per slice of up to kSlice instances, membership rows for the slice's distinct
     // vertices; every distinct vertex has its list decoded once per pass (count, then write)
@@ -683,7 +749,6 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
     uint32_t W = 1;
     while (W * 64 < std::min<uint32_t>(I, kSlice)) W <<= 1;
     const uint64_t nbN = scan_blocks(n), nbC = scan_blocks(NC);
-    DevMem tmp3;
     unsigned long long* d_rows = nullptr;
     uint64_t *d_did = nullptr, *d_pbase = nullptr, *d_uoff = nullptr, *d_nu = nullptr, *d_partN = nullptr,
              *d_ccnt = nullptr, *d_cpos = nullptr, *d_part3 = nullptr;
@@ -706,10 +771,12 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
         d_cpos = c.take<uint64_t>(NC + 1);
         d_part3 = c.take<uint64_t>(nbC + 2);
     };
-    Carve need3(nullptr);
-    plan3(need3);
-    if (tmp3.alloc(need3.off + 256) != hipSuccess) return GRASS_ENOMEM;
-    Carve c3(tmp3.p);
+    if (!replay) {
+        Carve need3(nullptr);
+        plan3(need3);
+        if (st.tmp3.alloc(need3.off + 256) != hipSuccess) return GRASS_ENOMEM;
+    }
+    Carve c3(st.tmp3.p);
     plan3(c3);
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
@@ -771,11 +838,13 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
     }
     e = exclusive_scan(U64Gen{d_ccnt}, Count{NC, nullptr}, NC, d_cpos, d_part3, d_part3 + nbC + 1, s);
     if (e != hipSuccess) return status_of(e);
-    uint64_t NI = 0;
-    rc = sync_read(s, &NI, d_cpos + NC, 8);
-    if (rc) return rc;
+    if (!replay) {
+        int rc = sync_read(s, &st.NI, d_cpos + NC, 8);
+        if (rc) return rc;
+        if (res->ind.alloc(st.NI * 8) != hipSuccess) return GRASS_ENOMEM;
+    }
+    const uint64_t NI = st.NI;
     res->out.induced = NI;
-    if (res->ind.alloc(NI * 8) != hipSuccess) return GRASS_ENOMEM;
     uint32_t* o_isrc = res->ind.as<uint32_t>();
     uint32_t* o_idst = o_isrc + NI;
     res->out.ind_src_dev = o_isrc;
@@ -791,11 +860,45 @@ int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t ta
     hipLaunchKernelGGL(mdrw_ind_off_kernel, dim3((I + 256) / 256), dim3(256), 0, s, vert_off, d_coff, d_cpos, I,
                        ind_off);
     e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(s);  // the temporaries are freed on return
-    if (e != hipSuccess) return status_of(e);
-    *out = res.release();
+    if (e == hipSuccess && !replay) e = hipStreamSynchronize(s);  // the working memory goes with the state
+    return status_of(e);
+}
+
+int mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t target, uint64_t seed,
+         hipStream_t s, grass_mdrw_result** out) {
+    if (!out) return GRASS_EINVAL;
+    *out = nullptr;
+    MdrwState st;
+    int rc = mdrw_prepare(reqs, I, target, seed, st);
+    if (!rc) rc = mdrw_run(g, st, s, false);
+    if (rc) return rc;
+    *out = st.res;
+    st.res = nullptr;
     return GRASS_OK;
 }
+
+// ---------------------------------------------------------------------------- captured batches
+
+struct MdrwCapture {
+    MdrwState st;
+};
+
+int mdrw_capture_prepare(grass_graph* g, const grass_mdrw_request* reqs, uint32_t I, uint32_t target,
+                         uint64_t seed, hipStream_t s, MdrwCapture** out) {
+    std::unique_ptr<MdrwCapture> c(new (std::nothrow) MdrwCapture());
+    if (!c) return GRASS_ENOMEM;
+    int rc = mdrw_prepare(reqs, I, target, seed, c->st);
+    if (!rc) rc = mdrw_run(g, c->st, s, false);
+    if (rc) return rc;
+    *out = c.release();
+    return GRASS_OK;
+}
+
+int mdrw_capture_enqueue(grass_graph* g, MdrwCapture* c, hipStream_t s) { return mdrw_run(g, c->st, s, true); }
+
+const grass_mdrw_result* mdrw_capture_result(const MdrwCapture* c) { return c->st.res; }
+
+void mdrw_capture_free(MdrwCapture* c) { delete c; }
 
 }  // namespace grass
 

@@ -29,6 +29,7 @@ struct grass_plan {
     hipEvent_t fork = nullptr, join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    MdrwCapture* md = nullptr;  // an MDRW batch (grass_plan_create_mdrw)
 
     ~grass_plan() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -41,11 +42,13 @@ struct grass_plan {
         if (s_side) (void)hipStreamDestroy(s_side);
         ws_rw.release();
         ws_kh.release();
+        if (md) mdrw_capture_free(md);
     }
 
     // The batch: walks on s_main, the k-hop pipeline on s_side forked from and joined back into
     // s_main, so that in the graph the two run side by side.
     int enqueue() {
+        if (md) return mdrw_capture_enqueue(g, md, s_main);
         if (!rw.empty()) {
             int rc = rw_batch_enqueue(g, rw.data(), (uint32_t)rw.size(), seed, s_main, ws_rw, nullptr);
             if (rc) return rc;
@@ -64,7 +67,66 @@ struct grass_plan {
     }
 };
 
+namespace {
+
+// Captures p->enqueue() on s_main into p->graph and instantiates it.
+int capture(grass_plan* p) {
+    if (hipStreamBeginCapture(p->s_main, hipStreamCaptureModeRelaxed) != hipSuccess) return GRASS_EDEVICE;
+    int erc = p->enqueue();
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(p->s_main, &gr);
+    p->graph = gr;
+    if (erc) return erc;
+    if (e != hipSuccess || !gr) return GRASS_EDEVICE;
+    if (hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess) return GRASS_EDEVICE;
+    return GRASS_OK;
+}
+
+int make_streams(grass_plan* p) {
+    if (hipStreamCreateWithFlags(&p->s_main, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&p->s_side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&p->join, hipEventDisableTiming) != hipSuccess)
+        return GRASS_EDEVICE;
+    return GRASS_OK;
+}
+
+}  // namespace
+
 extern "C" {
+
+int grass_plan_create_mdrw(grass_graph* g, const grass_mdrw_request* reqs, uint32_t count, uint32_t target,
+                           uint64_t seed, grass_plan** out) {
+    if (!g || !out || (count && !reqs)) return GRASS_EINVAL;
+    *out = nullptr;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    if (!grass::enter_device(g->device)) return GRASS_EDEVICE;
+    grass_plan* p = new (std::nothrow) grass_plan();
+    int rc = p ? GRASS_OK : GRASS_ENOMEM;
+    if (!rc) {
+        p->g = g;
+        p->seed = seed;
+        rc = make_streams(p);
+    }
+    // 1. one eager run (synchronous): validates the requests, sizes the outputs and working memory
+    if (!rc) rc = mdrw_capture_prepare(g, reqs, count, target, seed, p->s_main, &p->md);
+    // 2. capture the same batch into those buffers
+    if (!rc) rc = capture(p);
+    (void)hipSetDevice(prev);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return GRASS_OK;
+}
+
+int grass_plan_mdrw_output(const grass_plan* p, grass_mdrw_output* out) {
+    if (!p || !out || !p->md) return GRASS_EINVAL;
+    *out = mdrw_capture_result(p->md)->out;
+    return GRASS_OK;
+}
 
 int grass_plan_create(grass_graph* g, const grass_rw_request* rw, uint32_t nrw, const grass_khop_request* kh,
                       uint32_t nkh, const grass_khop_output* kout, uint64_t seed, grass_plan** out) {
@@ -81,30 +143,14 @@ int grass_plan_create(grass_graph* g, const grass_rw_request* rw, uint32_t nrw, 
         p->kh.assign(kh, kh + nkh);
         if (kout) p->kout = *kout;
         p->seed = seed;
-        if (hipStreamCreateWithFlags(&p->s_main, hipStreamNonBlocking) != hipSuccess ||
-            hipStreamCreateWithFlags(&p->s_side, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&p->join, hipEventDisableTiming) != hipSuccess)
-            rc = GRASS_EDEVICE;
+        rc = make_streams(p);
     }
     // 1. one eager run: validates the requests and sizes every workspace
     if (!rc) rc = p->enqueue();
     if (!rc && hipStreamSynchronize(p->s_main) != hipSuccess) rc = GRASS_EDEVICE;
     if (!rc && !p->kh.empty() && *p->ws_kh.result_slot() > p->kout.capacity) rc = GRASS_ECAPACITY;
     // 2. capture (nothing allocates now: the workspaces already fit)
-    if (!rc) {
-        if (hipStreamBeginCapture(p->s_main, hipStreamCaptureModeRelaxed) != hipSuccess) {
-            rc = GRASS_EDEVICE;
-        } else {
-            int erc = p->enqueue();
-            hipGraph_t gr = nullptr;
-            hipError_t e = hipStreamEndCapture(p->s_main, &gr);
-            p->graph = gr;
-            if (erc) rc = erc;
-            else if (e != hipSuccess || !gr) rc = GRASS_EDEVICE;
-        }
-    }
-    if (!rc && hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess) rc = GRASS_EDEVICE;
+    if (!rc) rc = capture(p);
     (void)hipSetDevice(prev);
     if (rc) {
         delete p;

@@ -148,8 +148,8 @@ struct grass_graph {
     uint64_t graph_bytes = 0, offset_bytes = 0, total_bits = 0;
     uint32_t ubl = 0;
     uint64_t device_bytes = 0;
-    uint32_t* d_words = nullptr;
-    uint64_t* d_start = nullptr;
+    uint32_t* d_words = nullptr;  // .graph words: LE payload; GT: used at load only
+    uint64_t* d_start = nullptr;  // record bit starts: used at load only
     uint32_t* d_deg = nullptr;
     uint4* d_hdr = nullptr;  // decoded-header cache, 16 B per vertex
     uint32_t* d_gtx = nullptr;  // GT: checkpointed payload stream the samplers decode
@@ -166,7 +166,6 @@ struct grass_graph {
     grass::DevGraph dev() const {
         grass::DevGraph d;
         d.w = d_gtx ? d_gtx : d_words;
-        d.start = d_start;
         d.deg = d_deg;
         d.hdr = d_hdr;
         d.n = (uint32_t)n;
@@ -274,5 +273,15 @@ int khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t count, u
 
 int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t count, uint64_t seed,
                  grass_khop_output* out, hipStream_t stream, Workspace& ws);
+
+// mdrw.hip: an MDRW batch kept for capture.  _prepare runs it once eagerly on `stream` (which
+// sizes its outputs and working memory); _enqueue enqueues the same batch again into those
+// buffers without allocating or reading anything back (so it can be captured).
+struct MdrwCapture;
+int mdrw_capture_prepare(grass_graph* g, const grass_mdrw_request* reqs, uint32_t count, uint32_t target,
+                         uint64_t seed, hipStream_t stream, MdrwCapture** out);
+int mdrw_capture_enqueue(grass_graph* g, MdrwCapture* c, hipStream_t stream);
+const grass_mdrw_result* mdrw_capture_result(const MdrwCapture* c);
+void mdrw_capture_free(MdrwCapture* c);
 
 }  // namespace grass

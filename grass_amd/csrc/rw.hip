@@ -77,7 +77,7 @@ __device__ __forceinline__ uint64_t walk_wave(const DevGraph& g, bool active, ui
         if (live) {
             Hdr h;
             load_hdr<FMT>(g, v, h);
-            if (COUNT) acc += 8 + bytes_of_bits(h.hbits);
+            if (COUNT) acc += 8 + bytes_of_bits(header_bits<FMT>(h));
             if (h.d == 0) {
                 live = false;
             } else {

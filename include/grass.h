@@ -85,8 +85,10 @@ int grass_close(grass_graph *g);
 
 int grass_info(const grass_graph *g, grass_graph_info *info);
 
-/* Device views of the resident graph: big-endian .graph words, record bit starts (n+1),
- * degrees (n).  Valid until grass_close. */
+/* Device views of the resident graph: the payload words the samplers decode (LE: the .graph
+ * image as big-endian words; GT: the payload stream built at load, which replaces the image),
+ * and the degrees (n).  *starts_dev is set to NULL: record starts are used at load only (the
+ * header cache holds each record's payload position).  Valid until grass_close. */
 int grass_device_arrays(const grass_graph *g, const uint32_t **words_dev,
                         const uint64_t **starts_dev, const uint32_t **degree_dev);
 
@@ -294,6 +296,17 @@ int grass_plan_launch(grass_plan *p, void *stream);
 /* Edge total of the last completed launch's k-hop part (read after the stream passed it). */
 int grass_plan_khop_total(const grass_plan *p, uint64_t *total);
 int grass_plan_destroy(grass_plan *p);
+
+/* A plan of one MDRW batch: grass_mdrw on `reqs`, `target` and `seed`, run once eagerly (which
+ * sizes the outputs and the working memory) and captured; each grass_plan_launch replays it into
+ * the same outputs, whose device views grass_plan_mdrw_output returns (valid until
+ * grass_plan_destroy; read them after the launch stream has passed the replay).  The requests and
+ * the seed are fixed at creation, so every replay writes the result grass_mdrw returns for them.
+ * Replaces the step loop that drives MultiRWApp (LE/src/grass.cu:1453-1861 over
+ * LE/src/apps/multiRW/multiRW.cu:22-97) with one submission. */
+int grass_plan_create_mdrw(grass_graph *g, const grass_mdrw_request *reqs, uint32_t count,
+                           uint32_t target, uint64_t seed, grass_plan **out);
+int grass_plan_mdrw_output(const grass_plan *p, grass_mdrw_output *out);
 
 /* ------------------------------------------------------------------ serving
  * No reference counterpart (the reference runs one compile-time app over every vertex,

@@ -74,3 +74,50 @@ def test_plan_replays_equal_oracle(ga, oracle_mod, fmt):
     plan.close()
     g.close()
     torch.cuda.set_stream(torch.cuda.default_stream())
+
+
+MDRW_KEYS = ("step_off", "step_src", "step_dst", "vert_off", "verts", "ind_off", "ind_src", "ind_dst")
+
+
+def _clobber(ga, g, plan):
+    """Overwrites every output array of an MDRW plan with 0xFF bytes, so that a replay that did not
+    rewrite them shows."""
+    o = ga.MdrwOutput()
+    assert ga.lib().grass_plan_mdrw_output(plan._h, ga.C.byref(o)) == ga.OK
+    I = plan.I
+    for ptr, nbytes in ((o.step_off_dev, 8 * (I + 1)), (o.vert_off_dev, 8 * (I + 1)), (o.ind_off_dev, 8 * (I + 1)),
+                        (o.step_src_dev, 4 * o.steps), (o.step_dst_dev, 4 * o.steps), (o.verts_dev, 4 * o.verts),
+                        (o.ind_src_dev, 4 * o.induced), (o.ind_dst_dev, 4 * o.induced)):
+        if nbytes:
+            junk = np.full(nbytes, 0xFF, np.uint8)
+            assert ga.lib().grass_copy_to_device(g.handle, ptr, junk.ctypes.data, nbytes) == ga.OK
+
+
+@pytest.mark.parametrize("fmt", FMTS)
+def test_mdrw_plan_replays_equal_oracle(ga, oracle_mod, fmt):
+    import torch
+    c = load_case("rmat12")
+    gb, ob = c[f"{fmt}_graph"].tobytes(), c[f"{fmt}_offset"].tobytes()
+    g, og = ga.Graph.from_bytes(gb, ob, fmt), oracle_mod.Graph(gb, ob, fmt)
+    rng = np.random.default_rng(21)
+    cases = [([8, 100, 700, 1, 2500], 3000, [5, 6, 7, 8, 9]),
+             # more instances than one induced-edge slice holds (512): the capture holds both slices
+             ([int(x) for x in rng.integers(1, 30, size=600)], 80, list(range(1000, 1600)))]
+    stream = torch.cuda.Stream()
+    for sizes, target, inst in cases:
+        print(f"[plan mdrw] fmt={fmt} instances={len(sizes)} target={target}", flush=True)
+        exp = og.mdrw(sizes, target=target, seed=77, instances=inst)
+        plan = g.mdrw_plan(sizes, target, seed=77, instances=inst)
+        for rep in range(3):
+            _clobber(ga, g, plan)
+            plan.launch(stream=stream)
+            stream.synchronize()
+            res = plan.result()
+            for k in MDRW_KEYS:
+                np.testing.assert_array_equal(res[k], exp[k], err_msg=f"replay {rep}: {k}")
+            # eager work on the same graph between replays
+            g.mdrw([50, 60], target=500, seed=1)
+            g.random_walk(100, 10, seed=3, instance=5)
+            g.sync()
+        plan.close()
+    g.close()
