@@ -149,6 +149,7 @@ def lib():
             "grass_decode_csr": (I32, [P, P, P, P]),
             "grass_random_walk": (I32, [P, P, U64, U32, U64, U32, P, P]),
             "grass_random_walk_batch": (I32, [P, P, U32, U64, P]),
+            "grass_random_walk_batch_noalloc": (I32, [P, P, U32, U64, P]),
             "grass_random_walk_bytes": (I32, [P, P, U64, U32, U64, U32, P, P]),
             "grass_random_walk_batch_bytes": (I32, [P, P, U32, U64, P, P]),
             "grass_khop_bound": (I32, [P, P, U32, P]),

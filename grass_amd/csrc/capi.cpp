@@ -290,6 +290,14 @@ int grass_random_walk_batch(grass_graph* g, const grass_rw_request* reqs, uint32
     return random_walk_batch(g, reqs, count, seed, pick(g, stream), nullptr);
 }
 
+int grass_random_walk_batch_noalloc(grass_graph* g, const grass_rw_request* reqs, uint32_t count,
+                                    uint64_t seed, void* stream) {
+    if (!g) return GRASS_EINVAL;
+    DeviceGuard guard(g->device);
+    if (!guard.ok) return GRASS_EDEVICE;
+    return random_walk_batch_noalloc(g, reqs, count, seed, pick(g, stream));
+}
+
 int grass_khop_bound(const grass_graph* g, const grass_khop_request* reqs, uint32_t count,
                      uint64_t* max_edges) {
     if (!g || (count && !reqs) || !max_edges) return GRASS_EINVAL;

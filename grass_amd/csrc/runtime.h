@@ -262,6 +262,9 @@ int random_walk_batch(grass_graph* g, const grass_rw_request* reqs, uint32_t cou
 // before the stream has passed the batch.
 int rw_batch_enqueue(grass_graph* g, const grass_rw_request* reqs, uint32_t count, uint64_t seed,
                      hipStream_t stream, Workspace& ws, unsigned long long* bytes_dev);
+// grass_random_walk_batch_noalloc: enqueues only, with the library's fixed device array
+int random_walk_batch_noalloc(grass_graph* g, const grass_rw_request* reqs, uint32_t count,
+                              uint64_t seed, hipStream_t stream);
 
 // khop.hip
 int khop_bound(const grass_graph* g, const grass_khop_request* reqs, uint32_t count,

@@ -297,6 +297,99 @@ int rw_batch_enqueue(grass_graph* g, const grass_rw_request* reqs, uint32_t coun
     return status_of(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ batch without allocation
+// grass_random_walk_batch_noalloc keeps its working memory in one fixed device array per GPU:
+// the work-queue counter, the caller's requests as copied in, and the descriptors built from
+// them on the device.  The array's host-side name is not a device address: the host takes the
+// address with hipGetSymbolAddress once per device and passes that pointer to the kernels.
+constexpr size_t kNaQueueOff = 0;
+constexpr size_t kNaReqOff = 256;
+constexpr size_t kNaDescOff =
+    kNaReqOff + ((GRASS_RW_NOALLOC_MAX_REQUESTS * sizeof(grass_rw_request) + 255) & ~(size_t)255);
+constexpr size_t kNaBytes = kNaDescOff + GRASS_RW_NOALLOC_MAX_REQUESTS * sizeof(RwDesc);
+static_assert(sizeof(grass_rw_request) % 8 == 0, "request copies are read as an array");
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+constexpr int kNaPrepBlock = 1024;
+constexpr uint32_t kNaPerThread = (GRASS_RW_NOALLOC_MAX_REQUESTS + kNaPrepBlock - 1) / kNaPrepBlock;
+
+// Descriptors of the copied requests in their given order: granule ranges by one workgroup's
+// exclusive scan of ceil(walkers / 64).  A request without walkers gets an empty range, so the
+// batch kernel's search (last request whose range starts at or before the granule) skips it.
+// Also zeroes the queue counter (a kernel, not a memset, so the call can be captured).
+__global__ __launch_bounds__(kNaPrepBlock) void rw_noalloc_prepare_kernel(
+    const grass_rw_request* __restrict__ reqs, uint32_t R, RwDesc* __restrict__ desc,
+    unsigned long long* __restrict__ queue) {
+    __shared__ uint64_t lds[kNaPrepBlock / kWave];
+    const uint32_t t = threadIdx.x, i0 = t * kNaPerThread;
+    uint64_t gr[kNaPerThread];
+    uint64_t sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kNaPerThread; ++j) {
+        const uint32_t i = i0 + j;
+        gr[j] = i < R ? (reqs[i].walkers + kWave - 1) / kWave : 0;
+        sum += gr[j];
+    }
+    uint64_t total = 0;
+    uint64_t run = block_exclusive_sum<uint64_t, kNaPrepBlock>(sum, total, lds);
+#pragma unroll
+    for (uint32_t j = 0; j < kNaPerThread; ++j) {
+        const uint32_t i = i0 + j;
+        if (i < R) {
+            const grass_rw_request r = reqs[i];
+            desc[i] = RwDesc{r.walkers, run, r.starts_dev, r.out_dev, r.length, r.instance};
+        }
+        run += gr[j];
+    }
+    if (t == 0) *queue = 0;
+}
+
+namespace {
+std::mutex na_mu;
+void* na_ptr[64] = {};  // device address of `scratch` on each GPU (set on the first call there)
+}  // namespace
+
+// The current device's copy of `scratch` (nullptr if its address cannot be taken).
+static unsigned char* noalloc_scratch() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lock(na_mu);
+    if (!na_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        na_ptr[dev] = p;
+    }
+    return (unsigned char*)na_ptr[dev];
+}
+
+int random_walk_batch_noalloc(grass_graph* g, const grass_rw_request* reqs, uint32_t count,
+                              uint64_t seed, hipStream_t s) {
+    if (count == 0) return GRASS_OK;
+    if (!reqs) return GRASS_EINVAL;
+    if (count > GRASS_RW_NOALLOC_MAX_REQUESTS) return GRASS_ECAPACITY;
+    uint64_t gran = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const grass_rw_request& r = reqs[i];
+        if (r.walkers == 0) continue;
+        if (!r.out_dev || r.walkers > 0xFFFFFFFFull || r.length == 0xFFFFFFFFu) return GRASS_EINVAL;
+        gran += (r.walkers + kWave - 1) / kWave;
+    }
+    if (gran == 0) return GRASS_OK;
+    unsigned char* base = noalloc_scratch();
+    if (!base) return GRASS_EDEVICE;
+    grass_rw_request* d_reqs = (grass_rw_request*)(base + kNaReqOff);
+    RwDesc* d_desc = (RwDesc*)(base + kNaDescOff);
+    unsigned long long* d_queue = (unsigned long long*)(base + kNaQueueOff);
+    hipError_t e = hipMemcpyAsync(d_reqs, reqs, count * sizeof(grass_rw_request), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return status_of(e);
+    hipLaunchKernelGGL(rw_noalloc_prepare_kernel, dim3(1), dim3(kNaPrepBlock), 0, s, d_reqs, count, d_desc,
+                       d_queue);
+    if (g->fmt == kLE) launch_rw_batch<kLE, false>(g, s, d_desc, count, gran, d_queue, seed, g->d_err, nullptr);
+    else launch_rw_batch<kGT, false>(g, s, d_desc, count, gran, d_queue, seed, g->d_err, nullptr);
+    return status_of(hipGetLastError());
+}
+
 int random_walk_batch(grass_graph* g, const grass_rw_request* reqs, uint32_t count, uint64_t seed,
                       hipStream_t s, uint64_t* bytes) {
     if (bytes) *bytes = 0;

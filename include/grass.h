@@ -134,6 +134,20 @@ typedef struct {
 int grass_random_walk_batch(grass_graph *g, const grass_rw_request *reqs, uint32_t count,
                             uint64_t seed, void *stream);
 
+/* grass_random_walk_batch that allocates nothing: no hipMalloc, hipFree or host staging buffer on
+ * any call.  No counterpart in the reference; it is for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation).  The batch's work
+ * queue, a copy of `reqs` and its descriptors live in one fixed device array of the library per
+ * GPU (about 590 KB, held until the process ends), so count <= GRASS_RW_NOALLOC_MAX_REQUESTS
+ * (GRASS_ECAPACITY beyond).  Requests run in their given order (grass_random_walk_batch puts the
+ * longest walks first); every output is identical to grass_random_walk_batch's.  Enqueues on
+ * `stream` and returns without waiting: keep `reqs` unchanged until the stream has passed the
+ * call.  Every call on one GPU shares that array, so calls that overlap on one GPU race: order
+ * them on one stream.  Errors a kernel records are reported by grass_sync. */
+#define GRASS_RW_NOALLOC_MAX_REQUESTS 8192
+int grass_random_walk_batch_noalloc(grass_graph *g, const grass_rw_request *reqs, uint32_t count,
+                                    uint64_t seed, void *stream);
+
 /* Measurement companions (no reference counterpart; the reference only prints timers,
  * LE/src/grass.cu:1865-1866,1984-1987): run the same walks without writing them and return
  * their algorithmic HBM bytes -- per walker 4 (start written, +4 if read from starts_dev), per
