@@ -89,6 +89,39 @@ def serving_bench(g, seed, rng, seconds=3.0, burst=6000):
     return out
 
 
+def random_line_ceiling(buffer_bytes, lanes, steps):
+    """The GPU's rate of random 64-byte lines (libgrass_calib.so, walk-shaped chains: two dependent
+    random lines per step, `lanes` chains of `steps` steps over a buffer of the resident graph's
+    size), as {lines_per_s, GBps (of 64-byte lines), buffer_bytes}; None when the calibration
+    library is not built."""
+    import ctypes as C
+    import torch
+    path = os.path.join(REPO, "grass_amd", "libgrass_calib.so")
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    L.grass_calib_gather.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lines = max(buffer_bytes // 64, 1)
+    buf = torch.zeros(lines * 16, dtype=torch.int32, device="cuda")
+    sink = torch.zeros(8, dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    best = None
+    for it in range(4):  # first run warms up
+        ms, nl = C.c_double(), C.c_uint64()
+        rc = L.grass_calib_gather(C.c_void_p(buf.data_ptr()), lines, 2, steps, lanes, 11 + it, C.c_void_p(stream),
+                                  C.c_void_p(sink.data_ptr()), C.byref(ms), C.byref(nl))
+        if rc:
+            return None
+        if it:
+            rate = nl.value / (ms.value * 1e-3)
+            best = rate if best is None else max(best, rate)
+    del buf
+    torch.cuda.empty_cache()
+    return {"lines_per_s": best, "GBps": round(best * 64 / 1e9, 1), "buffer_bytes": lines * 64,
+            "shape": f"{lanes} chains x {steps} steps x 2 dependent random lines"}
+
+
 def plain_four_queues(g, reqs, kseeds, kfans, seed, reps=3):
     """The step's requests one at a time (a launch group of one request each: one walk launch, or
     one k-hop pipeline), four groups in flight on their own streams and no host synchronisation
@@ -476,12 +509,21 @@ def main():
         traffic, tcc_hit = None, None
     else:
         roof["profiled_launch_ms"] = prof_ms
+    # what random 64-byte lines the memory system delivers, for the walk kernel's traffic (its
+    # accesses are random lines, not a stream): same buffer size as the resident graph and about
+    # the walk's chains and steps
+    walkers = sum(r["walkers"] for r in reqs)
+    ceil = random_line_ceiling(g.device_bytes, walkers, max(1, round(rw_edges / max(walkers, 1))))
+    if ceil:
+        roof["random_line_ceiling"] = ceil
     if tcc_hit is not None:
         roof["tcc_hit_rate"] = tcc_hit
     if traffic is not None:
         roof["traffic"] = traffic
         roof["traffic_per_edge"] = round(traffic / rw_edges, 3)
         roof["traffic_GBps"] = round(traffic / (kern_ms * 1e-3) / 1e9, 2)
+        if ceil:
+            roof["traffic_frac_of_random_line_ceiling"] = round(traffic / (kern_ms * 1e-3) / 1e9 / ceil["GBps"], 4)
 
     extra = {}
     if not args.no_graph:

@@ -95,6 +95,8 @@ __global__ __launch_bounds__(kWave) void mdrw_walk_kernel(DevGraph g, const MdDe
             if (j < m) bs += D[j];
         }
         uint64_t W = wave_sum(bs);
+        // lane l: degree sum of blocks 0..l, kept up to date by each step's one-entry change
+        uint64_t incl = wave_inclusive_sum(bs);
         uint32_t t = 0;
         bool bad = false;
         // The draws of a step do not depend on the walk: lane l computes those of step t0 + l once
@@ -110,23 +112,38 @@ __global__ __launch_bounds__(kWave) void mdrw_walk_kernel(DevGraph g, const MdDe
             r.w = (uint32_t)__builtin_amdgcn_readlane(ahead.w, tl);
             const uint64_t x = mulhi64(((uint64_t)r.y << 32) | r.x, W);
             // block of the pick: first lane whose inclusive block sum exceeds x (exists: total W > x)
-            const uint64_t incl = wave_inclusive_sum(bs);
             const unsigned long long mk = __ballot(incl > x);
             const int L = __ffsll(mk) - 1;
-            const uint64_t xr = x - __shfl(incl - bs, L, kWave);
-            // entry of the pick inside block L
+            const uint64_t xr = x - readlane_u64(incl - bs, L);
+            // entry of the pick inside block L: a 32-bit scan of its degrees when they sum below 2^32
             uint32_t j = 0;
-            uint64_t acc = 0;
-            for (uint32_t c0 = 0; c0 < B; c0 += kWave) {
-                const uint32_t o = c0 + lane, idx = L * B + o;
-                const uint64_t w = (o < B && idx < m) ? D[idx] : 0;
-                const uint64_t inc2 = wave_inclusive_sum(w) + acc;
-                const unsigned long long mk2 = __ballot(inc2 > xr);
-                if (mk2) {
-                    j = L * B + c0 + (__ffsll(mk2) - 1);
-                    break;
+            if (readlane_u64(bs, L) <= 0xFFFFFFFFull) {
+                const uint32_t xr32 = (uint32_t)xr;
+                uint32_t acc = 0;
+                for (uint32_t c0 = 0; c0 < B; c0 += kWave) {
+                    const uint32_t o = c0 + lane, idx = L * B + o;
+                    const uint32_t w = (o < B && idx < m) ? D[idx] : 0u;
+                    const uint32_t inc2 = wave_inclusive_sum(w) + acc;
+                    const unsigned long long mk2 = __ballot(inc2 > xr32);
+                    if (mk2) {
+                        j = L * B + c0 + (__ffsll(mk2) - 1);
+                        break;
+                    }
+                    acc = __shfl(inc2, kWave - 1, kWave);
                 }
-                acc = __shfl(inc2, kWave - 1, kWave);
+            } else {
+                uint64_t acc = 0;
+                for (uint32_t c0 = 0; c0 < B; c0 += kWave) {
+                    const uint32_t o = c0 + lane, idx = L * B + o;
+                    const uint64_t w = (o < B && idx < m) ? D[idx] : 0;
+                    const uint64_t inc2 = wave_inclusive_sum(w) + acc;
+                    const unsigned long long mk2 = __ballot(inc2 > xr);
+                    if (mk2) {
+                        j = L * B + c0 + (__ffsll(mk2) - 1);
+                        break;
+                    }
+                    acc = __shfl(inc2, kWave - 1, kWave);
+                }
             }
             const uint32_t u = F[j], du = D[j];
             Hdr h;
@@ -155,6 +172,7 @@ __global__ __launch_bounds__(kWave) void mdrw_walk_kernel(DevGraph g, const MdDe
                 }
             }
             if (lane == L) bs = bs - du + dv;
+            if (lane >= L) incl = incl - du + dv;
             W = W - du + dv;
             __syncthreads();
         }
