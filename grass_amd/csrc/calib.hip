@@ -1,4 +1,4 @@
-// Random-access ceilings of the memory system (libgrass_calib.so, measurement only).
+// Random-access ceilings and synchronisation costs of the GPU (libgrass_calib.so, measurement only).
 //
 // The sampling kernels do not stream: a random-walk step reads one random 16-byte header and one
 // random span of the payload stream, so their HBM traffic is a count of random 64-byte lines,
@@ -65,9 +65,104 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(const uint32_t* __restri
     if (acc == 0x9E3779B9u) atomicAdd(sink, 1ull);  // keeps the loads; practically never taken
 }
 
+// One grid-wide barrier: every workgroup arrives on a monotonic counter (barrier k is complete when
+// it reaches nblocks * (k + 1)); the last one to arrive publishes generation k + 1, the others
+// wait for it.  The wait is bounded (about a second): a workgroup that gives up reports it and
+// returns, so the kernel always ends.
+__device__ __forceinline__ bool grid_barrier(unsigned* cnt, unsigned* gen, unsigned nblocks, unsigned& my) {
+    __shared__ int ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ok = 1;
+        const unsigned g = my;
+        __threadfence();
+        const unsigned arrived = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == nblocks * (g + 1) - 1) {
+            __hip_atomic_store(gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned spins = 0;
+            while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <= g) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > (1u << 24)) {
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        __threadfence();
+        my = g + 1;
+    }
+    __syncthreads();
+    return ok;
+}
+
+// `iters` rounds of (one global store per thread, one grid barrier); words: state[0] counter,
+// state[1] generation, state[2] workgroups that gave up waiting.
+__global__ __launch_bounds__(kBlock) void barrier_kernel(unsigned* state, unsigned nblocks, unsigned iters,
+                                                         unsigned* __restrict__ data, uint32_t words) {
+    unsigned my = 0;
+    for (unsigned i = 0; i < iters; ++i) {
+        data[(blockIdx.x * kBlock + threadIdx.x + i * 97u) % words] = i;
+        if (!grid_barrier(state, state + 1, nblocks, my)) {
+            if (threadIdx.x == 0) atomicAdd(state + 2, 1u);
+            return;
+        }
+    }
+}
+
+// One global store per thread: a kernel whose cost is its launch.
+__global__ __launch_bounds__(kBlock) void store_kernel(unsigned* __restrict__ data, uint32_t words, unsigned i) {
+    data[(blockIdx.x * kBlock + threadIdx.x + i * 97u) % words] = i;
+}
+
+__global__ void zero_kernel(unsigned* p, unsigned n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+// What it costs to make every workgroup of a launch wait for every other one: `iters` grid barriers
+// inside one launch of `blocks` workgroups (at most what the device holds at once) against `iters`
+// dependent launches of the same workgroups on one stream.  scratch_dev: 16 bytes + data_words x 4
+// bytes of device memory.  Returns 0, or 4 if a barrier wait gave up (*barrier_us is then invalid).
+int grass_calib_sync_cost(void* scratch_dev, uint32_t data_words, uint32_t blocks, uint32_t iters, void* stream,
+                          double* barrier_us, double* boundary_us) {
+    if (!scratch_dev || !data_words || !blocks || !iters || !barrier_us || !boundary_us) return 1;
+    hipStream_t s = (hipStream_t)stream;
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)barrier_kernel, kBlock, 0);
+    if ((uint64_t)blocks > (uint64_t)(cus > 0 ? cus : 1) * (per > 0 ? per : 1)) return 1;  // all must be resident
+    unsigned* state = (unsigned*)scratch_dev;
+    unsigned* data = state + 4;
+    hipEvent_t e0, e1, e2;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess || hipEventCreate(&e2) != hipSuccess)
+        return 2;
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, state, 4u);
+    (void)hipEventRecord(e0, s);
+    hipLaunchKernelGGL(barrier_kernel, dim3(blocks), dim3(kBlock), 0, s, state, blocks, iters, data, data_words);
+    (void)hipEventRecord(e1, s);
+    for (uint32_t i = 0; i < iters; ++i)
+        hipLaunchKernelGGL(store_kernel, dim3(blocks), dim3(kBlock), 0, s, data, data_words, i);
+    (void)hipEventRecord(e2, s);
+    hipError_t e = hipEventSynchronize(e2);
+    float t1 = 0, t2 = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t1, e0, e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t2, e1, e2);
+    unsigned gave_up = 0;
+    if (e == hipSuccess) e = hipMemcpy(&gave_up, state + 2, 4, hipMemcpyDeviceToHost);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (e != hipSuccess) return 3;
+    *barrier_us = t1 * 1e3 / iters;
+    *boundary_us = t2 * 1e3 / iters;
+    return gave_up ? 4 : 0;
+}
+
 
 // Runs one launch of `mode` over a device buffer of lines x 64 bytes and returns its duration
 // (HIP events on `stream`) in *ms and the lines it read in *lines_read.  The grid is the

@@ -343,12 +343,11 @@ __device__ __forceinline__ uint32_t gt_element(const DevGraph& g, const Hdr& h, 
                                                uint32_t i, int& head) {
     const uint32_t sh = g.cp_shift;
     const uint64_t bo = gtx_block_bit(h, blk, sh);
-    head = 1;
-    if (i == 0) return gt_head(g.w, h, v, blk, bo);
-    const uint64_t at = bo + gtx_field(h, i, sh);
-    if ((i & ((1u << sh) - 1)) == 0) return read_bits(g.w, at, 32);
-    head = 0;
-    return read_bits(g.w, at, h.f2) + 1;
+    // one load whatever the kind of field, so that lanes of different kinds do not diverge
+    const bool hd = i == 0, abs = !hd && (i & ((1u << sh) - 1)) == 0;
+    const uint32_t raw = read_bits(g.w, bo + gtx_field(h, i, sh), hd ? h.f1 : abs ? 32u : h.f2);
+    head = hd || abs;
+    return hd ? (blk >= h.fp ? v + raw + 1 : v - raw - 1) : abs ? raw : raw + 1;
 }
 
 // Division by a divisor every lane shares, as a multiply-high and one correction: with
@@ -357,7 +356,13 @@ struct UDiv {
     uint32_t d = 1, m = 0;
     __device__ __forceinline__ void init(uint32_t dd) {
         d = dd;
-        m = dd > 1 ? (uint32_t)(0x100000000ull / dd) : 0u;
+        // floor(2^32 / d) from a 32-bit division: floor((2^32 - 1) / d), plus one when d divides
+        // 2^32 (a power of two), so no 64-bit division is emulated
+        m = 0;
+        if (dd > 1) {
+            m = 0xFFFFFFFFu / dd;
+            if ((uint64_t)(m + 1) * dd == 0x100000000ull) ++m;
+        }
     }
     __device__ __forceinline__ uint32_t div(uint32_t n) const {
         if (d == 1) return n;

@@ -75,7 +75,8 @@ struct GtList {
         d1.init(q + 1);
         d0.init(q);
     }
-    // Element k: the absolute id at a block head or checkpoint (head = 1), else the gap + 1.
+    // Element k: the absolute id at a block head or checkpoint (head = 1), else the gap + 1.  The
+    // field's bit, width and kind are chosen first, so every lane issues the same one load.
     __device__ __forceinline__ uint32_t at(const uint32_t* __restrict__ w, uint32_t k, int& head) const {
         uint32_t blk, i;
         uint64_t bo;
@@ -89,16 +90,12 @@ struct GtList {
             blk = b2 + R;
             bo = pay + (uint64_t)R * F1 + (uint64_t)b2 * F0;
         }
-        head = 1;
-        if (i == 0) {
-            const uint32_t x = read_bits(w, bo, f1);
-            return blk >= fp ? v + x + 1 : v - x - 1;
-        }
         const uint32_t j = i - 1;
-        const uint64_t p = bo + f1 + j * f2 + (j >> sh) * (32 - f2);
-        if ((i & ((1u << sh) - 1)) == 0) return read_bits(w, p, 32);
-        head = 0;
-        return read_bits(w, p, f2) + 1;
+        const bool hd = i == 0, abs = !hd && (i & ((1u << sh) - 1)) == 0;
+        const uint32_t off = hd ? 0u : f1 + j * f2 + (j >> sh) * (32 - f2);
+        const uint32_t raw = read_bits(w, bo + off, hd ? f1 : abs ? 32u : f2);
+        head = hd || abs;
+        return hd ? (blk >= fp ? v + raw + 1 : v - raw - 1) : abs ? raw : raw + 1;
     }
 };
 
@@ -106,6 +103,18 @@ struct GtList {
 // run): log2(span) shuffle steps instead of six.
 __device__ __forceinline__ uint32_t short_segmented_sum(uint32_t x, int head, uint32_t span) {
     const int lane = lane_id();
+    if (span == 4) {  // the default checkpoint interval: two fixed steps
+#pragma unroll
+        for (int off = 1; off < 4; off <<= 1) {
+            const uint32_t px = __shfl_up(x, off, kWave);
+            const int ph = __shfl_up(head, off, kWave);
+            if (lane >= off && !head) {
+                x += px;
+                head = ph;
+            }
+        }
+        return x;
+    }
     for (uint32_t off = 1; off < span && off < (uint32_t)kWave; off <<= 1) {
         const uint32_t px = __shfl_up(x, off, kWave);
         const int ph = __shfl_up(head, off, kWave);
@@ -174,21 +183,25 @@ __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
             // Owner of slot ee: one coalesced load of the 64 row ends after `cur`, then a 6-step
             // search over them in registers; slots more than 64 vertices past `cur` (only
             // possible across vertices without edges) fall back to a binary search of rowptr.
+            // (row ends as 32-bit offsets from the start of `cur`'s list, which is at or before
+            // every slot of the iteration; one list is < 2^31 entries, larger offsets saturate)
             const uint64_t bj = (uint64_t)cur + 1 + lane <= g.n ? rowptr[cur + 1 + lane] : ~0ull;
+            const uint32_t rj = bj - rc > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)(bj - rc);
+            const uint32_t er = (uint32_t)(ee - rc);
             uint32_t pos = 0;
 #pragma unroll
             for (uint32_t st = 32; st >= 1; st >>= 1) {
-                const uint64_t t = __shfl(bj, (int)(pos + st - 1), kWave);
-                if (t <= ee) pos += st;
+                const uint32_t t = __shfl(rj, (int)(pos + st - 1), kWave);
+                if (t <= er) pos += st;
             }
-            const uint64_t last = __shfl(bj, kWave - 1, kWave);
-            if (pos == kWave - 1 && last <= ee) pos = kWave;
-            const uint64_t prev = __shfl(bj, (int)(pos > 0 ? pos - 1 : 0), kWave);
+            const uint32_t last = __shfl(rj, kWave - 1, kWave);
+            if (pos == kWave - 1 && last <= er) pos = kWave;
+            const uint32_t prev = __shfl(rj, (int)(pos > 0 ? pos - 1 : 0), kWave);
             uint32_t v;
             uint64_t vstart;
             if (pos < (uint32_t)kWave) {
                 v = cur + pos;
-                vstart = pos ? prev : rc;
+                vstart = rc + (pos ? prev : 0u);
             } else {
                 v = owner_of(rowptr, cur + kWave, vhi, ee);
                 vstart = rowptr[v];

@@ -566,7 +566,7 @@ int grass_session_create(grass_graph* g, const grass_session_config* cfg, grass_
     if (!s->cfg.memory_budget) s->cfg.memory_budget = 4ull << 30;
     if (!s->cfg.max_batch) s->cfg.max_batch = 512;
     if (!s->cfg.max_inflight) s->cfg.max_inflight = 2;
-    if (!s->cfg.max_group_edges) s->cfg.max_group_edges = 8ull << 20;
+    if (!s->cfg.max_group_edges) s->cfg.max_group_edges = 6ull << 20;
     s->cfg.max_batch = std::min<uint32_t>(s->cfg.max_batch, 1u << 16);
     s->cfg.max_inflight = std::min<uint32_t>(s->cfg.max_inflight, 64);
     int prev = 0;

@@ -91,7 +91,7 @@ def khop_request(seeds, fanout, instance):
 class Session:
     def __init__(self, graph, memory_budget=4 << 30, max_batch=512, max_inflight=0, seed=42,
                  max_group_edges=0):
-        """max_inflight / max_group_edges = 0 take the library defaults (2 groups, 8M edges)."""
+        """max_inflight / max_group_edges = 0 take the library defaults (2 groups, 6M edges)."""
         self.graph = graph
         cfg = Config(memory_budget, max_batch, max_inflight, seed, max_group_edges)
         h = C.c_void_p()

@@ -352,7 +352,7 @@ typedef struct {
     uint32_t max_batch;        /* requests per launch group (0 = 512) */
     uint32_t max_inflight;     /* launch groups on the device at once (0 = 2) */
     uint64_t seed;
-    uint64_t max_group_edges;  /* edge bound that closes a group early (0 = 8M) */
+    uint64_t max_group_edges;  /* edge bound that closes a group early (0 = 6M) */
 } grass_session_config;
 
 typedef struct {
