@@ -197,21 +197,31 @@ __host__ __device__ __forceinline__ uint64_t payload_bits(int fmt, const Hdr& h)
     return (uint64_t)h.C * h.f1 + (uint64_t)(h.d - h.C) * h.f2;
 }
 
+// floor(n / d) for 1 <= d < 2^31 (a chunk or block length, at most a degree): a float reciprocal
+// estimate corrected by one step.  While n < 2^20 the estimate's relative error (rounding of d,
+// the reciprocal, the product: under 2^-22) is under 0.25 in absolute terms, so the truncated
+// estimate is the quotient or one off it and one signed-remainder step fixes it; larger numerators
+// take the integer division.  A per-lane division whose divisor differs from lane to lane, at a
+// fraction of the integer sequence's cost.
+__device__ __forceinline__ uint32_t udiv_small(uint32_t n, uint32_t d) {
+    if (n >= (1u << 20)) return n / d;
+    uint32_t q = (uint32_t)((float)n * __builtin_amdgcn_rcpf((float)d));
+    const int32_t r = (int32_t)(n - q * d);
+    if (r < 0) --q;
+    else if (r >= (int32_t)d) ++q;
+    return q;
+}
+
 // Position k of an LE record -> (chunk c, index i in chunk, chunk length len).
 __device__ __forceinline__ void le_locate(const Hdr& h, uint32_t k, uint32_t& c, uint32_t& i,
                                           uint32_t& len) {
-    uint32_t big = h.R * (h.q + 1);
-    if (k < big) {
-        len = h.q + 1;
-        c = k / len;
-        i = k - c * len;
-    } else {
-        len = h.q;
-        uint32_t kk = k - big;
-        c = kk / len;
-        i = kk - c * len;
-        c += h.R;
-    }
+    const uint32_t big = h.R * (h.q + 1);
+    const bool lo = k < big;
+    len = lo ? h.q + 1 : h.q;
+    const uint32_t kk = lo ? k : k - big;
+    c = udiv_small(kk, len);
+    i = kk - c * len;
+    if (!lo) c += h.R;
 }
 
 // Neighbour k of vertex v, LE (LE/src/bcg.cuh:196-261); 0 <= k < d.
@@ -239,21 +249,18 @@ __device__ __forceinline__ uint32_t le_get(const uint32_t* __restrict__ w, const
     uint32_t x = read_bits(w, so, h.f2 + 1);
     uint32_t mag = x >> 1;
     if (fbits) *fbits = 2 * rb + h.f2 + 1;
-    return (z - a) / len * i + a + ((x & 1) ? 0u - mag : mag);
+    return udiv_small(z - a, len) * i + a + ((x & 1) ? 0u - mag : mag);
 }
 
 // Position k of a GT record -> (block, index i in block).
 __device__ __forceinline__ void gt_locate(const Hdr& h, uint32_t k, uint32_t& blk, uint32_t& i) {
-    uint32_t big = (h.q + 1) * h.R;
-    if (k < big) {
-        blk = k / (h.q + 1);
-        i = k - blk * (h.q + 1);
-    } else {
-        uint32_t kk = k - big;
-        uint32_t b2 = kk / h.q;
-        i = kk - b2 * h.q;
-        blk = b2 + h.R;
-    }
+    const uint32_t big = (h.q + 1) * h.R;
+    const bool lo = k < big;
+    const uint32_t len = lo ? h.q + 1 : h.q;
+    const uint32_t kk = lo ? k : k - big;
+    const uint32_t b = udiv_small(kk, len);
+    i = kk - b * len;
+    blk = lo ? b : b + h.R;
 }
 
 // Bit where block blk of a GT record starts in .graph (GT/src/bcg.cuh:161).

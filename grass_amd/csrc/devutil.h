@@ -42,6 +42,26 @@ __device__ __forceinline__ T wave_inclusive_sum(T x) {
     return x;
 }
 
+// A lane's value moved by a DPP control (0 where the control has no source lane or the row/bank
+// mask leaves the lane out).
+template <int CTRL, int ROW_MASK = 0xf, int BANK_MASK = 0xf>
+__device__ __forceinline__ uint32_t dpp_move(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROW_MASK, BANK_MASK, true);
+}
+
+// 32-bit inclusive sum with DPP moves (VALU, no LDS round trips): a scan within each 16-lane row
+// (row_shr 1, 2, 4, 8), then row 0's total into row 1 and row 2's into row 3 (row_bcast:15), then
+// rows 0-1's total into rows 2 and 3 (row_bcast:31).
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t x) {
+    x += dpp_move<0x111>(x);
+    x += dpp_move<0x112>(x);
+    x += dpp_move<0x114>(x);
+    x += dpp_move<0x118>(x);
+    x += dpp_move<0x142, 0xa>(x);
+    x += dpp_move<0x143, 0xc>(x);
+    return x;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T x) {
 #pragma unroll
