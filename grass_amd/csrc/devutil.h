@@ -24,6 +24,29 @@ __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int src) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Lane `src` of a per-lane value read (wave-uniform) / replaced by a uniform value; the other lanes
+// keep theirs.  One v_readlane / v_writelane per 32-bit half.
+__device__ __forceinline__ uint32_t lane_get(uint32_t v, int src) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, src);
+}
+__device__ __forceinline__ uint64_t lane_get(uint64_t v, int src) { return readlane_u64(v, src); }
+// (clang has no writelane builtin.  The lane select goes through M0: on this target a VOP3 reads at
+// most one SGPR besides M0.  M0 is written by a scalar move, so no VALU wait states are needed;
+// the s_nop is a margin.)
+__device__ __forceinline__ uint32_t lane_set(uint32_t v, int src, uint32_t x) {
+    asm("s_mov_b32 m0, %2\n\ts_nop 0\n\tv_writelane_b32 %0, %1, m0" : "+v"(v) : "s"(x), "s"(src) : "m0");
+    return v;
+}
+__device__ __forceinline__ uint64_t lane_set(uint64_t v, int src, uint64_t x) {
+    return ((uint64_t)lane_set((uint32_t)(v >> 32), src, (uint32_t)(x >> 32)) << 32) |
+           lane_set((uint32_t)v, src, (uint32_t)x);
+}
+
+// Lanes below this one among the set bits of a 64-bit lane mask.
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long lanes) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(lanes >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)lanes, 0u));
+}
+
 // Orders one wave's LDS accesses across lanes (a wave-scope fence around a wave barrier).
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -60,6 +83,26 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t x) {
     x += dpp_move<0x142, 0xa>(x);
     x += dpp_move<0x143, 0xc>(x);
     return x;
+}
+
+// OR of a 64-bit value over the wave, returned uniform: the same DPP tree as the scan above, per
+// 32-bit half, leaves the OR of lanes 0..63 in lane 63.
+__device__ __forceinline__ unsigned long long wave_or_u64(unsigned long long v) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    lo |= dpp_move<0x111>(lo);
+    hi |= dpp_move<0x111>(hi);
+    lo |= dpp_move<0x112>(lo);
+    hi |= dpp_move<0x112>(hi);
+    lo |= dpp_move<0x114>(lo);
+    hi |= dpp_move<0x114>(hi);
+    lo |= dpp_move<0x118>(lo);
+    hi |= dpp_move<0x118>(hi);
+    lo |= dpp_move<0x142, 0xa>(lo);
+    hi |= dpp_move<0x142, 0xa>(hi);
+    lo |= dpp_move<0x143, 0xc>(lo);
+    hi |= dpp_move<0x143, 0xc>(hi);
+    return ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)hi, kWave - 1) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)lo, kWave - 1);
 }
 
 template <typename T>

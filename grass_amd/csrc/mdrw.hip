@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <memory>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "devutil.h"
@@ -372,14 +373,16 @@ struct UChunkGen {  // chunks of the list of dense vertex d
     }
 };
 
-// xpair[pbase[d] + r] = p for pair p = (instance i, vertex x), d = dmap[x], r = rank of i among
-// the slice's instances holding x: from a row's bits to its pairs (and so to their output slots).
+// xpair[pbase[d] + r] = coff[p] for pair p = (instance i, vertex x), d = dmap[x], r = rank of i
+// among the slice's instances holding x: from a row's bits to its pairs' first chunk slots (and so
+// to their output positions).
 __global__ __launch_bounds__(256) void mdrw_xpair_kernel(const uint32_t* __restrict__ verts,
                                                          const uint64_t* __restrict__ voff, uint32_t i0,
                                                          uint32_t i1, uint32_t W, const uint32_t* __restrict__ dmap,
                                                          const unsigned long long* __restrict__ rows,
                                                          const uint64_t* __restrict__ pbase,
-                                                         uint32_t* __restrict__ xpair) {
+                                                         const uint64_t* __restrict__ coff,
+                                                         uint64_t* __restrict__ xpair) {
     const uint64_t p0 = voff[i0], p1 = voff[i1];
     for (uint64_t p = p0 + (uint64_t)blockIdx.x * 256 + threadIdx.x; p < p1; p += (uint64_t)gridDim.x * 256) {
         const uint32_t l = inst_of(voff, i0, i1, p) - i0;
@@ -387,7 +390,7 @@ __global__ __launch_bounds__(256) void mdrw_xpair_kernel(const uint32_t* __restr
         uint64_t r = 0;
         for (uint32_t w = 0; w < (l >> 6); ++w) r += __popcll(rows[d * W + w]);
         r += __popcll(rows[d * W + (l >> 6)] & ((1ull << (l & 63)) - 1));
-        xpair[pbase[d] + r] = (uint32_t)p;
+        xpair[pbase[d] + r] = coff[p];
     }
 }
 
@@ -414,7 +417,9 @@ __global__ __launch_bounds__(256) void mdrw_uexpand_kernel(const uint64_t* __res
 // COUNT: each lane adds its own entry's matches to per-wave LDS counters (the order of the adds
 // does not matter).  WRITE: lane l keeps the next output slot of instance 64 w + l of each mask
 // word w in registers, and per matched instance one ballot ranks the wave's hits in list order.
-template <int FMT, bool WRITE>
+// MODE 0 counts; MODE 1 writes with 32-bit output slots (fewer than 2^32 induced edges), MODE 2
+// with 64-bit ones.
+template <int FMT, int MODE>
 __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, const unsigned long long* __restrict__ rows,
                                                                  uint32_t W, const ulonglong2* __restrict__ dbits,
                                                                  const uint32_t* __restrict__ dvert,
@@ -422,11 +427,12 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
                                                                  const uint32_t* __restrict__ ux,
                                                                  const uint32_t* __restrict__ uk,
                                                                  const uint64_t* __restrict__ pbase,
-                                                                 const uint32_t* __restrict__ xpair,
-                                                                 const uint64_t* __restrict__ coff,
+                                                                 const uint64_t* __restrict__ xpair,
                                                                  uint64_t* __restrict__ ccnt,
                                                                  const uint64_t* __restrict__ cpos,
                                                                  uint32_t* __restrict__ idst) {
+    constexpr bool WRITE = MODE != 0;
+    using Slot = typename std::conditional<MODE == 2, uint64_t, uint32_t>::type;
     __shared__ uint32_t counts[WRITE ? 1 : kIndWaves][kSlice];  // COUNT: per wave and instance
     __shared__ unsigned long long staged[kIndWaves][kWave * kRowPitch];  // the window's rows
     const int lane = lane_id();
@@ -441,7 +447,7 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
         const uint32_t x = __builtin_amdgcn_readfirstlane(dvert[dx]);
         const uint64_t pb = pbase[dx];
         unsigned long long mx[kSliceWords];
-        uint64_t acc[kSliceWords];  // WRITE, lane l: next output slot of instance 64 w + l
+        Slot acc[kSliceWords];  // WRITE, lane l: next output slot of instance 64 w + l
         uint32_t rbase = 0;
 #pragma unroll
         for (uint32_t w = 0; w < kSliceWords; ++w) {
@@ -449,7 +455,7 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
             mx[w] = readfirstlane_u64(v);
             acc[w] = 0;
             if (WRITE && ((mx[w] >> lane) & 1ull))
-                acc[w] = cpos[coff[xpair[pb + rbase + __popcll(mx[w] & below)]] + kc];
+                acc[w] = (Slot)cpos[xpair[pb + rbase + __popcll(mx[w] & below)] + kc];
             if (!WRITE) cnt[w * 64 + lane] = 0;
             rbase += __popcll(mx[w]);
         }
@@ -467,12 +473,17 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
             gt_locate(h, kb, blk, i);
             start = kb - (i & ((1u << g.cp_shift) - 1));
         }
+        // Window pipeline: the next window's entries are decoded and their rows requested before
+        // the current window is counted or written, so the row loads (and, in WRITE, the stores
+        // of the current window) are in flight together rather than one after the other.
         uint32_t carry = 0;
-        for (uint32_t base = start; base < ke; base += kWave) {
+        // entry base + lane of the chunk -> its value y and its row dy (kNoRow when no sample of the
+        // slice holds y, or the entry is not this chunk's)
+        auto decode_window = [&](uint32_t base, uint32_t& y, uint32_t& dy) {
             const uint32_t k = base + lane;
             const bool act = k < ke;
             int head;
-            uint32_t y = element_uniform<FMT>(g, h, x, act ? k : ke - 1, d1, d0, head);
+            y = element_uniform<FMT>(g, h, x, act ? k : ke - 1, d1, d0, head);
             if (FMT == kGT) {
                 if (lane == 0 && !head) {
                     y += carry;
@@ -481,28 +492,53 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
                 y = wave_segmented_sum(y, head);
                 carry = __shfl(y, kWave - 1, kWave);
             }
-            // row of y: none when no sample of the slice holds y (or the entry is not this chunk's)
-            uint32_t dy = kNoRow;
+            dy = kNoRow;
             if (act && k >= kb && y < g.n) {
                 const ulonglong2 bw = dbits[y >> 6];
                 const unsigned long long bit = 1ull << (y & 63);
                 if (bw.x & bit) dy = (uint32_t)bw.y + (uint32_t)__popcll(bw.x & (bit - 1));
             }
-            // Stage the 64 rows in LDS.  With W >= 2 the W / 2 lanes of a row each load 16 bytes
-            // of it, so a row is one memory access instead of one per mask word.
+        };
+        // The 64 rows of a window.  With W >= 2 the W / 2 lanes of a row each load 16 bytes of it,
+        // so a row is one memory access instead of one per mask word; the loads land in registers
+        // (fetch_rows) and go to LDS in the per-lane row layout later (stage_rows).
+        const uint32_t cpr = W >> 1, per = W >= 2 ? kWave / cpr : kWave;  // chunks per row, rows per round
+        ulonglong2 rv[kSliceWords / 2];
+        auto fetch_rows = [&](uint32_t dy) {
             if (W == 1) {
-                R[lane * kRowPitch] = dy != kNoRow ? rows[dy] : 0ull;
-            } else {
-                const uint32_t cpr = W >> 1, per = kWave / cpr;  // chunks per row, rows per round
-                for (uint32_t r = 0; r < cpr; ++r) {
-                    const uint32_t e = r * per + lane / cpr, q = lane % cpr;
-                    const uint32_t de = __shfl(dy, (int)e, kWave);
-                    ulonglong2 v = make_ulonglong2(0ull, 0ull);
-                    if (de != kNoRow) v = *(const ulonglong2*)(rows + (uint64_t)de * W + 2 * q);
-                    R[e * kRowPitch + 2 * q] = v.x;
-                    R[e * kRowPitch + 2 * q + 1] = v.y;
-                }
+                rv[0].x = dy != kNoRow ? rows[dy] : 0ull;
+                return;
             }
+#pragma unroll
+            for (uint32_t r = 0; r < kSliceWords / 2; ++r) {
+                if (r >= cpr) break;
+                const uint32_t e = r * per + lane / cpr, q = lane % cpr;
+                const uint32_t de = __shfl(dy, (int)e, kWave);
+                rv[r] = make_ulonglong2(0ull, 0ull);
+                if (de != kNoRow) rv[r] = *(const ulonglong2*)(rows + (uint64_t)de * W + 2 * q);
+            }
+        };
+        auto stage_rows = [&]() {
+            if (W == 1) {
+                R[lane * kRowPitch] = rv[0].x;
+                return;
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < kSliceWords / 2; ++r) {
+                if (r >= cpr) break;
+                const uint32_t e = r * per + lane / cpr, q = lane % cpr;
+                R[e * kRowPitch + 2 * q] = rv[r].x;
+                R[e * kRowPitch + 2 * q + 1] = rv[r].y;
+            }
+        };
+        uint32_t y, dy;
+        decode_window(start, y, dy);
+        fetch_rows(dy);
+        stage_rows();
+        for (uint32_t base = start; base < ke; base += kWave) {
+            const bool more = base + kWave < ke;  // uniform
+            uint32_t yn = 0;
+            if (more) decode_window(base + kWave, yn, dy);
             wave_sync();
 #pragma unroll
             for (uint32_t w = 0; w < kSliceWords; ++w) {
@@ -513,21 +549,33 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
                         atomicAdd(&cnt[w * 64 + __builtin_ctzll(b2)], 1u);
                     continue;
                 }
-                unsigned long long bits = mx[w];
-                while (bits) {  // uniform loop over x's instances in this word
-                    const int b = __builtin_ctzll(bits);
-                    bits &= bits - 1;
-                    const bool hit = (m >> b) & 1ull;
-                    const unsigned long long lanes = __ballot(hit);
-                    if (!lanes) continue;
-                    if (WRITE) {  // the sources are filled per pair by mdrw_src_kernel
-                        const uint64_t slot = readlane_u64(acc[w], b);
-                        if (hit) idst[slot + __popcll(lanes & below)] = y;
+                // Uniform loop over the instances with at least one hit in the window (a third of
+                // x's instances in the window have none), by 32-bit halves of the word: per
+                // instance one ballot, one masked store of its run (the sources are filled per pair
+                // by mdrw_src_kernel) and one writelane that moves its slot on.
+                const unsigned long long bits = wave_or_u64(m);
+#pragma unroll
+                for (int half = 0; half < 2; ++half) {
+                    uint32_t hb = (uint32_t)(bits >> (32 * half));
+                    const uint32_t mh = (uint32_t)(m >> (32 * half));
+                    while (hb) {
+                        const int b = __builtin_ctz(hb);
+                        hb &= hb - 1;
+                        const int src = b + 32 * half;
+                        const bool hit = (mh >> b) & 1u;
+                        const unsigned long long lanes = __ballot(hit);
+                        const Slot slot = lane_get(acc[w], src);
+                        if (hit) idst[(uint64_t)slot + lanes_below(lanes)] = y;
+                        acc[w] = lane_set(acc[w], src, (Slot)(slot + (Slot)__popcll(lanes)));
                     }
-                    if (lane == b) acc[w] += __popcll(lanes);
                 }
             }
-            wave_sync();  // the rows are restaged for the next window
+            wave_sync();  // every lane is done with the rows: the next window's replace them
+            if (more) {
+                fetch_rows(dy);
+                stage_rows();
+            }
+            y = yn;
         }
         if (!WRITE) {
             wave_sync();
@@ -535,7 +583,7 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
 #pragma unroll
             for (uint32_t w = 0; w < kSliceWords; ++w) {
                 if ((mx[w] >> lane) & 1ull)
-                    ccnt[coff[xpair[pb + rbase + __popcll(mx[w] & below)]] + kc] = cnt[w * 64 + lane];
+                    ccnt[xpair[pb + rbase + __popcll(mx[w] & below)] + kc] = cnt[w * 64 + lane];
                 rbase += __popcll(mx[w]);
             }
             wave_sync();  // the counters are cleared for the next chunk
@@ -544,17 +592,28 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
 }
 
 // The induced edges of pair p = (instance, x) fill [cpos[coff[p]], cpos[coff[p + 1]]) and all
-// have source x: one wave per pair writes them with coalesced stores.
+// have source x.  A workgroup takes a tile of 256 consecutive pairs, loads their bounds and ids
+// with one coalesced pass (instead of a dependent pair of random loads per pair), then its waves
+// fill the tile's pairs one after another with coalesced stores.
 __global__ __launch_bounds__(256) void mdrw_src_kernel(const uint32_t* __restrict__ verts,
                                                        const uint64_t* __restrict__ coff,
                                                        const uint64_t* __restrict__ cpos, uint64_t NV,
                                                        uint32_t* __restrict__ isrc) {
-    const int lane = lane_id();
-    const uint64_t waves = (uint64_t)gridDim.x * (256 / kWave);
-    for (uint64_t p = ((uint64_t)blockIdx.x * 256 + threadIdx.x) / kWave; p < NV; p += waves) {
-        const uint32_t x = verts[p];
-        const uint64_t b = cpos[coff[p]], e = cpos[coff[p + 1]];
-        for (uint64_t i = b + lane; i < e; i += kWave) isrc[i] = x;
+    __shared__ uint64_t bnd[257];
+    __shared__ uint32_t xs[256];
+    const int lane = lane_id(), wid = threadIdx.x / kWave;
+    for (uint64_t t0 = (uint64_t)blockIdx.x * 256; t0 < NV; t0 += (uint64_t)gridDim.x * 256) {
+        const uint32_t np = (uint32_t)min<uint64_t>(256, NV - t0);
+        if (threadIdx.x < np) bnd[threadIdx.x] = cpos[coff[t0 + threadIdx.x]];
+        if (threadIdx.x == 0) bnd[np] = cpos[coff[t0 + np]];
+        if (threadIdx.x < np) xs[threadIdx.x] = verts[t0 + threadIdx.x];
+        __syncthreads();
+        for (uint32_t j = wid; j < np; j += 256 / kWave) {
+            const uint32_t x = xs[j];
+            const uint64_t e = bnd[j + 1];
+            for (uint64_t i = bnd[j] + lane; i < e; i += kWave) isrc[i] = x;
+        }
+        __syncthreads();  // the next tile's bounds replace these
     }
 }
 
@@ -770,7 +829,8 @@ static int mdrw_run(grass_graph* g, MdrwState& st, hipStream_t s, bool replay) {
     unsigned long long* d_rows = nullptr;
     uint64_t *d_did = nullptr, *d_pbase = nullptr, *d_uoff = nullptr, *d_nu = nullptr, *d_partN = nullptr,
              *d_ccnt = nullptr, *d_cpos = nullptr, *d_part3 = nullptr;
-    uint32_t *d_dmap = nullptr, *d_dvert = nullptr, *d_xpair = nullptr, *d_ux = nullptr, *d_uk = nullptr;
+    uint32_t *d_dmap = nullptr, *d_dvert = nullptr, *d_ux = nullptr, *d_uk = nullptr;
+    uint64_t* d_xpair = nullptr;
     ulonglong2* d_dbits = nullptr;
     auto plan3 = [&](Carve& c) {
         d_rows = c.take<unsigned long long>(n * W);  // the slice's distinct vertices use a prefix
@@ -782,7 +842,7 @@ static int mdrw_run(grass_graph* g, MdrwState& st, hipStream_t s, bool replay) {
         d_dmap = c.take<uint32_t>(n);
         d_dbits = c.take<ulonglong2>((n + kWave - 1) / kWave);
         d_dvert = c.take<uint32_t>(n);
-        d_xpair = c.take<uint32_t>(NV);
+        d_xpair = c.take<uint64_t>(NV);
         d_ux = c.take<uint32_t>(NC);
         d_uk = c.take<uint32_t>(NC);
         d_ccnt = c.take<uint64_t>(NC + 1);
@@ -822,36 +882,29 @@ static int mdrw_run(grass_graph* g, MdrwState& st, hipStream_t s, bool replay) {
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(mdrw_xpair_kernel, dim3(sgrid), dim3(256), 0, s, verts, voff, i0, i1, W,
                            (const uint32_t*)d_dmap, (const unsigned long long*)d_rows, (const uint64_t*)d_pbase,
-                           d_xpair);
+                           (const uint64_t*)d_coff, d_xpair);
         hipLaunchKernelGGL(mdrw_uexpand_kernel, dim3(sgrid), dim3(256), 0, s, (const uint64_t*)d_uoff, d_nd,
                            d_ux, d_uk);
         return hipGetLastError();
     };
-    auto induced = [&](bool write, uint32_t* idst) {
-        const unsigned long long* rows = d_rows;
-        const ulonglong2* dbits = d_dbits;
-        const uint32_t* dvert = d_dvert;
-        if (le && !write)
-            hipLaunchKernelGGL((mdrw_induced_kernel<kLE, false>), dim3(sgrid), dim3(kIndBlock), 0, s, dg, rows, W,
-                               dbits, dvert, (const uint64_t*)d_nu, d_ux, d_uk, d_pbase, d_xpair, d_coff, d_ccnt,
-                               d_cpos, idst);
-        else if (le)
-            hipLaunchKernelGGL((mdrw_induced_kernel<kLE, true>), dim3(sgrid), dim3(kIndBlock), 0, s, dg, rows, W,
-                               dbits, dvert, (const uint64_t*)d_nu, d_ux, d_uk, d_pbase, d_xpair, d_coff, d_ccnt,
-                               d_cpos, idst);
-        else if (!write)
-            hipLaunchKernelGGL((mdrw_induced_kernel<kGT, false>), dim3(sgrid), dim3(kIndBlock), 0, s, dg, rows, W,
-                               dbits, dvert, (const uint64_t*)d_nu, d_ux, d_uk, d_pbase, d_xpair, d_coff, d_ccnt,
-                               d_cpos, idst);
+    // mode 0 counts, 1 writes with 32-bit output slots, 2 with 64-bit ones
+    auto induced = [&](int mode, uint32_t* idst) {
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(sgrid), dim3(kIndBlock), 0, s, dg, (const unsigned long long*)d_rows, W,
+                               (const ulonglong2*)d_dbits, (const uint32_t*)d_dvert, (const uint64_t*)d_nu,
+                               (const uint32_t*)d_ux, (const uint32_t*)d_uk, (const uint64_t*)d_pbase,
+                               (const uint64_t*)d_xpair, d_ccnt, (const uint64_t*)d_cpos,
+                               idst);
+        };
+        if (le)
+            go(mode == 0 ? mdrw_induced_kernel<kLE, 0> : mode == 1 ? mdrw_induced_kernel<kLE, 1> : mdrw_induced_kernel<kLE, 2>);
         else
-            hipLaunchKernelGGL((mdrw_induced_kernel<kGT, true>), dim3(sgrid), dim3(kIndBlock), 0, s, dg, rows, W,
-                               dbits, dvert, (const uint64_t*)d_nu, d_ux, d_uk, d_pbase, d_xpair, d_coff, d_ccnt,
-                               d_cpos, idst);
+            go(mode == 0 ? mdrw_induced_kernel<kGT, 0> : mode == 1 ? mdrw_induced_kernel<kGT, 1> : mdrw_induced_kernel<kGT, 2>);
         return hipGetLastError();
     };
     for (uint32_t i0 = 0; i0 < I; i0 += kSlice) {
         e = prep(i0, std::min<uint32_t>(I, i0 + kSlice));
-        if (e == hipSuccess) e = induced(false, nullptr);
+        if (e == hipSuccess) e = induced(0, nullptr);
         if (e != hipSuccess) return status_of(e);
     }
     e = exclusive_scan(U64Gen{d_ccnt}, Count{NC, nullptr}, NC, d_cpos, d_part3, d_part3 + nbC + 1, s);
@@ -870,7 +923,7 @@ static int mdrw_run(grass_graph* g, MdrwState& st, hipStream_t s, bool replay) {
     for (uint32_t i0 = 0; i0 < I; i0 += kSlice) {
         // a single slice keeps its masks and ranks from the count pass
         e = I > kSlice ? prep(i0, std::min<uint32_t>(I, i0 + kSlice)) : hipSuccess;
-        if (e == hipSuccess) e = induced(true, o_idst);
+        if (e == hipSuccess) e = induced(NI >> 32 ? 2 : 1, o_idst);
         if (e != hipSuccess) return status_of(e);
     }
     hipLaunchKernelGGL(mdrw_src_kernel, dim3(sgrid), dim3(256), 0, s, (const uint32_t*)o_verts,
