@@ -620,7 +620,7 @@ def main():
             grass_amd._check(L.grass_decode_csr(g.handle, grass_amd.C.c_void_p(rowptr.data_ptr()),
                                                 grass_amd.C.c_void_p(col.data_ptr()),
                                                 grass_amd.C.c_void_p(sptr)), "decode")
-        _, ev_d = timed(dec, 3, 1, events=True)
+        _, ev_d = timed(dec, 5, 3, events=True)
         dms = float(np.mean(ev_d))
         dbytes = len(gb) + len(ob) + 8 * (g.n + 1) + 4 * g.m
         extra["decode"] = {"ms": round(dms, 3), "decoded_bytes_per_s": (len(gb) + len(ob)) / (dms * 1e-3),

@@ -104,6 +104,8 @@ void free_graph(grass_graph* g) {
     if (g->d_err) (void)hipFree(g->d_err);
     g->ws.release();
     g->ws_bytes.release();
+    g->dec_part.release();
+    if (g->dec_ev) (void)hipEventDestroy(g->dec_ev);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
 }

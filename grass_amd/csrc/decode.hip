@@ -77,7 +77,8 @@ struct GtList {
     }
     // Element k: the absolute id at a block head or checkpoint (head = 1), else the gap + 1.  The
     // field's bit, width and kind are chosen first, so every lane issues the same one load.
-    __device__ __forceinline__ uint32_t at(const uint32_t* __restrict__ w, uint32_t k, int& head) const {
+    template <class Src>
+    __device__ __forceinline__ uint32_t at(const Src& w, uint32_t k, int& head) const {
         uint32_t blk, i;
         uint64_t bo;
         if (k < big) {
@@ -93,27 +94,53 @@ struct GtList {
         const uint32_t j = i - 1;
         const bool hd = i == 0, abs = !hd && (i & ((1u << sh) - 1)) == 0;
         const uint32_t off = hd ? 0u : f1 + j * f2 + (j >> sh) * (32 - f2);
-        const uint32_t raw = read_bits(w, bo + off, hd ? f1 : abs ? 32u : f2);
+        const uint32_t raw = w.read(bo + off, hd ? f1 : abs ? 32u : f2);
         head = hd || abs;
         return hd ? (blk >= fp ? v + raw + 1 : v - raw - 1) : abs ? raw : raw + 1;
     }
 };
 
+// Where the GT payload words come from: the stream in global memory, or a wave's LDS copy of the
+// span of the stream its range of slots covers (L[0] is stream word W0).
+struct GlobalWords {
+    const uint32_t* __restrict__ w;
+    __device__ __forceinline__ uint32_t read(uint64_t b, uint32_t nbits) const { return read_bits(w, b, nbits); }
+};
+struct LdsWords {
+    const uint32_t* L;
+    uint64_t W0;
+    __device__ __forceinline__ uint32_t read(uint64_t b, uint32_t nbits) const {
+        const uint32_t o = (uint32_t)((b >> 5) - W0);
+        const uint64_t x = ((uint64_t)L[o] << 32) | L[o + 1];
+        return (uint32_t)(x >> (32 - (uint32_t)(b & 31))) >> (32 - nbits);
+    }
+};
+
+// gt_element (codec.h) on a word source.
+template <class Src>
+__device__ __forceinline__ uint32_t gt_element_src(const Src& w, const Hdr& h, uint32_t v, uint32_t blk, uint32_t i,
+                                                   uint32_t sh, int& head) {
+    const uint64_t bo = gtx_block_bit(h, blk, sh);
+    const bool hd = i == 0, abs = !hd && (i & ((1u << sh) - 1)) == 0;
+    const uint32_t raw = w.read(bo + gtx_field(h, i, sh), hd ? h.f1 : abs ? 32u : h.f2);
+    head = hd || abs;
+    return hd ? (blk >= h.fp ? v + raw + 1 : v - raw - 1) : abs ? raw : raw + 1;
+}
+
 // Segmented inclusive sum over runs of at most `span` lanes (a lane with head != 0 starts a
 // run): log2(span) shuffle steps instead of six.
 __device__ __forceinline__ uint32_t short_segmented_sum(uint32_t x, int head, uint32_t span) {
     const int lane = lane_id();
-    if (span == 4) {  // the default checkpoint interval: two fixed steps
-#pragma unroll
-        for (int off = 1; off < 4; off <<= 1) {
-            const uint32_t px = __shfl_up(x, off, kWave);
-            const int ph = __shfl_up(head, off, kWave);
-            if (lane >= off && !head) {
-                x += px;
-                head = ph;
-            }
-        }
-        return x;
+    if (span == 4) {
+        // the default checkpoint interval: runs of at most four lanes, so a lane adds the values
+        // of the (up to three) lanes before it back to its run's head, moved by wave shifts of
+        // one lane (DPP, no LDS); lane 0 always starts a run here
+        const uint32_t hf = head ? 1u : 0u;
+        const uint32_t v1 = dpp_move<0x138>(x), h1 = dpp_move<0x138>(hf);
+        const uint32_t v2 = dpp_move<0x138>(v1), h2 = dpp_move<0x138>(h1);
+        const uint32_t v3 = dpp_move<0x138>(v2);
+        const bool a1 = !hf, a2 = a1 && !h1, a3 = a2 && !h2;
+        return x + (a1 ? v1 : 0u) + (a2 ? v2 : 0u) + (a3 ? v3 : 0u);
     }
     for (uint32_t off = 1; off < span && off < (uint32_t)kWave; off <<= 1) {
         const uint32_t px = __shfl_up(x, off, kWave);
@@ -126,30 +153,13 @@ __device__ __forceinline__ uint32_t short_segmented_sum(uint32_t x, int head, ui
     return x;
 }
 
-template <int FMT>
-__global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
-                                                               const uint64_t* __restrict__ rowptr,
-                                                               const uint32_t* __restrict__ wstart,
-                                                               uint32_t* __restrict__ col,
-                                                               uint64_t m, uint32_t per_wave) {
-    const uint64_t wave = ((uint64_t)blockIdx.x * kDecBlock + threadIdx.x) / kWave;
+// The slots [e0, e1) of one wave: vlo owns e0, vhi bounds the owner of e1 - 1, decoding starts
+// at eb <= e0 (a run's absolute value); GT payload words come from `src`.
+template <int FMT, class Src>
+__device__ __forceinline__ void decode_range(const DevGraph& g, const Src& src, const uint64_t* __restrict__ rowptr,
+                                             uint32_t* __restrict__ col, uint64_t e0, uint64_t e1, uint64_t eb,
+                                             uint32_t vlo, uint32_t vhi, uint32_t span) {
     const int lane = lane_id();
-    const uint64_t e0 = wave * per_wave;
-    if (e0 >= m) return;  // uniform per wave
-    const uint64_t e1 = e0 + per_wave < m ? e0 + per_wave : m;
-    // owner of e0, and a bound on the owner of e1 - 1 (the next wave's first owner)
-    const uint32_t vlo = wstart[wave];
-    const uint32_t vhi = e1 < m ? wstart[wave + 1] : g.n - 1;
-    // GT: an absolute value (block head or checkpoint) starts every run of at most P positions
-    const uint32_t span = 1u << g.cp_shift;
-    uint64_t eb = e0;
-    if (FMT == kGT) {
-        Hdr h;
-        load_hdr<kGT>(g, vlo, h);
-        uint32_t blk, i;
-        gt_locate(h, (uint32_t)(e0 - rowptr[vlo]), blk, i);
-        eb = e0 - (i & (span - 1));  // back to the block head or checkpoint
-    }
     uint32_t carry = 0;
     // owner of the iteration's first slot (a lower bound for every lane's) and its row bounds,
     // wave-uniform, reloaded only when it changes
@@ -177,15 +187,19 @@ __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
                     d0.init(uh.q);
                 }
             }
-            if (FMT == kGT) x = gl.at(g.w, (uint32_t)(ee - rc), head);
+            if (FMT == kGT) x = gl.at(src, (uint32_t)(ee - rc), head);
             else x = element_uniform<FMT>(g, uh, hv, (uint32_t)(ee - rc), d1, d0, head);
         } else {
-            // Owner of slot ee: one coalesced load of the 64 row ends after `cur`, then a 6-step
-            // search over them in registers; slots more than 64 vertices past `cur` (only
-            // possible across vertices without edges) fall back to a binary search of rowptr.
+            // Owner of slot ee: one coalesced load of the 64 row ends after `cur` (and, beside
+            // it, of those 64 vertices' headers), then a 6-step search over them in registers;
+            // slots more than 64 vertices past `cur` (only possible across vertices without
+            // edges) fall back to a binary search of rowptr.  The owner's header then comes from
+            // the lane that loaded it, and the next window's row bounds from the row ends.
             // (row ends as 32-bit offsets from the start of `cur`'s list, which is at or before
             // every slot of the iteration; one list is < 2^31 entries, larger offsets saturate)
             const uint64_t bj = (uint64_t)cur + 1 + lane <= g.n ? rowptr[cur + 1 + lane] : ~0ull;
+            uint4 hj = make_uint4(0u, 0u, 0u, 0u);
+            if (cur + (uint32_t)lane < g.n) hj = g.hdr[cur + lane];
             const uint32_t rj = bj - rc > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)(bj - rc);
             const uint32_t er = (uint32_t)(ee - rc);
             uint32_t pos = 0;
@@ -197,6 +211,9 @@ __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
             const uint32_t last = __shfl(rj, kWave - 1, kWave);
             if (pos == kWave - 1 && last <= er) pos = kWave;
             const uint32_t prev = __shfl(rj, (int)(pos > 0 ? pos - 1 : 0), kWave);
+            const int src_lane = (int)(pos < (uint32_t)kWave ? pos : 0u);
+            uint4 hv4 = make_uint4(__shfl(hj.x, src_lane, kWave), __shfl(hj.y, src_lane, kWave),
+                                   __shfl(hj.z, src_lane, kWave), __shfl(hj.w, src_lane, kWave));
             uint32_t v;
             uint64_t vstart;
             if (pos < (uint32_t)kWave) {
@@ -205,22 +222,29 @@ __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
             } else {
                 v = owner_of(rowptr, cur + kWave, vhi, ee);
                 vstart = rowptr[v];
+                hv4 = g.hdr[v];
             }
-            const uint32_t nc = __builtin_amdgcn_readfirstlane(__shfl(v, kWave - 1, kWave));
-            if (nc != cur) {
-                cur = nc;
+            const uint32_t p63 = (uint32_t)__builtin_amdgcn_readlane((int)pos, kWave - 1);
+            if (p63 < (uint32_t)kWave) {
+                if (p63) {
+                    cur += p63;
+                    rc = readlane_u64(bj, (int)p63 - 1);
+                    rn = readlane_u64(bj, (int)p63);
+                }
+            } else {
+                cur = __builtin_amdgcn_readlane((int)v, kWave - 1);
                 rc = rowptr[cur];
                 rn = rowptr[cur + 1];
             }
             const uint32_t k = (uint32_t)(ee - vstart);
             Hdr h;
-            load_hdr<FMT>(g, v, h);
+            unpack_hdr<FMT>(hv4, h);
             if (FMT == kLE) {
                 x = le_get(g.w, h, v, k);
             } else {
                 uint32_t blk, i;
                 gt_locate(h, k, blk, i);
-                x = gt_element(g, h, v, blk, i, head);
+                x = gt_element_src(src, h, v, blk, i, g.cp_shift, head);
             }
         }
         if (FMT == kGT) {
@@ -229,24 +253,106 @@ __global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
                 head = 1;
             }
             x = short_segmented_sum(x, head, span);
-            carry = __shfl(x, kWave - 1, kWave);
+            carry = (uint32_t)__builtin_amdgcn_readlane((int)x, kWave - 1);
         }
         if (act && e >= e0) col[e] = x;
     }
+}
+
+
+// LDS words a wave may copy its span into (the rest of a larger span is read from global memory).
+constexpr uint32_t kDecStageMax = 2048;
+__host__ __device__ constexpr uint32_t dec_stage_words(uint32_t per_wave) {
+    return (per_wave < kDecStageMax ? per_wave : kDecStageMax) + 64;
+}
+
+// GT with stage != 0: the wave first copies the span of the payload stream its slots use (every
+// field from its first run's absolute value to slot e1) into LDS with coalesced loads, all in
+// flight together, when it fits `stage` words, and then decodes from LDS: no dependent global
+// load per 64 slots.
+template <int FMT>
+__global__ __launch_bounds__(kDecBlock) void decode_csr_kernel(DevGraph g,
+                                                               const uint64_t* __restrict__ rowptr,
+                                                               const uint32_t* __restrict__ wstart,
+                                                               uint32_t* __restrict__ col,
+                                                               uint64_t m, uint32_t per_wave, uint32_t stage) {
+    extern __shared__ uint32_t dec_lds[];
+    const uint64_t wave = ((uint64_t)blockIdx.x * kDecBlock + threadIdx.x) / kWave;
+    const int lane = lane_id();
+    const uint64_t e0 = wave * per_wave;
+    if (e0 >= m) return;  // uniform per wave
+    const uint64_t e1 = e0 + per_wave < m ? e0 + per_wave : m;
+    // owner of e0, and a bound on the owner of e1 - 1 (the next wave's first owner)
+    const uint32_t vlo = wstart[wave];
+    const uint32_t vhi = e1 < m ? wstart[wave + 1] : g.n - 1;
+    // GT: an absolute value (block head or checkpoint) starts every run of at most P positions
+    const uint32_t span = 1u << g.cp_shift;
+    uint64_t eb = e0;
+    if (FMT == kGT) {
+        const uint32_t sh = g.cp_shift;
+        Hdr h;
+        load_hdr<kGT>(g, vlo, h);
+        uint32_t blk, i;
+        gt_locate(h, (uint32_t)(e0 - rowptr[vlo]), blk, i);
+        const uint32_t ib = i & ~(span - 1);
+        eb = e0 - (i - ib);  // back to the block head or checkpoint
+        if (stage) {
+            // bits [B0, B1): from eb's field to slot e1's (the stream's end for the last wave);
+            // a field read touches its word and the next, so words W0 ..= ((B1 - 1) >> 5) + 1
+            const uint64_t B0 = gtx_block_bit(h, blk, sh) + gtx_field(h, ib, sh);
+            Hdr hu;
+            load_hdr<kGT>(g, vhi, hu);
+            uint64_t B1;
+            if (e1 < m) {
+                uint32_t b1, i1;
+                gt_locate(hu, (uint32_t)(e1 - rowptr[vhi]), b1, i1);
+                B1 = gtx_block_bit(hu, b1, sh) + gtx_field(hu, i1, sh);
+            } else {
+                B1 = hu.pay + gtx_payload_bits(hu, sh);
+            }
+            const uint64_t W0 = B0 >> 5;
+            const uint64_t NW = B1 > B0 ? ((B1 - 1) >> 5) + 2 - W0 : 0;
+            if (NW && NW <= stage) {
+                uint32_t* L = dec_lds + (threadIdx.x / kWave) * stage;
+                const uint32_t nw = (uint32_t)NW;
+                for (uint32_t c = 0; c < nw; c += 8 * kWave) {
+                    uint32_t t[8];
+#pragma unroll
+                    for (uint32_t u = 0; u < 8; ++u) {
+                        const uint32_t j = c + u * kWave + lane;
+                        t[u] = j < nw ? g.w[W0 + j] : 0u;
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < 8; ++u) {
+                        const uint32_t j = c + u * kWave + lane;
+                        if (j < nw) L[j] = t[u];
+                    }
+                }
+                wave_sync();
+                decode_range<FMT>(g, LdsWords{L, W0}, rowptr, col, e0, e1, eb, vlo, vhi, span);
+                return;
+            }
+        }
+    }
+    decode_range<FMT>(g, GlobalWords{g.w}, rowptr, col, e0, e1, eb, vlo, vhi, span);
 }
 
 int decode_csr(grass_graph* g, uint64_t* rowptr, uint32_t* col, hipStream_t s) {
     const uint64_t n = g->n, m = g->m;
     const uint64_t nb = scan_blocks(n);
     {
+        // The scan's partials have a buffer of their own, ordered between calls on the device:
+        // each call's stream waits for the previous call's scan (on whatever stream) before its
+        // own scan reuses the buffer, so no call waits on the host.
         std::lock_guard<std::mutex> lock(g->mu);
-        hipError_t e = g->ws.scratch.reserve((nb + 4) * 8);
-        if (e != hipSuccess) return GRASS_ENOMEM;
-        uint64_t* part = (uint64_t*)g->ws.scratch.ptr;
-        e = exclusive_scan(DegGen{g->d_deg}, Count{n, nullptr}, n, rowptr, part, part + nb, s);
+        if (!g->dec_ev && hipEventCreateWithFlags(&g->dec_ev, hipEventDisableTiming) != hipSuccess)
+            return GRASS_ENOMEM;
+        hipError_t e = hipStreamWaitEvent(s, g->dec_ev, 0);
         if (e != hipSuccess) return status_of(e);
-        // the scratch is reused by the next call on any stream: finish the scan first
-        e = hipStreamSynchronize(s);
+        if (g->dec_part.reserve((nb + 4) * 8) != hipSuccess) return GRASS_ENOMEM;
+        uint64_t* part = (uint64_t*)g->dec_part.ptr;
+        e = exclusive_scan(DegGen{g->d_deg}, Count{n, nullptr}, n, rowptr, part, part + nb, s);
+        if (e == hipSuccess) e = hipEventRecord(g->dec_ev, s);
         if (e != hipSuccess) return status_of(e);
     }
     if (m == 0 || !col) return GRASS_OK;
@@ -278,12 +384,15 @@ int decode_csr(grass_graph* g, uint64_t* rowptr, uint32_t* col, hipStream_t s) {
         }
         wstart = g->d_wstart[slot];
     }
-    if (g->fmt == kLE)
+    if (g->fmt == kLE) {
         hipLaunchKernelGGL(decode_csr_kernel<kLE>, dim3((unsigned)blocks), dim3(kDecBlock), 0, s,
-                           g->dev(), (const uint64_t*)rowptr, wstart, col, m, per_wave);
-    else
-        hipLaunchKernelGGL(decode_csr_kernel<kGT>, dim3((unsigned)blocks), dim3(kDecBlock), 0, s,
-                           g->dev(), (const uint64_t*)rowptr, wstart, col, m, per_wave);
+                           g->dev(), (const uint64_t*)rowptr, wstart, col, m, per_wave, 0u);
+    } else {
+        const uint32_t stage = dec_stage_words(per_wave);
+        hipLaunchKernelGGL(decode_csr_kernel<kGT>, dim3((unsigned)blocks), dim3(kDecBlock),
+                           (size_t)stage * 4 * (kDecBlock / kWave), s, g->dev(), (const uint64_t*)rowptr, wstart,
+                           col, m, per_wave, stage);
+    }
     return status_of(hipGetLastError());
 }
 

@@ -112,7 +112,7 @@ struct TuneState {
     std::atomic<uint32_t> rw_stage{32};              // walk columns staged per LDS flush: 8, 16, 32
     std::atomic<uint32_t> rw_claim{1};               // 64-walker granules per queue access: 1, 2, 4
     std::atomic<uint32_t> khop_lane_scale{1};        // k-hop lanes per vertex = scale x fanout cover
-    std::atomic<uint32_t> decode_edges_per_wave{2048};  // full-decode slots per wave
+    std::atomic<uint32_t> decode_edges_per_wave{1024};  // full-decode slots per wave
 };
 
 // Bump allocator over a Scratch region (256-byte aligned slices).
@@ -157,6 +157,8 @@ struct grass_graph {
     uint32_t* d_err = nullptr;  // [0] first error kind, [1] detail
     // full decode: first vertex of every wave, one table per slots-per-wave value 512 << i
     uint32_t* d_wstart[grass::kWaveStartSlots] = {};
+    grass::Scratch dec_part;     // full decode: the row-pointer scan's partials (under mu)
+    hipEvent_t dec_ev = nullptr;  // recorded after each full decode's scan
     hipStream_t stream = nullptr;
     std::mutex mu;
     grass::Workspace ws;     // working memory of the synchronous entry points (under mu)
