@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <memory>
 #include <new>
 #include <type_traits>
@@ -916,6 +917,10 @@ static int mdrw_run(grass_graph* g, MdrwState& st, hipStream_t s, bool replay) {
     }
     const uint64_t NI = st.NI;
     res->out.induced = NI;
+    // The write pass keeps 32-bit output slots below 2^32 induced edges; GRASS_MDRW_SLOTS64=1
+    // takes the 64-bit variant whatever the count (the tests cover it on small batches that way).
+    const char* s64 = getenv("GRASS_MDRW_SLOTS64");
+    const bool slots64 = s64 && s64[0] == '1';
     uint32_t* o_isrc = res->ind.as<uint32_t>();
     uint32_t* o_idst = o_isrc + NI;
     res->out.ind_src_dev = o_isrc;
@@ -923,7 +928,7 @@ static int mdrw_run(grass_graph* g, MdrwState& st, hipStream_t s, bool replay) {
     for (uint32_t i0 = 0; i0 < I; i0 += kSlice) {
         // a single slice keeps its masks and ranks from the count pass
         e = I > kSlice ? prep(i0, std::min<uint32_t>(I, i0 + kSlice)) : hipSuccess;
-        if (e == hipSuccess) e = induced(NI >> 32 ? 2 : 1, o_idst);
+        if (e == hipSuccess) e = induced(NI >> 32 || slots64 ? 2 : 1, o_idst);
         if (e != hipSuccess) return status_of(e);
     }
     hipLaunchKernelGGL(mdrw_src_kernel, dim3(sgrid), dim3(256), 0, s, (const uint32_t*)o_verts,

@@ -114,3 +114,19 @@ def test_mdrw_rejects_bad_shapes(ga, oracle_mod):
     res = g.mdrw([], target=100, seed=1)
     assert len(res["verts"]) == 0 and list(res["step_off"]) == [0]
     g.close()
+
+
+@pytest.mark.parametrize("fmt", FMTS)
+def test_mdrw_64bit_output_slots(ga, oracle_mod, fmt, monkeypatch):
+    """The induced write pass keeps 32-bit output slots below 2^32 induced edges and 64-bit ones
+    above; GRASS_MDRW_SLOTS64=1 selects the 64-bit variant on a small batch, which must give the
+    same lists."""
+    print(f"[mdrw] 64-bit output slots fmt={fmt}", flush=True)
+    monkeypatch.setenv("GRASS_MDRW_SLOTS64", "1")
+    g, og = _pair(ga, oracle_mod, "rmat12", fmt)
+    sizes = [1, 64, 65, 300, 1000, 1500]
+    res = g.mdrw(sizes, target=1500, seed=5)
+    exp = og.mdrw(sizes, target=1500, seed=5)
+    assert int(res["ind_off"][-1]) > 0
+    _same(res, exp)
+    g.close()
