@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "devutil.h"
@@ -86,17 +88,13 @@ struct DedupIn {
     }
 };
 
-__global__ __launch_bounds__(kKhBlock) void dedup_insert_kernel(DedupIn in, Count E,
-                                                                const uint64_t* __restrict__ eoff_r,
-                                                                uint32_t R, HashTab t, uint32_t gen,
-                                                                uint32_t* err) {
-    const uint64_t n = E.get();
-    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-         p += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t v = in.at(p);
-        if (v == kInvalid) continue;
-        uint64_t key48 = ((uint64_t)seg_of(eoff_r, R, p) << 31) | v;
-        unsigned long long full = ((unsigned long long)gen << 48) | key48;
+// The 16-byte generation-tagged table behind the interface the dedup kernels use.
+struct GenTab {
+    HashTab t;
+    uint32_t gen;
+    __device__ __forceinline__ void insert(uint32_t r, uint32_t v, uint64_t p, uint32_t* err) const {
+        const uint64_t key48 = ((uint64_t)r << 31) | v;
+        const unsigned long long full = ((unsigned long long)gen << 48) | key48;
         uint64_t h = ht_home(t, key48);
         uint64_t probes = 0;
         for (;;) {
@@ -110,7 +108,7 @@ __global__ __launch_bounds__(kKhBlock) void dedup_insert_kernel(DedupIn in, Coun
             h = (h + 1) & t.mask;
             if (++probes > t.mask) {  // cannot happen while slots > entries; never spin
                 atomicCAS(err, kErrNone, kErrBadNeighbor);
-                break;
+                return;
             }
         }
         // min of (generation, position): read first, since most repeats come after the first
@@ -118,32 +116,86 @@ __global__ __launch_bounds__(kKhBlock) void dedup_insert_kernel(DedupIn in, Coun
         const unsigned long long mine = ((unsigned long long)(0xFFFFu - gen) << 32) | (uint32_t)p;
         if (__atomic_load_n(t.val(h), __ATOMIC_RELAXED) > mine) atomicMin(t.val(h), mine);
     }
+    __device__ __forceinline__ bool first_is(uint32_t r, uint32_t v, uint64_t p) const {
+        const uint64_t key48 = ((uint64_t)r << 31) | v;
+        const unsigned long long full = ((unsigned long long)gen << 48) | key48;
+        uint64_t h = ht_home(t, key48);
+        for (uint64_t probes = 0; probes <= t.mask; ++probes) {
+            const ulonglong2 kv = *(const ulonglong2*)t.key(h);  // key and val in one load
+            if (kv.x == full) return (uint32_t)kv.y == (uint32_t)p;
+            h = (h + 1) & t.mask;
+        }
+        return false;
+    }
+};
+
+// 8 bytes per slot when the request, the vertex and the position fit one word:
+// ((request << vb | vertex) << pb) | position, all ones = empty (positions < 2^pb - 1).  A CAS
+// claims an empty slot with the key and the position at once; atomicMin on a word of the same
+// key lowers only its position bits.  Half the footprint of the 16-byte table, one atomic per new
+// key, and a pass clears only the slots its own entries need (no generations).
+struct PackTab {
+    unsigned long long* slot;
+    uint64_t mask;
+    uint32_t shift, vb, pb;
+    __device__ __forceinline__ uint64_t home(unsigned long long k) const {
+        return (((k + 1) * 0x9E3779B97F4A7C15ull) >> shift) & mask;
+    }
+    __device__ __forceinline__ void insert(uint32_t r, uint32_t v, uint64_t p, uint32_t* err) const {
+        const unsigned long long k = ((unsigned long long)r << vb) | v;
+        const unsigned long long w = (k << pb) | p;
+        uint64_t h = home(k);
+        for (uint64_t probes = 0; probes <= mask; ++probes) {
+            unsigned long long cur = __atomic_load_n(slot + h, __ATOMIC_RELAXED);
+            if (cur == ~0ull) {
+                cur = atomicCAS(slot + h, ~0ull, w);
+                if (cur == ~0ull) return;
+            }
+            if ((cur >> pb) == k) {
+                if (cur > w) atomicMin(slot + h, w);
+                return;
+            }
+            h = (h + 1) & mask;
+        }
+        atomicCAS(err, kErrNone, kErrBadNeighbor);  // cannot happen: slots > entries
+    }
+    __device__ __forceinline__ bool first_is(uint32_t r, uint32_t v, uint64_t p) const {
+        const unsigned long long k = ((unsigned long long)r << vb) | v;
+        uint64_t h = home(k);
+        for (uint64_t probes = 0; probes <= mask; ++probes) {
+            const unsigned long long cur = slot[h];
+            if (cur == ~0ull) break;
+            if ((cur >> pb) == k) return (cur & ((1ull << pb) - 1)) == p;
+            h = (h + 1) & mask;
+        }
+        return false;
+    }
+};
+
+template <class Tab>
+__global__ __launch_bounds__(kKhBlock) void dedup_insert_kernel(DedupIn in, Count E,
+                                                                const uint64_t* __restrict__ eoff_r,
+                                                                uint32_t R, Tab t, uint32_t* err) {
+    const uint64_t n = E.get();
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t v = in.at(p);
+        if (v == kInvalid) continue;
+        t.insert(seg_of(eoff_r, R, p), v, p, err);
+    }
 }
 
 // flag[p] = 1 when entry p is the first occurrence of its (request, vertex) key.
+template <class Tab>
 __global__ __launch_bounds__(kKhBlock) void dedup_mark_kernel(DedupIn in, Count E,
                                                               const uint64_t* __restrict__ eoff_r,
-                                                              uint32_t R, HashTab t, uint32_t gen,
+                                                              uint32_t R, Tab t,
                                                               uint32_t* __restrict__ flag) {
     const uint64_t n = E.get();
     for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
          p += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t v = in.at(p);
-        uint32_t first = 0;
-        if (v != kInvalid) {
-            uint64_t key48 = ((uint64_t)seg_of(eoff_r, R, p) << 31) | v;
-            unsigned long long full = ((unsigned long long)gen << 48) | key48;
-            uint64_t h = ht_home(t, key48);
-            for (uint64_t probes = 0; probes <= t.mask; ++probes) {
-                const ulonglong2 kv = *(const ulonglong2*)t.key(h);  // key and val in one load
-                if (kv.x == full) {
-                    first = (uint32_t)kv.y == (uint32_t)p;
-                    break;
-                }
-                h = (h + 1) & t.mask;
-            }
-        }
-        flag[p] = first;
+        flag[p] = v != kInvalid && t.first_is(seg_of(eoff_r, R, p), v, p);
     }
 }
 
@@ -469,9 +521,54 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
     uint64_t* d_ftot = d_sc + 3;
     uint64_t* d_hb = d_sc + 8;
 
-    HashTab ht;
-    e = ensure_table(ws, Dmax, ht);
-    if (e != hipSuccess) return GRASS_ENOMEM;
+    // Dedup table: 8-byte packed slots when (request, vertex, position) fit one word, else the
+    // 16-byte generation-tagged table.  GRASS_KHOP_SLOT=16 forces the latter.
+    auto bits = [](uint64_t x) {
+        uint32_t k = 0;
+        while (k < 64 && (x >> k) != 0) ++k;
+        return k;
+    };
+    const uint32_t vb = std::max<uint32_t>(bits(g->n ? g->n - 1 : 0), 1u);
+    const uint32_t pb = std::max<uint32_t>(bits(std::max(S, Dmax)), 1u);  // positions < 2^pb - 1
+    const char* force16 = getenv("GRASS_KHOP_SLOT");
+    const bool packed = bits(R - 1) + vb + pb <= 64 && !(force16 && strcmp(force16, "16") == 0);
+    auto pack_slots = [](uint64_t entries) {  // load <= 2/3
+        uint64_t slots = 1024;
+        while (slots < entries + entries / 2 + 2) slots <<= 1;
+        return slots;
+    };
+    HashTab ht{};
+    if (packed) {
+        if (ws.ht.reserve(pack_slots(std::max(S, Dmax)) * 8) != hipSuccess) return GRASS_ENOMEM;
+        ws.ht_gen = 0xFFFF;  // the 16-byte table's next use starts from a clear
+    } else {
+        e = ensure_table(ws, Dmax, ht);
+        if (e != hipSuccess) return GRASS_ENOMEM;
+    }
+    // the table for `entries` positions: packed (cleared now, only the slots it uses) or the next
+    // generation of the 16-byte one
+    auto pack_tab = [&](uint64_t entries) {
+        const uint64_t slots = pack_slots(entries);
+        const unsigned grid = (unsigned)std::min<uint64_t>((slots + 255) / 256, 4096);
+        hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(grid), dim3(256), 0, s,
+                           (unsigned long long*)ws.ht.ptr, slots, ~0ull);
+        return PackTab{(unsigned long long*)ws.ht.ptr, slots - 1, (uint32_t)(64 - __builtin_ctzll(slots)), vb, pb};
+    };
+    auto dedup = [&](const DedupIn& in, Count E, const uint64_t* off_r, uint64_t entries) {
+        if (packed) {
+            const PackTab t = pack_tab(entries);
+            hipLaunchKernelGGL(dedup_insert_kernel<PackTab>, dim3(2048), dim3(kKhBlock), 0, s, in, E, off_r, R, t,
+                               err_word(g, ws));
+            hipLaunchKernelGGL(dedup_mark_kernel<PackTab>, dim3(2048), dim3(kKhBlock), 0, s, in, E, off_r, R, t,
+                               d_flag);
+        } else {
+            const GenTab t{ht, next_gen(ws, ht, s)};
+            hipLaunchKernelGGL(dedup_insert_kernel<GenTab>, dim3(2048), dim3(kKhBlock), 0, s, in, E, off_r, R, t,
+                               err_word(g, ws));
+            hipLaunchKernelGGL(dedup_mark_kernel<GenTab>, dim3(2048), dim3(kKhBlock), 0, s, in, E, off_r, R, t,
+                               d_flag);
+        }
+    };
 
     e = hipMemcpyAsync(d_req, hreq, R * sizeof(KhReq), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(d_sbeg, sbeg, (R + 1) * 8, hipMemcpyHostToDevice, s);
@@ -486,11 +583,7 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
                               d_seedv, err);
     {
         DedupIn in{d_seedv, nullptr, S};
-        uint32_t gen = next_gen(ws, ht, s);
-        hipLaunchKernelGGL(dedup_insert_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, Count{S, nullptr},
-                           d_sbeg, R, ht, gen, err);
-        hipLaunchKernelGGL(dedup_mark_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, Count{S, nullptr},
-                           d_sbeg, R, ht, gen, d_flag);
+        dedup(in, Count{S, nullptr}, d_sbeg, S);
         e = exclusive_scan(FlagGen{d_flag}, Count{S, nullptr}, Dmax, d_fpos, d_part, d_ftot, s);
         if (e != hipSuccess) return status_of(e);
         hipLaunchKernelGGL(compact_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, Count{S, nullptr}, d_sbeg, R,
@@ -528,11 +621,7 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
         if (h + 1 == H) break;
         DedupIn in{out->dst_dev, d_hb + h, out->capacity};
         Count E{0, d_etot};
-        uint32_t gen = next_gen(ws, ht, s);
-        hipLaunchKernelGGL(dedup_insert_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, E, d_eoff_r, R, ht, gen,
-                           err);
-        hipLaunchKernelGGL(dedup_mark_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, E, d_eoff_r, R, ht, gen,
-                           d_flag);
+        dedup(in, E, d_eoff_r, Dmax);
         e = exclusive_scan(FlagGen{d_flag}, E, Dmax, d_fpos, d_part, d_ftot, s);
         if (e != hipSuccess) return status_of(e);
         hipLaunchKernelGGL(compact_kernel, dim3(gridE), dim3(kKhBlock), 0, s, in, E, d_eoff_r, R, d_flag, d_fpos,

@@ -34,10 +34,15 @@ def _compare(ga, res, exp, R, hops):
         assert counts[: hops[r]] == [int(x) for x in exp["hop_count"][r, : hops[r]]]
 
 
+@pytest.mark.parametrize("slot", ["8", "16"])
 @pytest.mark.parametrize("fmt", FMTS)
 @pytest.mark.parametrize("case", ["rmat12", "degrees", "hub"])
-def test_khop_matches_oracle(ga, oracle_mod, case, fmt):
-    print(f"[khop] case={case} fmt={fmt}", flush=True)
+def test_khop_matches_oracle(ga, oracle_mod, case, fmt, slot, monkeypatch):
+    """Both dedup tables: 8-byte packed slots (the default when the request, vertex and position
+    bits fit a word) and the 16-byte generation-tagged ones (GRASS_KHOP_SLOT=16)."""
+    print(f"[khop] case={case} fmt={fmt} slot={slot}", flush=True)
+    if slot == "16":
+        monkeypatch.setenv("GRASS_KHOP_SLOT", "16")
     g, og = _pair(ga, oracle_mod, case, fmt)
     rng = np.random.default_rng(11)
     sizes = [1, 3, 17, 64, 200, 1000, 0, 5]
