@@ -61,27 +61,40 @@ __global__ __launch_bounds__(1024) void scan_partials_kernel(T* partials, uint64
     if (threadIdx.x == 0 && total) *total = tot;
 }
 
-// out[i] = sum of gen(0..i-1) for i < n, and out[n] = total.
+// out[i] = sum of gen(0..i-1) for i < n, and out[n] = total.  Wave w of a tile takes its
+// kWave * kScanItems consecutive items, kScanItems consecutive ones per lane: a lane sums its
+// own serially, one wave scan of the lane sums and one exchange of the wave sums give every
+// lane its start, and the lane writes its prefixes (no per-item workgroup scan).
 template <class Gen>
 __global__ __launch_bounds__(kScanBlock) void scan_write_kernel(Gen gen, Count cnt,
                                                                 const uint64_t* partials,
                                                                 const uint64_t* total,
                                                                 uint64_t* out) {
-    __shared__ uint64_t lds[kScanBlock / kWave];
+    __shared__ uint64_t wsum[kScanBlock / kWave];
     const uint64_t n = cnt.get();
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
     if (base >= n) {
         if (base == 0 && threadIdx.x == 0) out[0] = 0;  // empty sequence
         return;
     }
-    uint64_t carry = partials[blockIdx.x];
+    const int lane = lane_id(), wid = threadIdx.x / kWave;
+    const uint64_t i0 = base + ((uint64_t)wid * kWave + lane) * kScanItems;
+    uint64_t x[kScanItems];
+    uint64_t s = 0;
+#pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
-        uint64_t i = base + (uint64_t)j * kScanBlock + threadIdx.x;
-        uint64_t x = i < n ? gen(i) : 0;
-        uint64_t tot;
-        uint64_t ex = block_exclusive_sum<uint64_t, kScanBlock>(x, tot, lds);
-        if (i < n) out[i] = carry + ex;
-        carry += tot;
+        x[j] = i0 + j < n ? gen(i0 + j) : 0;
+        s += x[j];
+    }
+    const uint64_t inc = wave_inclusive_sum(s);
+    if (lane == kWave - 1) wsum[wid] = inc;
+    __syncthreads();
+    uint64_t pre = partials[blockIdx.x] + inc - s;
+    for (int w = 0; w < wid; ++w) pre += wsum[w];
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        if (i0 + j < n) out[i0 + j] = pre;
+        pre += x[j];
     }
     if (threadIdx.x == 0 && base + kScanTile >= n) out[n] = *total;
 }
