@@ -61,10 +61,24 @@ __global__ __launch_bounds__(1024) void scan_partials_kernel(T* partials, uint64
     if (threadIdx.x == 0 && total) *total = tot;
 }
 
-// out[i] = sum of gen(0..i-1) for i < n, and out[n] = total.  Wave w of a tile takes its
-// kWave * kScanItems consecutive items, kScanItems consecutive ones per lane: a lane sums its
-// own serially, one wave scan of the lane sums and one exchange of the wave sums give every
-// lane its start, and the lane writes its prefixes (no per-item workgroup scan).
+// 64-bit inclusive sum over the wave with DPP moves of the two halves (no LDS).
+__device__ __forceinline__ uint64_t wave_inclusive_sum_u64(uint64_t x) {
+#define GRASS_SCAN_STEP(CTRL, RM)                                                                  \
+    x += ((uint64_t)dpp_move<CTRL, RM>((uint32_t)(x >> 32)) << 32) | dpp_move<CTRL, RM>((uint32_t)x)
+    GRASS_SCAN_STEP(0x111, 0xf);
+    GRASS_SCAN_STEP(0x112, 0xf);
+    GRASS_SCAN_STEP(0x114, 0xf);
+    GRASS_SCAN_STEP(0x118, 0xf);
+    GRASS_SCAN_STEP(0x142, 0xa);
+    GRASS_SCAN_STEP(0x143, 0xc);
+#undef GRASS_SCAN_STEP
+    return x;
+}
+
+// out[i] = sum of gen(0..i-1) for i < n, and out[n] = total.  Wave w of a tile takes
+// kWave * kScanItems consecutive items in kScanItems rounds of kWave consecutive ones (coalesced
+// loads and stores): its lanes' sums and one exchange of the wave sums give each wave its start,
+// and each round is one DPP wave scan carried into the next (no workgroup scan per round).
 template <class Gen>
 __global__ __launch_bounds__(kScanBlock) void scan_write_kernel(Gen gen, Count cnt,
                                                                 const uint64_t* partials,
@@ -78,23 +92,26 @@ __global__ __launch_bounds__(kScanBlock) void scan_write_kernel(Gen gen, Count c
         return;
     }
     const int lane = lane_id(), wid = threadIdx.x / kWave;
-    const uint64_t i0 = base + ((uint64_t)wid * kWave + lane) * kScanItems;
+    const uint64_t wb = base + (uint64_t)wid * kWave * kScanItems + lane;
     uint64_t x[kScanItems];
     uint64_t s = 0;
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
-        x[j] = i0 + j < n ? gen(i0 + j) : 0;
+        const uint64_t i = wb + (uint64_t)j * kWave;
+        x[j] = i < n ? gen(i) : 0;
         s += x[j];
     }
-    const uint64_t inc = wave_inclusive_sum(s);
-    if (lane == kWave - 1) wsum[wid] = inc;
+    const uint64_t st = wave_inclusive_sum_u64(s);
+    if (lane == kWave - 1) wsum[wid] = st;
     __syncthreads();
-    uint64_t pre = partials[blockIdx.x] + inc - s;
-    for (int w = 0; w < wid; ++w) pre += wsum[w];
+    uint64_t carry = partials[blockIdx.x];
+    for (int w = 0; w < wid; ++w) carry += wsum[w];
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
-        if (i0 + j < n) out[i0 + j] = pre;
-        pre += x[j];
+        const uint64_t i = wb + (uint64_t)j * kWave;
+        const uint64_t inc = wave_inclusive_sum_u64(x[j]);
+        if (i < n) out[i] = carry + inc - x[j];
+        carry += readlane_u64(inc, kWave - 1);
     }
     if (threadIdx.x == 0 && base + kScanTile >= n) out[n] = *total;
 }
