@@ -100,7 +100,7 @@ class Tuning(C.Structure):
 class TuneReport(C.Structure):
     _fields_ = [("rw_count", C.c_uint32), ("rw_stage", C.c_uint32 * 9), ("rw_claim", C.c_uint32 * 9),
                 ("rw_ms", C.c_double * 9), ("khop_count", C.c_uint32),
-                ("khop_lane_scale", C.c_uint32 * 2), ("khop_ms", C.c_double * 2),
+                ("khop_lane_scale", C.c_uint32 * 3), ("khop_ms", C.c_double * 3),
                 ("decode_count", C.c_uint32), ("decode_edges_per_wave", C.c_uint32 * 5),
                 ("decode_ms", C.c_double * 5)]
 

@@ -320,6 +320,14 @@ __device__ __forceinline__ uint32_t sum_fields(const uint32_t* __restrict__ w, u
 // a checkpoint) plus the at most P - 1 gaps (body + 1) stored right after it.  fbits receives
 // the bits the same access costs in the .graph layout with a separate checkpoint table: 32 for
 // a checkpoint (head_bit for a head) plus the gap fields.
+// The absolute field and the gaps after it are read with one 16-byte load (four words from the
+// field's first word; the stream is padded, so the words past its end exist): 128 bits hold the
+// field's offset in its word (< 32), the field and `rem` gaps when they add up to 128 or less --
+// a checkpoint and three gaps of up to 21 bits.  Longer spans read the gaps word by word.
+struct __attribute__((packed, aligned(4))) Words4 {
+    uint32_t x, y, z, w;
+};
+
 __device__ __forceinline__ uint32_t gt_get_cp(const DevGraph& g, const Hdr& h, uint32_t v, uint32_t k,
                                               uint32_t* fbits = nullptr) {
     uint32_t blk, i;
@@ -328,18 +336,31 @@ __device__ __forceinline__ uint32_t gt_get_cp(const DevGraph& g, const Hdr& h, u
     const uint64_t bo = gtx_block_bit(h, blk, sh);
     const uint32_t b = (i >> sh) << sh;  // 0 when the vertex has no checkpoints (q < P)
     const uint32_t rem = i - b;
-    uint32_t val;
-    uint64_t gaps;
-    if (b) {
-        const uint64_t at = bo + gtx_field(h, b, sh);
-        val = read_bits(g.w, at, 32);
-        gaps = at + 32;
-    } else {
-        val = gt_head(g.w, h, v, blk, bo);
-        gaps = bo + h.f1;
+    const uint64_t at = b ? bo + gtx_field(h, b, sh) : bo;  // the checkpoint, or the block head
+    const uint32_t aw = b ? 32u : h.f1;
+    const Words4 q = *(const Words4*)(g.w + (at >> 5));
+    uint64_t hi = ((uint64_t)q.x << 32) | q.y, lo = ((uint64_t)q.z << 32) | q.w;
+    const uint32_t s = (uint32_t)(at & 31);
+    const uint32_t a = (uint32_t)((hi << s) >> (64 - aw));
+    uint32_t val = b ? a : (blk >= h.fp ? v + a + 1 : v - a - 1);
+    if (rem) {
+        const uint32_t f2 = h.f2, sft = s + aw;  // 1 <= sft <= 63
+        if (sft + rem * f2 <= 128) {
+            hi = (hi << sft) | (lo >> (64 - sft));
+            lo <<= sft;
+            uint32_t sum = rem;
+            if (f2)  // (a well-formed block with gaps has f2 >= 1; 0-bit fields read as 0)
+                for (uint32_t t = 0; t < rem; ++t) {
+                    sum += (uint32_t)(hi >> (64 - f2));
+                    hi = (hi << f2) | (lo >> (64 - f2));
+                    lo <<= f2;
+                }
+            val += sum;
+        } else {
+            val += sum_fields(g.w, at + aw, rem, f2) + rem;
+        }
     }
-    if (rem) val += sum_fields(g.w, gaps, rem, h.f2) + rem;
-    if (fbits) *fbits = (b ? 32u : h.f1) + rem * h.f2;
+    if (fbits) *fbits = aw + rem * h.f2;
     return val;
 }
 

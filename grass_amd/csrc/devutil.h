@@ -25,21 +25,18 @@ __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int src) {
 }
 
 // Lane `src` of a per-lane value read (wave-uniform) / replaced by a uniform value; the other lanes
-// keep theirs.  One v_readlane / v_writelane per 32-bit half.
+// keep theirs.  A readlane per 32-bit half; the write is a compare with the lane id and a select
+// (clang has no writelane builtin, and a v_writelane in inline asm would need M0, which the
+// compiler reserves).
 __device__ __forceinline__ uint32_t lane_get(uint32_t v, int src) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, src);
 }
 __device__ __forceinline__ uint64_t lane_get(uint64_t v, int src) { return readlane_u64(v, src); }
-// (clang has no writelane builtin.  The lane select goes through M0: on this target a VOP3 reads at
-// most one SGPR besides M0.  M0 is written by a scalar move, so no VALU wait states are needed;
-// the s_nop is a margin.)
 __device__ __forceinline__ uint32_t lane_set(uint32_t v, int src, uint32_t x) {
-    asm("s_mov_b32 m0, %2\n\ts_nop 0\n\tv_writelane_b32 %0, %1, m0" : "+v"(v) : "s"(x), "s"(src) : "m0");
-    return v;
+    return lane_id() == src ? x : v;
 }
 __device__ __forceinline__ uint64_t lane_set(uint64_t v, int src, uint64_t x) {
-    return ((uint64_t)lane_set((uint32_t)(v >> 32), src, (uint32_t)(x >> 32)) << 32) |
-           lane_set((uint32_t)v, src, (uint32_t)x);
+    return lane_id() == src ? x : v;
 }
 
 // Lanes below this one among the set bits of a 64-bit lane mask.
@@ -82,6 +79,18 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t x) {
     x += dpp_move<0x118>(x);
     x += dpp_move<0x142, 0xa>(x);
     x += dpp_move<0x143, 0xc>(x);
+    return x;
+}
+
+// Inclusive max over lanes 0..lane, the same DPP tree (a masked-off or out-of-range source reads
+// as 0, which leaves an unsigned max unchanged).
+__device__ __forceinline__ uint32_t wave_inclusive_max(uint32_t x) {
+    x = max(x, dpp_move<0x111>(x));
+    x = max(x, dpp_move<0x112>(x));
+    x = max(x, dpp_move<0x114>(x));
+    x = max(x, dpp_move<0x118>(x));
+    x = max(x, dpp_move<0x142, 0xa>(x));
+    x = max(x, dpp_move<0x143, 0xc>(x));
     return x;
 }
 

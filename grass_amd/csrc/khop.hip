@@ -308,8 +308,10 @@ __global__ __launch_bounds__(kKhBlock) void sample_kernel(DevGraph g, const uint
         if (!__any(has)) break;  // wave-uniform exit
         uint32_t v = 0, c = 0, d = 0, inst = 0;
         uint64_t ebase = 0;
+        uint4 hw = make_uint4(0, 0, 0, 0);
         if (has) {
             v = fv[i];
+            hw = g.hdr[v];  // issued before the draws, which do not need it
             const KhReq& q = reqs[fr[i]];
             uint32_t f = h < q.hops ? q.fan[h] : 0u;
             d = fd[i];
@@ -339,7 +341,7 @@ __global__ __launch_bounds__(kKhBlock) void sample_kernel(DevGraph g, const uint
         }
         if (gl < (int)c) {
             Hdr hd;
-            load_hdr<FMT>(g, v, hd);
+            unpack_hdr<FMT>(hw, hd);
             uint32_t nb = get_neighbor<FMT>(g, hd, v, x);
             if (nb >= g.n) {  // never let a bad id reach the next frontier
                 atomicCAS(err, kErrNone, kErrBadNeighbor);
@@ -349,6 +351,115 @@ __global__ __launch_bounds__(kKhBlock) void sample_kernel(DevGraph g, const uint
             if (p < cap) {
                 osrc[p] = v;
                 odst[p] = nb;
+            }
+        }
+    }
+}
+
+// The same draws with the lanes packed: a wave takes a chunk of K consecutive frontier items (one
+// per lane: their vertex, request, degree, edge offset and header, loaded once) and lays their c
+// draws side by side in rounds of 64 lanes, an item never split between two rounds.  Lane L of a
+// round is draw B + L of the chunk (edges numbered from the chunk's first), owned by the item
+// whose [st, st + c) holds it: each item of the round writes its lane into an LDS slot at st - B,
+// and an inclusive max over the slots hands every lane its owner.  Floyd's test and the ranks
+// run within the owner's lanes as in the grouped kernel, so the edges are the same; an item of
+// c = 4 takes 4 lanes rather than the hop's group width, and a round makes no dependent loads
+// besides the neighbour's own.  K is 64, or less when the frontier would leave waves idle.
+template <int FMT>
+__global__ __launch_bounds__(kKhBlock) void sample_packed_kernel(DevGraph g, const uint32_t* __restrict__ fv,
+                                                                 const uint32_t* __restrict__ fr,
+                                                                 const uint32_t* __restrict__ fd,
+                                                                 const uint64_t* __restrict__ icount,
+                                                                 const KhReq* __restrict__ reqs, uint32_t h,
+                                                                 uint64_t seed,
+                                                                 const uint64_t* __restrict__ eoff,
+                                                                 const uint64_t* __restrict__ hb, int fmax,
+                                                                 uint32_t* __restrict__ osrc,
+                                                                 uint32_t* __restrict__ odst, uint64_t cap,
+                                                                 uint32_t* err) {
+    __shared__ uint32_t owner_lds[kKhBlock];
+    uint32_t* own = owner_lds + (threadIdx.x & ~(kWave - 1));
+    const int lane = lane_id();
+    const uint64_t I = *icount;
+    const uint64_t hbase = hb[h];
+    const uint64_t nw = (uint64_t)gridDim.x * blockDim.x / kWave;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    uint32_t K = kWave;
+    while (K > 8 && K * nw > I) K >>= 1;
+    for (uint64_t base = wave * K; base < I; base += nw * K) {  // wave-uniform
+        const uint64_t i = base + lane;
+        uint32_t v = 0, c = 0, d = 0, inst = 0;
+        uint64_t e = 0;
+        uint4 hw = make_uint4(0, 0, 0, 0);
+        if (lane < (int)K && i < I) {
+            v = fv[i];
+            const KhReq& q = reqs[fr[i]];
+            const uint32_t f = h < q.hops ? q.fan[h] : 0u;
+            d = fd[i];
+            c = d < f ? d : f;
+            inst = q.inst;
+            e = eoff[i];
+            if (c) hw = g.hdr[v];
+        }
+        const uint64_t e0 = readlane_u64(e, 0);  // lane 0 holds item `base` < I
+        const uint32_t st = (uint32_t)(e - e0);  // chunk-local first draw (< 64 * 64)
+        unsigned long long left = __ballot(c > 0);
+        while (left) {  // one round
+            const uint32_t B = lane_get(st, __builtin_ctzll(left));
+            const bool in = c > 0 && st >= B && st + c <= B + kWave;
+            left &= ~__ballot(in);
+            own[lane] = 0;
+            wave_sync();
+            if (in) own[st - B] = lane + 1;
+            wave_sync();
+            const int j = (int)wave_inclusive_max(own[lane]) - 1;  // slot 0 holds the round's first item
+            wave_sync();
+            const uint32_t jv = __shfl(v, j, kWave), jc = __shfl(c, j, kWave), jd = __shfl(d, j, kWave);
+            const uint32_t js = __shfl(st, j, kWave), ji = __shfl(inst, j, kWave);
+            const uint32_t gl = B + lane - js;  // draw of the owner this lane makes
+            const bool act = gl < jc;
+            const bool all = jc == jd;  // every neighbour: positions 0..d-1, no draws
+            const int seg = (int)(js - B);
+            const uint32_t jj = jd - jc + gl;
+            uint32_t x = gl, rank = gl;
+            if (__any(act && !all)) {  // wave-uniform: every lane takes part in the shuffles
+                uint32_t t = 0;
+                if (act && !all) t = (uint32_t)uniform(seed, ji, jv, h, gl, (uint64_t)jj + 1);
+                // Floyd: x_k = t_k unless t_k equals an earlier x, then j_k; resolved in draw order
+                bool dup = false;
+                for (int k = 0; k < fmax; ++k) {
+                    const uint32_t xk = __shfl(dup ? jj : t, seg + k, kWave);
+                    if ((int)gl > k && k < (int)jc && t == xk) dup = true;
+                }
+                const uint32_t xd = dup ? jj : t;
+                uint32_t rk = 0;
+                for (int k = 0; k < fmax; ++k) {
+                    const uint32_t xk = __shfl(xd, seg + k, kWave);
+                    if (k < (int)jc && xk < xd) ++rk;
+                }
+                if (!all) {
+                    x = xd;
+                    rank = rk;
+                }
+            }
+            uint4 jh;
+            jh.x = __shfl(hw.x, j, kWave);
+            jh.y = __shfl(hw.y, j, kWave);
+            jh.z = __shfl(hw.z, j, kWave);
+            jh.w = __shfl(hw.w, j, kWave);
+            if (act) {
+                Hdr hd;
+                unpack_hdr<FMT>(jh, hd);
+                uint32_t nb = get_neighbor<FMT>(g, hd, jv, x);
+                if (nb >= g.n) {  // never let a bad id reach the next frontier
+                    atomicCAS(err, kErrNone, kErrBadNeighbor);
+                    nb = kInvalid;
+                }
+                const uint64_t p = hbase + e0 + js + rank;
+                if (p < cap) {
+                    osrc[p] = jv;
+                    odst[p] = nb;
+                }
             }
         }
     }
@@ -402,13 +513,21 @@ static unsigned sample_grid() {
     return grid;
 }
 
-// Hop h with G lanes per frontier vertex: the smallest of 4, 6, 8, 10, 12, 16, 21, 32, 64 that
-// covers the hop's largest fanout (times the tuned scale), so that a fanout-10 hop runs six
-// vertices per wave instead of four in 16-lane groups.
+// Hop h with packed lanes (G = 0), or with G lanes per frontier vertex: the smallest of 4, 6, 8,
+// 10, 12, 16, 21, 32, 64 that covers the hop's largest fanout (times the tuned scale), so that a
+// fanout-10 hop runs six vertices per wave instead of four in 16-lane groups.
 template <int FMT>
-static void launch_sample(grass_graph* g, uint32_t G, hipStream_t s, const uint32_t* fv, const uint32_t* fr,
-                          const uint32_t* fd, const uint64_t* ic, const KhReq* reqs, uint32_t h, uint64_t seed, const uint64_t* eoff,
-                          const uint64_t* hb, uint32_t* osrc, uint32_t* odst, uint64_t cap, uint32_t* err) {
+static void launch_sample(grass_graph* g, uint32_t G, uint32_t fmax, hipStream_t s,
+                          const uint32_t* fv, const uint32_t* fr, const uint32_t* fd, const uint64_t* ic,
+                          const KhReq* reqs, uint32_t h, uint64_t seed, const uint64_t* eoff, const uint64_t* hb,
+                          uint32_t* osrc, uint32_t* odst, uint64_t cap, uint32_t* err) {
+    if (G == 0) {
+        static unsigned grid = 0;
+        if (!grid) grid = persistent_grid((const void*)sample_packed_kernel<FMT>, kKhBlock);
+        hipLaunchKernelGGL(sample_packed_kernel<FMT>, dim3(grid), dim3(kKhBlock), 0, s, g->dev(), fv, fr, fd, ic,
+                           reqs, h, seed, eoff, hb, (int)fmax, osrc, odst, cap, err);
+        return;
+    }
 #define GRASS_SAMPLE(GG)                                                                                   \
     hipLaunchKernelGGL((sample_kernel<FMT, GG>), dim3(sample_grid<FMT, GG>()), dim3(kKhBlock), 0, s, g->dev(), \
                        fv, fr, fd, ic, reqs, h, seed, eoff, hb, osrc, odst, cap, err)
@@ -591,12 +710,17 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
         hipLaunchKernelGGL(front_offsets_kernel, dim3((R + 256) / 256), dim3(256), 0, s, d_sbeg, R, d_fpos,
                            d_foff[0], d_icount[0]);
     }
-    uint32_t lanes[GRASS_KHOP_MAX_HOPS];
+    // lanes per frontier vertex of each hop (0: packed lanes) and the hop's largest fanout
+    uint32_t lanes[GRASS_KHOP_MAX_HOPS], fmax[GRASS_KHOP_MAX_HOPS];
+    const uint32_t scale = g->tune.khop_lane_scale.load(std::memory_order_relaxed);
     for (uint32_t h = 0; h < H; ++h) {
         uint32_t f = 0;
         for (uint32_t r = 0; r < R; ++r)
             if (h < reqs[r].hops) f = std::max(f, reqs[r].fanout[h]);
-        const uint32_t need = f * g->tune.khop_lane_scale.load(std::memory_order_relaxed);
+        fmax[h] = f;
+        lanes[h] = 0;
+        if (!scale) continue;
+        const uint32_t need = f * scale;
         uint32_t G = 64;
         for (uint32_t c : {4u, 6u, 8u, 10u, 12u, 16u, 21u, 32u})
             if (c >= need) {
@@ -613,11 +737,11 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
         hipLaunchKernelGGL(hop_offsets_kernel, dim3((R + 256) / 256), dim3(256), 0, s, d_foff[cur], R, d_eoff,
                            d_eoff_r, d_hb, h, out->offsets_dev);
         if (g->fmt == kLE)
-            launch_sample<kLE>(g, lanes[h], s, d_fv[cur], d_fr[cur], d_fd[cur], d_icount[cur], d_req, h, seed, d_eoff,
-                               d_hb, out->src_dev, out->dst_dev, out->capacity, err);
+            launch_sample<kLE>(g, lanes[h], fmax[h], s, d_fv[cur], d_fr[cur], d_fd[cur], d_icount[cur],
+                               d_req, h, seed, d_eoff, d_hb, out->src_dev, out->dst_dev, out->capacity, err);
         else
-            launch_sample<kGT>(g, lanes[h], s, d_fv[cur], d_fr[cur], d_fd[cur], d_icount[cur], d_req, h, seed, d_eoff,
-                               d_hb, out->src_dev, out->dst_dev, out->capacity, err);
+            launch_sample<kGT>(g, lanes[h], fmax[h], s, d_fv[cur], d_fr[cur], d_fd[cur], d_icount[cur],
+                               d_req, h, seed, d_eoff, d_hb, out->src_dev, out->dst_dev, out->capacity, err);
         if (h + 1 == H) break;
         DedupIn in{out->dst_dev, d_hb + h, out->capacity};
         Count E{0, d_etot};

@@ -4,7 +4,8 @@
 // Knobs (none changes a result; every output is keyed by request, walker and Philox counters):
 //   rw_stage   LDS tile width of the walk kernels (row-segment size of the walk-matrix stores);
 //   rw_claim   granules a wave takes from the batch queue per atomic (queue granularity);
-//   khop_lane_scale  lanes per frontier vertex of the k-hop sampling kernel (decode width);
+//   khop_lane_scale  lanes per frontier vertex of the k-hop sampling kernel (decode width), or 0:
+//              each vertex takes as many lanes as it has draws (packed lanes);
 //   decode_edges_per_wave  output slots per wave of the full decode (work per wave).
 #include <hip/hip_runtime.h>
 #include <string.h>
@@ -23,7 +24,7 @@ bool valid(const grass_tuning& t) {
     auto in = [](uint32_t x, std::initializer_list<uint32_t> set) {
         return std::find(set.begin(), set.end(), x) != set.end();
     };
-    return in(t.rw_stage, {8, 16, 32}) && in(t.rw_claim, {1, 2, 4}) && in(t.khop_lane_scale, {1, 2}) &&
+    return in(t.rw_stage, {8, 16, 32}) && in(t.rw_claim, {1, 2, 4}) && in(t.khop_lane_scale, {0, 1, 2}) &&
            in(t.decode_edges_per_wave, {512, 1024, 2048, 4096, 8192});
 }
 
@@ -159,7 +160,7 @@ int grass_autotune(grass_graph* g, uint32_t what, grass_tuning* chosen, grass_tu
         if (!rc) rc = khop_bound(g, reqs.data(), R, &cap, nullptr, nullptr, nullptr);
         if (!rc && hipMalloc(&d_out.p, cap * 8 + 8ull * 2 * (R + 1)) != hipSuccess) rc = GRASS_ENOMEM;
         double bms = -1;
-        for (uint32_t sc : {1u, 2u}) {
+        for (uint32_t sc : {0u, 1u, 2u}) {
             if (rc) break;
             grass_tuning t = best;
             t.khop_lane_scale = sc;

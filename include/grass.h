@@ -259,8 +259,9 @@ void grass_mdrw_free(grass_mdrw_result *res);
 typedef struct {
     uint32_t rw_stage;              /* walk columns a wave stages in LDS per flush: 8, 16, 32 */
     uint32_t rw_claim;              /* 64-walker granules a wave takes per queue access: 1, 2, 4 */
-    uint32_t khop_lane_scale;       /* k-hop lanes per frontier vertex = scale x the smallest of
-                                       8/16/32/64 covering the hop's fanout (capped at 64): 1, 2 */
+    uint32_t khop_lane_scale;       /* k-hop lanes per frontier vertex: 0 = as many as the vertex
+                                       has draws (packed lanes), or scale x the smallest group
+                                       width covering the hop's fanout (capped at 64): 1, 2 */
     uint32_t decode_edges_per_wave; /* full-decode output slots per wave: 512 ... 8192, power of 2 */
 } grass_tuning;
 
@@ -278,8 +279,8 @@ typedef struct {
     uint32_t rw_stage[9], rw_claim[9];
     double rw_ms[9];
     uint32_t khop_count;
-    uint32_t khop_lane_scale[2];
-    double khop_ms[2];
+    uint32_t khop_lane_scale[3];
+    double khop_ms[3];
     uint32_t decode_count;
     uint32_t decode_edges_per_wave[5];
     double decode_ms[5];

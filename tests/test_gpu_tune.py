@@ -41,7 +41,7 @@ def test_every_candidate_gives_the_oracle_result(ga, oracle_mod, fmt):
             np.testing.assert_array_equal(ga.to_numpy_u32(o), e)
         np.testing.assert_array_equal(ga.to_numpy_u32(g.random_walk(300, 33, seed=8, instance=1)),
                                       og.random_walk(300, 33, seed=8, instance=1)[0])
-    for sc in (1, 2):
+    for sc in (0, 1, 2):
         g.set_tuning(khop_lane_scale=sc)
         res = g.khop_batch(seeds, fans, seed=8)
         assert res["total"] == len(exp_k["dst"])
@@ -62,7 +62,7 @@ def test_autotune_and_validation(ga, oracle_mod):
     gb, ob = c["gt_graph"].tobytes(), c["gt_offset"].tobytes()
     g, og = ga.Graph.from_bytes(gb, ob, "gt"), oracle_mod.Graph(gb, ob, "gt")
     chosen, rep = g.autotune()
-    assert len(rep["rw"]) == 9 and len(rep["khop"]) == 2 and len(rep["decode"]) == 5
+    assert len(rep["rw"]) == 9 and len(rep["khop"]) == 3 and len(rep["decode"]) == 5
     assert all(ms > 0 for *_, ms in rep["rw"] + rep["khop"] + rep["decode"])
     assert chosen == g.tuning()
     assert chosen["rw_stage"] in (8, 16, 32) and chosen["rw_claim"] in (1, 2, 4)
