@@ -11,9 +11,10 @@
 //                 Philox(seed, {instance, f, h, k}); edges (f, N_f[pos]) in ascending pos;
 //   frontier h+1 = destinations of hop h, duplicates dropped (first appearance kept).
 // Per hop: one fused count+scan (exclusive scan over items of min(fanout, deg)), a sampling
-// kernel with G lanes per frontier vertex (the c draws are made in parallel; Floyd's
-// "taken?" test is resolved in draw order with wave shuffles; ranks give the sorted order), and
-// a dedup of (request, vertex) keys in a generation-tagged hash table where the smallest edge
+// kernel (the c draws of a frontier vertex are made in parallel by c lanes -- in groups of a
+// fixed width, or packed side by side when the hop's vertices often have fewer neighbours than
+// the fanout; Floyd's "taken?" test is resolved in draw order with wave shuffles; ranks give the
+// sorted order), and a dedup of (request, vertex) keys in a hash table where the smallest edge
 // position wins, followed by a flag scan and a compaction into the next frontier.
 // Output is hop-major: edges of (hop h, request r) are [off[h][r], off[h][r+1]).
 #include <hip/hip_runtime.h>
@@ -281,26 +282,25 @@ __global__ void hop_offsets_kernel(const uint64_t* __restrict__ foff, uint32_t R
     }
 }
 
+// Lane layouts of the sampling kernel.
+enum SampleLanes : int { kLanesPacked = 0, kLanesAuto = 1, kLanesGroups = 2 };
+
+#define GRASS_SAMPLE_PARAMS                                                                              \
+    DevGraph g, const uint32_t* __restrict__ fv, const uint32_t* __restrict__ fr,                      \
+        const uint32_t* __restrict__ fd, uint64_t I, const KhReq* __restrict__ reqs, uint32_t h,       \
+        uint64_t seed, const uint64_t* __restrict__ eoff, uint64_t hbase, uint32_t* __restrict__ osrc, \
+        uint32_t* __restrict__ odst, uint64_t cap, uint32_t* err
+#define GRASS_SAMPLE_ARGS g, fv, fr, fd, I, reqs, h, seed, eoff, hbase, osrc, odst, cap, err
+
+// Groups of G lanes, one frontier item per group: kWave / G items per wave and pass; when G does
+// not divide 64 the last lanes are idle.
 template <int FMT, int G>
-__global__ __launch_bounds__(kKhBlock) void sample_kernel(DevGraph g, const uint32_t* __restrict__ fv,
-                                                          const uint32_t* __restrict__ fr,
-                                                          const uint32_t* __restrict__ fd,
-                                                          const uint64_t* __restrict__ icount,
-                                                          const KhReq* __restrict__ reqs, uint32_t h,
-                                                          uint64_t seed,
-                                                          const uint64_t* __restrict__ eoff,
-                                                          const uint64_t* __restrict__ hb,
-                                                          uint32_t* __restrict__ osrc,
-                                                          uint32_t* __restrict__ odst, uint64_t cap,
-                                                          uint32_t* err) {
-    // kWave / G groups of G lanes per wave; when G does not divide 64 the last lanes are idle
+__device__ __forceinline__ void sample_groups(GRASS_SAMPLE_PARAMS) {
     constexpr int GPW = kWave / G;
     const int lane = lane_id();
     const int grp = lane / G;
     const int gl = lane - grp * G;
     const int gbase = grp * G;
-    const uint64_t I = *icount;
-    const uint64_t hbase = hb[h];
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / kWave * GPW;
     for (uint64_t i = wave * GPW + grp;; i += groups) {
@@ -356,32 +356,20 @@ __global__ __launch_bounds__(kKhBlock) void sample_kernel(DevGraph g, const uint
     }
 }
 
-// The same draws with the lanes packed: a wave takes a chunk of K consecutive frontier items (one
-// per lane: their vertex, request, degree, edge offset and header, loaded once) and lays their c
-// draws side by side in rounds of 64 lanes, an item never split between two rounds.  Lane L of a
-// round is draw B + L of the chunk (edges numbered from the chunk's first), owned by the item
-// whose [st, st + c) holds it: each item of the round writes its lane into an LDS slot at st - B,
-// and an inclusive max over the slots hands every lane its owner.  Floyd's test and the ranks
-// run within the owner's lanes as in the grouped kernel, so the edges are the same; an item of
-// c = 4 takes 4 lanes rather than the hop's group width, and a round makes no dependent loads
-// besides the neighbour's own.  K is 64, or less when the frontier would leave waves idle.
+// Packed lanes: a wave takes a chunk of K consecutive frontier items (one per lane: their vertex,
+// request, degree, edge offset and header, loaded once) and lays their c draws side by side in
+// rounds of 64 lanes, an item never split between two rounds.  Lane L of a round is draw B + L of
+// the chunk (edges numbered from the chunk's first), owned by the item whose [st, st + c) holds
+// it: each item of the round writes its lane into an LDS slot at st - B, and an inclusive max over
+// the slots hands every lane its owner.  Floyd's test and the ranks run within the owner's lanes
+// as in the groups, so the edges are the same; an item of c = 4 takes 4 lanes rather than a
+// group, and a round makes no dependent loads besides the neighbour's own.  K is 64, or less when
+// the frontier would leave waves idle.  fmax bounds c.
 template <int FMT>
-__global__ __launch_bounds__(kKhBlock) void sample_packed_kernel(DevGraph g, const uint32_t* __restrict__ fv,
-                                                                 const uint32_t* __restrict__ fr,
-                                                                 const uint32_t* __restrict__ fd,
-                                                                 const uint64_t* __restrict__ icount,
-                                                                 const KhReq* __restrict__ reqs, uint32_t h,
-                                                                 uint64_t seed,
-                                                                 const uint64_t* __restrict__ eoff,
-                                                                 const uint64_t* __restrict__ hb, int fmax,
-                                                                 uint32_t* __restrict__ osrc,
-                                                                 uint32_t* __restrict__ odst, uint64_t cap,
-                                                                 uint32_t* err) {
+__device__ __forceinline__ void sample_packed(GRASS_SAMPLE_PARAMS, int fmax) {
     __shared__ uint32_t owner_lds[kKhBlock];
     uint32_t* own = owner_lds + (threadIdx.x & ~(kWave - 1));
     const int lane = lane_id();
-    const uint64_t I = *icount;
-    const uint64_t hbase = hb[h];
     const uint64_t nw = (uint64_t)gridDim.x * blockDim.x / kWave;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     uint32_t K = kWave;
@@ -465,6 +453,56 @@ __global__ __launch_bounds__(kKhBlock) void sample_packed_kernel(DevGraph g, con
     }
 }
 
+// Whether a hop takes packed lanes: always, never, or (kLanesAuto) when its draws would fill less
+// than 70 % of the lanes of groups of G -- seeds, say, which are often short lists under a large
+// fanout.  Measured on R-MAT s24 2-hop [25, 10]: hop 0 fills 33 % of 32-lane groups and runs
+// twice as fast packed, hop 1 fills 98 % of 10-lane groups and runs 12 % faster in groups.  The
+// frontier size and the hop's edge count are on the device by the time the hop runs.
+__device__ __forceinline__ bool packed_hop(int lanes, uint64_t edges, uint64_t items, int G) {
+    return lanes == kLanesPacked || (lanes == kLanesAuto && edges * 10 < items * G * 7);
+}
+
+// One hop's draws in groups of G lanes; returns at once when the hop takes packed lanes.  (The two
+// layouts are separate kernels: in one kernel with both, the groups ran 18 % slower.)
+template <int FMT, int G>
+__global__ __launch_bounds__(kKhBlock) void sample_kernel(DevGraph g, const uint32_t* __restrict__ fv,
+                                                          const uint32_t* __restrict__ fr,
+                                                          const uint32_t* __restrict__ fd,
+                                                          const uint64_t* __restrict__ icount,
+                                                          const KhReq* __restrict__ reqs, uint32_t h,
+                                                          uint64_t seed,
+                                                          const uint64_t* __restrict__ eoff,
+                                                          const uint64_t* __restrict__ hb, int lanes,
+                                                          uint32_t* __restrict__ osrc,
+                                                          uint32_t* __restrict__ odst, uint64_t cap,
+                                                          uint32_t* err) {
+    const uint64_t I = *icount;
+    const uint64_t hbase = hb[h];
+    if (packed_hop(lanes, hb[h + 1] - hbase, I, G)) return;
+    sample_groups<FMT, G>(GRASS_SAMPLE_ARGS);
+}
+
+// One hop's draws in packed lanes; returns at once when the hop takes groups of G.
+template <int FMT>
+__global__ __launch_bounds__(kKhBlock) void sample_packed_kernel(DevGraph g, const uint32_t* __restrict__ fv,
+                                                                 const uint32_t* __restrict__ fr,
+                                                                 const uint32_t* __restrict__ fd,
+                                                                 const uint64_t* __restrict__ icount,
+                                                                 const KhReq* __restrict__ reqs, uint32_t h,
+                                                                 uint64_t seed,
+                                                                 const uint64_t* __restrict__ eoff,
+                                                                 const uint64_t* __restrict__ hb, int fmax,
+                                                                 int lanes, int G, uint32_t* __restrict__ osrc,
+                                                                 uint32_t* __restrict__ odst, uint64_t cap,
+                                                                 uint32_t* err) {
+    const uint64_t I = *icount;
+    const uint64_t hbase = hb[h];
+    if (!packed_hop(lanes, hb[h + 1] - hbase, I, G)) return;
+    sample_packed<FMT>(GRASS_SAMPLE_ARGS, fmax);
+}
+#undef GRASS_SAMPLE_PARAMS
+#undef GRASS_SAMPLE_ARGS
+
 int khop_bound(const grass_graph* g, const grass_khop_request* reqs, uint32_t count,
                uint64_t* edges, uint64_t* items, uint64_t* dedup, uint32_t* max_hops) {
     uint64_t E = 0, I = 0, D = 0;
@@ -513,24 +551,25 @@ static unsigned sample_grid() {
     return grid;
 }
 
-// Hop h with packed lanes (G = 0), or with G lanes per frontier vertex: the smallest of 4, 6, 8,
-// 10, 12, 16, 21, 32, 64 that covers the hop's largest fanout (times the tuned scale), so that a
-// fanout-10 hop runs six vertices per wave instead of four in 16-lane groups.
+// Hop h in `lanes` layout (SampleLanes) with groups of G lanes: the smallest of 4, 6, 8, 10, 12,
+// 16, 21, 32, 64 that covers the hop's largest fanout (twice that with the tuned scale 2), so that
+// a fanout-10 hop runs six vertices per wave instead of four in 16-lane groups.  kLanesAuto
+// launches both kernels and the one the hop does not take returns at once.
 template <int FMT>
-static void launch_sample(grass_graph* g, uint32_t G, uint32_t fmax, hipStream_t s,
+static void launch_sample(grass_graph* g, uint32_t G, uint32_t fmax, int lanes, hipStream_t s,
                           const uint32_t* fv, const uint32_t* fr, const uint32_t* fd, const uint64_t* ic,
                           const KhReq* reqs, uint32_t h, uint64_t seed, const uint64_t* eoff, const uint64_t* hb,
                           uint32_t* osrc, uint32_t* odst, uint64_t cap, uint32_t* err) {
-    if (G == 0) {
+    if (lanes != kLanesGroups) {
         static unsigned grid = 0;
         if (!grid) grid = persistent_grid((const void*)sample_packed_kernel<FMT>, kKhBlock);
         hipLaunchKernelGGL(sample_packed_kernel<FMT>, dim3(grid), dim3(kKhBlock), 0, s, g->dev(), fv, fr, fd, ic,
-                           reqs, h, seed, eoff, hb, (int)fmax, osrc, odst, cap, err);
-        return;
+                           reqs, h, seed, eoff, hb, (int)fmax, lanes, (int)G, osrc, odst, cap, err);
+        if (lanes == kLanesPacked) return;
     }
 #define GRASS_SAMPLE(GG)                                                                                   \
     hipLaunchKernelGGL((sample_kernel<FMT, GG>), dim3(sample_grid<FMT, GG>()), dim3(kKhBlock), 0, s, g->dev(), \
-                       fv, fr, fd, ic, reqs, h, seed, eoff, hb, osrc, odst, cap, err)
+                       fv, fr, fd, ic, reqs, h, seed, eoff, hb, lanes, osrc, odst, cap, err)
     if (G <= 4) GRASS_SAMPLE(4);
     else if (G <= 6) GRASS_SAMPLE(6);
     else if (G <= 8) GRASS_SAMPLE(8);
@@ -710,17 +749,18 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
         hipLaunchKernelGGL(front_offsets_kernel, dim3((R + 256) / 256), dim3(256), 0, s, d_sbeg, R, d_fpos,
                            d_foff[0], d_icount[0]);
     }
-    // lanes per frontier vertex of each hop (0: packed lanes) and the hop's largest fanout
+    // group width of each hop and the hop's largest fanout; the tuned scale picks the layout:
+    // 0 packed lanes, 1 packed or groups as the hop's draws fill the groups, 2 groups of twice
+    // the width
     uint32_t lanes[GRASS_KHOP_MAX_HOPS], fmax[GRASS_KHOP_MAX_HOPS];
     const uint32_t scale = g->tune.khop_lane_scale.load(std::memory_order_relaxed);
+    const int layout = scale == 0 ? kLanesPacked : scale == 1 ? kLanesAuto : kLanesGroups;
     for (uint32_t h = 0; h < H; ++h) {
         uint32_t f = 0;
         for (uint32_t r = 0; r < R; ++r)
             if (h < reqs[r].hops) f = std::max(f, reqs[r].fanout[h]);
         fmax[h] = f;
-        lanes[h] = 0;
-        if (!scale) continue;
-        const uint32_t need = f * scale;
+        const uint32_t need = f * std::max(scale, 1u);
         uint32_t G = 64;
         for (uint32_t c : {4u, 6u, 8u, 10u, 12u, 16u, 21u, 32u})
             if (c >= need) {
@@ -737,10 +777,10 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
         hipLaunchKernelGGL(hop_offsets_kernel, dim3((R + 256) / 256), dim3(256), 0, s, d_foff[cur], R, d_eoff,
                            d_eoff_r, d_hb, h, out->offsets_dev);
         if (g->fmt == kLE)
-            launch_sample<kLE>(g, lanes[h], fmax[h], s, d_fv[cur], d_fr[cur], d_fd[cur], d_icount[cur],
+            launch_sample<kLE>(g, lanes[h], fmax[h], layout, s, d_fv[cur], d_fr[cur], d_fd[cur], d_icount[cur],
                                d_req, h, seed, d_eoff, d_hb, out->src_dev, out->dst_dev, out->capacity, err);
         else
-            launch_sample<kGT>(g, lanes[h], fmax[h], s, d_fv[cur], d_fr[cur], d_fd[cur], d_icount[cur],
+            launch_sample<kGT>(g, lanes[h], fmax[h], layout, s, d_fv[cur], d_fr[cur], d_fd[cur], d_icount[cur],
                                d_req, h, seed, d_eoff, d_hb, out->src_dev, out->dst_dev, out->capacity, err);
         if (h + 1 == H) break;
         DedupIn in{out->dst_dev, d_hb + h, out->capacity};

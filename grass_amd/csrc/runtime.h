@@ -111,7 +111,7 @@ struct Workspace {
 struct TuneState {
     std::atomic<uint32_t> rw_stage{32};              // walk columns staged per LDS flush: 8, 16, 32
     std::atomic<uint32_t> rw_claim{1};               // 64-walker granules per queue access: 1, 2, 4
-    std::atomic<uint32_t> khop_lane_scale{1};        // k-hop lanes per vertex = scale x fanout cover
+    std::atomic<uint32_t> khop_lane_scale{1};        // k-hop lane layout: 0 packed, 1 auto, 2 wide groups
     std::atomic<uint32_t> decode_edges_per_wave{1024};  // full-decode slots per wave
 };
 

@@ -4,8 +4,9 @@
 // Knobs (none changes a result; every output is keyed by request, walker and Philox counters):
 //   rw_stage   LDS tile width of the walk kernels (row-segment size of the walk-matrix stores);
 //   rw_claim   granules a wave takes from the batch queue per atomic (queue granularity);
-//   khop_lane_scale  lanes per frontier vertex of the k-hop sampling kernel (decode width), or 0:
-//              each vertex takes as many lanes as it has draws (packed lanes);
+//   khop_lane_scale  lane layout of the k-hop sampling kernel: 0 packed lanes (a vertex takes as
+//              many lanes as it has draws), 1 per hop packed or fanout-wide groups as the draws
+//              fill the groups, 2 groups of twice the fanout cover;
 //   decode_edges_per_wave  output slots per wave of the full decode (work per wave).
 #include <hip/hip_runtime.h>
 #include <string.h>

@@ -259,9 +259,11 @@ void grass_mdrw_free(grass_mdrw_result *res);
 typedef struct {
     uint32_t rw_stage;              /* walk columns a wave stages in LDS per flush: 8, 16, 32 */
     uint32_t rw_claim;              /* 64-walker granules a wave takes per queue access: 1, 2, 4 */
-    uint32_t khop_lane_scale;       /* k-hop lanes per frontier vertex: 0 = as many as the vertex
-                                       has draws (packed lanes), or scale x the smallest group
-                                       width covering the hop's fanout (capped at 64): 1, 2 */
+    uint32_t khop_lane_scale;       /* k-hop sampling lanes: 0 = each frontier vertex takes as
+                                       many lanes as it has draws (packed); 1 = per hop, packed
+                                       when the hop's draws fill under 70 % of groups of the
+                                       smallest width covering its fanout, else those groups
+                                       (default); 2 = groups of twice that width */
     uint32_t decode_edges_per_wave; /* full-decode output slots per wave: 512 ... 8192, power of 2 */
 } grass_tuning;
 
