@@ -291,8 +291,10 @@ typedef struct {
 /* Times each candidate of the selected knobs (what = a mask of GRASS_TUNE_*, 0 = all) on
  * synthetic requests drawn from the graph with Philox -- 128 random-walk requests of 2048
  * walkers and lengths 20-80; 128 2-hop [25,10] requests of 400 seeds; one full decode (skipped
- * when its CSR does not fit in free HBM) -- keeps the fastest in the graph and returns it.
- * Synchronises; uses the graph's own stream.  report may be NULL. */
+ * when its CSR does not fit in free HBM) -- keeps the fastest in the graph and returns it.  A
+ * candidate other than the setting in force at the call must be 3 % faster than that setting to
+ * replace it (a near tie keeps the setting).  Synchronises; uses the graph's own stream.  report
+ * may be NULL. */
 int grass_autotune(grass_graph *g, uint32_t what, grass_tuning *chosen, grass_tune_report *report);
 
 /* ------------------------------------------------------------------ captured batches
