@@ -101,6 +101,7 @@ void free_graph(grass_graph* g) {
     if (g->d_gtx) (void)hipFree(g->d_gtx);
     for (uint32_t* t : g->d_wstart)
         if (t) (void)hipFree(t);
+    if (g->d_roff) (void)hipFree(g->d_roff);
     if (g->d_err) (void)hipFree(g->d_err);
     g->ws.release();
     g->ws_bytes.release();

@@ -260,6 +260,234 @@ __device__ __forceinline__ void decode_range(const DevGraph& g, const Src& src, 
 }
 
 
+// ---------------------------------------------------------------- GT runs (P <= 4)
+// A run is a block head or a checkpoint and the gap fields after it up to the next one: at most P
+// positions, the first absolute, so a run decodes without any other.  The run kernel gives each
+// lane one run: its slots and stream bit from the run's index in its list (no per-position
+// division), its fields from one 16-byte load of the stream (a fifth word when they reach past
+// it), its ids by adding the gaps in registers (no cross-lane scan).  The 64 runs of a wave's pass
+// fill consecutive output slots, which are staged in LDS and stored with coalesced writes.  Runs
+// are numbered over the graph (roff = exclusive scan of runs per vertex) and every wave takes a
+// fixed number of them, with its first vertex from a table like the slot-based one.
+
+constexpr uint32_t kRunMaxP = 4;  // the run kernel takes P <= kRunMaxP (the default P is 4)
+
+// Runs of a GT vertex with d entries: R blocks of q + 1 positions and C - R of q, each
+// ceil(len / P) runs.
+__host__ __device__ __forceinline__ uint64_t gt_runs(uint32_t d, uint32_t C, uint32_t q, uint32_t sh) {
+    if (!d) return 0;
+    const uint32_t R = d - C * q, P1 = (1u << sh) - 1;
+    return (uint64_t)R * ((q + 1 + P1) >> sh) + (uint64_t)(C - R) * ((q + P1) >> sh);
+}
+
+struct RunGen {  // runs of vertex i, from its header
+    const uint4* hdr;
+    uint32_t sh;
+    __device__ __forceinline__ uint64_t operator()(uint64_t i) const {
+        const uint4 e = hdr[i];
+        return gt_runs(e.z, e.w & 0xFFFFu, e.y >> 16, sh);
+    }
+};
+
+// One run's place: first output slot, stream bit of its absolute field, positions, whether that
+// field is the block head (else a checkpoint), and its block.
+struct RunAt {
+    uint64_t slot, bit;
+    uint32_t cnt, blk;
+    bool head;
+};
+
+// Run rr (< the vertex's runs) of a vertex: R blocks of nr1 runs, then blocks of nr0.  Div divides
+// by nr1 / nr0 (shared multiply-high reciprocals when the wave is in one list, else per lane).
+template <class Div1, class Div0>
+__device__ __forceinline__ RunAt run_at(const Hdr& h, uint64_t vstart, uint32_t rr, uint32_t sh, const Div1& div1,
+                                        const Div0& div0) {
+    const uint32_t P1 = (1u << sh) - 1;
+    const uint32_t nr1 = (h.q + 1 + P1) >> sh, nr0 = (h.q + P1) >> sh;
+    const uint32_t bigr = h.R * nr1;
+    uint32_t blk, j, len;
+    uint64_t bpos, bbit;
+    if (rr < bigr) {
+        blk = div1(rr, nr1);
+        j = rr - blk * nr1;
+        len = h.q + 1;
+        bpos = (uint64_t)blk * (h.q + 1);
+        bbit = h.pay + (uint64_t)blk * gtx_field(h, h.q + 1, sh);
+    } else {
+        const uint32_t t = rr - bigr, b2 = div0(t, nr0);
+        j = t - b2 * nr0;
+        blk = h.R + b2;
+        len = h.q;
+        bpos = (uint64_t)h.R * (h.q + 1) + (uint64_t)b2 * h.q;
+        bbit = h.pay + (uint64_t)h.R * gtx_field(h, h.q + 1, sh) + (uint64_t)b2 * gtx_field(h, h.q, sh);
+    }
+    const uint32_t i0 = j << sh;
+    RunAt a;
+    a.cnt = min(1u << sh, len - i0);
+    a.slot = vstart + bpos + i0;
+    a.bit = bbit + gtx_field(h, i0, sh);
+    a.head = i0 == 0;
+    a.blk = blk;
+    return a;
+}
+
+// The ids of one run into out[0 .. cnt): the absolute field, then each gap + 1 added on.  One
+// 16-byte load holds the field's offset in its word (< 32), a 32-bit field and three gaps of up to
+// 21 bits; a fifth word is read when they need more (at most 31 + 32 + 3 * 31 < 160 bits).
+__device__ __forceinline__ void run_ids(const uint32_t* __restrict__ w, const Hdr& h, uint32_t v, const RunAt& a,
+                                        uint32_t out[kRunMaxP]) {
+    const uint32_t* p = w + (a.bit >> 5);
+    const Words4 q = *(const Words4*)p;
+    const uint32_t s = (uint32_t)(a.bit & 31);
+    const uint32_t aw = a.head ? h.f1 : 32u, f2 = h.f2;
+    const uint32_t w4 = s + aw + (a.cnt - 1) * f2 > 128 ? p[4] : 0u;
+    uint64_t buf = (((uint64_t)q.x << 32) | q.y) << s;
+    uint32_t avail = 64 - s, n0 = q.z, n1 = q.w, n2 = w4;  // buffered bits, then the words after them
+    auto take = [&](uint32_t wdt) {  // 1 <= wdt <= 32
+        if (avail < wdt) {
+            buf |= (uint64_t)n0 << (32 - avail);
+            avail += 32;
+            n0 = n1;
+            n1 = n2;
+        }
+        const uint32_t r = (uint32_t)(buf >> (64 - wdt));
+        buf <<= wdt;
+        avail -= wdt;
+        return r;
+    };
+    const uint32_t a0 = take(aw);  // f1 >= 1 for a vertex with entries
+    uint32_t x = a.head ? (a.blk >= h.fp ? v + a0 + 1 : v - a0 - 1) : a0;
+    out[0] = x;
+#pragma unroll
+    for (uint32_t k = 1; k < kRunMaxP; ++k) {
+        if (k < a.cnt) {
+            x += (f2 ? take(f2) : 0u) + 1;  // (0-bit gap fields read as 0)
+            out[k] = x;
+        }
+    }
+}
+
+struct UDivBy {  // per-lane division by a list constant (float reciprocal, corrected)
+    __device__ __forceinline__ uint32_t operator()(uint32_t n, uint32_t d) const { return udiv_small(n, d); }
+};
+struct UDivShared {  // division by the one divisor it was made for
+    UDiv u;
+    __device__ __forceinline__ uint32_t operator()(uint32_t n, uint32_t) const { return u.div(n); }
+};
+
+// Wave w decodes runs [w * per_wave, ...) of the graph; rstart[w] is the vertex holding its first.
+__global__ __launch_bounds__(kDecBlock) void decode_runs_kernel(DevGraph g, const uint64_t* __restrict__ rowptr,
+                                                                const uint64_t* __restrict__ roff,
+                                                                const uint32_t* __restrict__ rstart,
+                                                                uint32_t* __restrict__ col, uint64_t nruns,
+                                                                uint32_t per_wave) {
+    __shared__ uint32_t stage_lds[(kDecBlock / kWave) * kWave * kRunMaxP];
+    uint32_t* L = stage_lds + (threadIdx.x / kWave) * (kWave * kRunMaxP);
+    const uint64_t wave = ((uint64_t)blockIdx.x * kDecBlock + threadIdx.x) / kWave;
+    const int lane = lane_id();
+    const uint64_t r0 = wave * per_wave;
+    if (r0 >= nruns) return;  // uniform per wave
+    const uint64_t r1 = r0 + per_wave < nruns ? r0 + per_wave : nruns;
+    const uint32_t sh = g.cp_shift;
+    // the vertex of the pass's first run and its run bounds, wave-uniform
+    uint32_t cur = __builtin_amdgcn_readfirstlane(rstart[wave]);
+    uint64_t rc = roff[cur], rn = roff[cur + 1];
+    uint32_t hv = 0xFFFFFFFFu;  // vertex whose list constants are cached below
+    Hdr uh;
+    uint64_t uvs = 0;
+    UDivShared u1, u0;
+    for (uint64_t base = r0; base < r1; base += kWave) {
+        const uint64_t rho = base + lane;
+        const bool act = rho < r1;
+        const uint64_t rr = act ? rho : r1 - 1;
+        RunAt a;
+        Hdr h;
+        uint32_t v;
+        if (base >= rc && base + (kWave - 1) < rn) {
+            // the whole pass is in one list: one header, shared divisors
+            if (cur != hv) {
+                hv = cur;
+                load_hdr<kGT>(g, hv, uh);
+                uvs = rowptr[hv];
+                const uint32_t P1 = (1u << sh) - 1;
+                u1.u.init((uh.q + 1 + P1) >> sh);
+                u0.u.init((uh.q + P1) >> sh);
+            }
+            h = uh;
+            v = hv;
+            a = run_at(h, uvs, (uint32_t)(rr - rc), sh, u1, u0);
+        } else {
+            // Owner of run rr among the 64 vertices after `cur` (their run ends, headers and
+            // first slots in one coalesced load each), found by a 6-step search in registers;
+            // a run more than 64 vertices on (only across vertices without entries) searches
+            // roff.  The next pass's bounds come from the run ends.
+            const uint64_t bj = (uint64_t)cur + 1 + lane <= g.n ? roff[cur + 1 + lane] : ~0ull;
+            uint4 hj = make_uint4(0u, 0u, 0u, 0u);
+            uint64_t sj = 0;
+            if (cur + (uint32_t)lane < g.n) {
+                hj = g.hdr[cur + lane];
+                sj = rowptr[cur + lane];
+            }
+            const uint32_t rj = bj - rc > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)(bj - rc);
+            const uint32_t er = (uint32_t)(rr - rc);
+            uint32_t pos = 0;
+#pragma unroll
+            for (uint32_t st = 32; st >= 1; st >>= 1) {
+                const uint32_t t = __shfl(rj, (int)(pos + st - 1), kWave);
+                if (t <= er) pos += st;
+            }
+            const uint32_t last = __shfl(rj, kWave - 1, kWave);
+            if (pos == kWave - 1 && last <= er) pos = kWave;
+            const uint32_t prev = __shfl(rj, (int)(pos > 0 ? pos - 1 : 0), kWave);
+            const int src_lane = (int)(pos < (uint32_t)kWave ? pos : 0u);
+            uint4 hv4 = make_uint4(__shfl(hj.x, src_lane, kWave), __shfl(hj.y, src_lane, kWave),
+                                   __shfl(hj.z, src_lane, kWave), __shfl(hj.w, src_lane, kWave));
+            uint64_t vs = ((uint64_t)__shfl((uint32_t)(sj >> 32), src_lane, kWave) << 32) |
+                          __shfl((uint32_t)sj, src_lane, kWave);
+            uint64_t rstart_v;
+            if (pos < (uint32_t)kWave) {
+                v = cur + pos;
+                rstart_v = rc + (pos ? prev : 0u);
+            } else {
+                v = owner_of(roff, cur + kWave, g.n - 1, rr);
+                rstart_v = roff[v];
+                hv4 = g.hdr[v];
+                vs = rowptr[v];
+            }
+            const uint32_t p63 = (uint32_t)__builtin_amdgcn_readlane((int)pos, kWave - 1);
+            if (p63 < (uint32_t)kWave) {
+                if (p63) {
+                    cur += p63;
+                    rc = readlane_u64(bj, (int)p63 - 1);
+                    rn = readlane_u64(bj, (int)p63);
+                }
+            } else {
+                cur = __builtin_amdgcn_readlane((int)v, kWave - 1);
+                rc = roff[cur];
+                rn = roff[cur + 1];
+            }
+            unpack_hdr<kGT>(hv4, h);
+            a = run_at(h, vs, (uint32_t)(rr - rstart_v), sh, UDivBy{}, UDivBy{});
+        }
+        uint32_t ids[kRunMaxP];
+        run_ids(g.w, h, v, a, ids);
+        // stage the pass's slots [sb, se) and store them coalesced
+        const uint64_t sb = readlane_u64(a.slot, 0);
+        const uint32_t lastl = (uint32_t)(r1 - base < (uint64_t)kWave ? r1 - base - 1 : kWave - 1);
+        const uint32_t count = (uint32_t)(readlane_u64(a.slot, (int)lastl) - sb) +
+                               (uint32_t)__builtin_amdgcn_readlane((int)a.cnt, (int)lastl);
+        if (act) {
+            const uint32_t o = (uint32_t)(a.slot - sb);
+#pragma unroll
+            for (uint32_t k = 0; k < kRunMaxP; ++k)
+                if (k < a.cnt) L[o + k] = ids[k];
+        }
+        wave_sync();
+        for (uint32_t k = lane; k < count; k += kWave) col[sb + k] = L[k];
+        wave_sync();  // the stage is read before the next pass writes it
+    }
+}
+
 // LDS words a wave may copy its span into (the rest of a larger span is read from global memory).
 constexpr uint32_t kDecStageMax = 2048;
 __host__ __device__ constexpr uint32_t dec_stage_words(uint32_t per_wave) {
@@ -357,14 +585,45 @@ int decode_csr(grass_graph* g, uint64_t* rowptr, uint32_t* col, hipStream_t s) {
     }
     if (m == 0 || !col) return GRASS_OK;
     const uint32_t per_wave = g->tune.decode_edges_per_wave.load(std::memory_order_relaxed);
-    const uint64_t waves = (m + per_wave - 1) / per_wave;
-    const uint64_t blocks = (waves + kDecBlock / kWave - 1) / (kDecBlock / kWave);
-    if (blocks > 0x7fffffffull) return GRASS_EINVAL;
-    // The waves' first vertices depend only on the graph and per_wave: built once per per_wave
-    // value from these row pointers, complete before it is published, never changed after.
     uint32_t slot = 0;
     while ((512u << slot) < per_wave && slot + 1 < kWaveStartSlots) ++slot;
     if ((512u << slot) != per_wave) return GRASS_EINVAL;
+    // GT with P <= 4 decodes by runs: about per_wave slots per wave, per_wave / P runs.  The run
+    // offsets (a scan over the headers) and the total are built once per graph.
+    const bool runs = g->fmt == kGT && (1u << g->cp_shift) <= kRunMaxP;
+    const uint32_t unit = runs ? per_wave >> g->cp_shift : per_wave;
+    uint64_t units = m;
+    if (runs) {
+        std::lock_guard<std::mutex> lock(g->mu);
+        if (!g->d_roff) {
+            uint64_t* r = nullptr;
+            uint64_t* part = nullptr;
+            if (hipMalloc(&r, (n + 1) * 8) != hipSuccess) return GRASS_ENOMEM;
+            if (hipMalloc(&part, (nb + 4) * 8) != hipSuccess) {
+                (void)hipFree(r);
+                return GRASS_ENOMEM;
+            }
+            uint64_t total = 0;
+            hipError_t e = exclusive_scan(RunGen{g->d_hdr, g->cp_shift}, Count{n, nullptr}, n, r, part, part + nb, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(&total, r + n, 8, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            (void)hipFree(part);
+            if (e != hipSuccess) {
+                (void)hipFree(r);
+                return status_of(e);
+            }
+            g->d_roff = r;
+            g->dec_runs = total;
+        }
+        units = g->dec_runs;
+    }
+    const uint64_t* uoff = runs ? (const uint64_t*)g->d_roff : (const uint64_t*)rowptr;
+    const uint64_t waves = (units + unit - 1) / unit;
+    const uint64_t blocks = (waves + kDecBlock / kWave - 1) / (kDecBlock / kWave);
+    if (blocks > 0x7fffffffull) return GRASS_EINVAL;
+    // The waves' first vertices depend only on the graph and per_wave: built once per per_wave
+    // value from the row pointers (or the run offsets), complete before it is published, never
+    // changed after.
     const uint32_t* wstart;
     {
         std::lock_guard<std::mutex> lock(g->mu);
@@ -372,8 +631,7 @@ int decode_csr(grass_graph* g, uint64_t* rowptr, uint32_t* col, hipStream_t s) {
             uint32_t* t = nullptr;
             if (hipMalloc(&t, (waves + 1) * 4) != hipSuccess) return GRASS_ENOMEM;
             const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 16384);
-            hipLaunchKernelGGL(wave_start_kernel, dim3(grid), dim3(256), 0, s, (const uint64_t*)rowptr,
-                               (uint32_t)n, per_wave, waves, t);
+            hipLaunchKernelGGL(wave_start_kernel, dim3(grid), dim3(256), 0, s, uoff, (uint32_t)n, unit, waves, t);
             hipError_t e = hipGetLastError();
             if (e == hipSuccess) e = hipStreamSynchronize(s);
             if (e != hipSuccess) {
@@ -384,7 +642,10 @@ int decode_csr(grass_graph* g, uint64_t* rowptr, uint32_t* col, hipStream_t s) {
         }
         wstart = g->d_wstart[slot];
     }
-    if (g->fmt == kLE) {
+    if (runs) {
+        hipLaunchKernelGGL(decode_runs_kernel, dim3((unsigned)blocks), dim3(kDecBlock), 0, s, g->dev(),
+                           (const uint64_t*)rowptr, uoff, wstart, col, units, unit);
+    } else if (g->fmt == kLE) {
         hipLaunchKernelGGL(decode_csr_kernel<kLE>, dim3((unsigned)blocks), dim3(kDecBlock), 0, s,
                            g->dev(), (const uint64_t*)rowptr, wstart, col, m, per_wave, 0u);
     } else {

@@ -155,8 +155,11 @@ struct grass_graph {
     uint32_t* d_gtx = nullptr;  // GT: checkpointed payload stream the samplers decode
     uint32_t cp_shift = grass::kGtCheckShiftDefault;  // GT checkpoint interval 1 << cp_shift
     uint32_t* d_err = nullptr;  // [0] first error kind, [1] detail
-    // full decode: first vertex of every wave, one table per slots-per-wave value 512 << i
+    // full decode: first vertex of every wave, one table per slots-per-wave value 512 << i (GT
+    // with P <= 4: of every wave's first run, the waves taking (512 << i) / P runs each)
     uint32_t* d_wstart[grass::kWaveStartSlots] = {};
+    uint64_t* d_roff = nullptr;  // full decode, GT with P <= 4: runs before each vertex (n + 1)
+    uint64_t dec_runs = 0;       // and their total
     grass::Scratch dec_part;     // full decode: the row-pointer scan's partials (under mu)
     hipEvent_t dec_ev = nullptr;  // recorded after each full decode's scan
     hipStream_t stream = nullptr;
