@@ -146,6 +146,32 @@ __device__ __forceinline__ uint32_t wave_segmented_sum(uint32_t x, int head) {
     return x;
 }
 
+// Segmented inclusive sum over runs of at most `span` lanes (a lane with head != 0 starts a
+// run): log2(span) shuffle steps instead of six.
+__device__ __forceinline__ uint32_t short_segmented_sum(uint32_t x, int head, uint32_t span) {
+    const int lane = lane_id();
+    if (span == 4) {
+        // the default checkpoint interval: runs of at most four lanes, so a lane adds the values
+        // of the (up to three) lanes before it back to its run's head, moved by wave shifts of
+        // one lane (DPP, no LDS); lane 0 always starts a run here
+        const uint32_t hf = head ? 1u : 0u;
+        const uint32_t v1 = dpp_move<0x138>(x), h1 = dpp_move<0x138>(hf);
+        const uint32_t v2 = dpp_move<0x138>(v1), h2 = dpp_move<0x138>(h1);
+        const uint32_t v3 = dpp_move<0x138>(v2);
+        const bool a1 = !hf, a2 = a1 && !h1, a3 = a2 && !h2;
+        return x + (a1 ? v1 : 0u) + (a2 ? v2 : 0u) + (a3 ? v3 : 0u);
+    }
+    for (uint32_t off = 1; off < span && off < (uint32_t)kWave; off <<= 1) {
+        const uint32_t px = __shfl_up(x, off, kWave);
+        const int ph = __shfl_up(head, off, kWave);
+        if (lane >= (int)off && !head) {
+            x += px;
+            head = ph;
+        }
+    }
+    return x;
+}
+
 // Exclusive sum over a workgroup of BLOCK threads; `total` receives the workgroup sum.
 // `lds` holds BLOCK / 64 elements.  Every thread of the workgroup must call it.
 template <typename T, int BLOCK>

@@ -490,7 +490,7 @@ __global__ __launch_bounds__(kIndBlock) void mdrw_induced_kernel(DevGraph g, con
                     y += carry;
                     head = 1;
                 }
-                y = wave_segmented_sum(y, head);
+                y = short_segmented_sum(y, head, 1u << g.cp_shift);  // runs of at most P entries
                 carry = __shfl(y, kWave - 1, kWave);
             }
             dy = kNoRow;
