@@ -1,7 +1,9 @@
 // Reduce-then-scan exclusive prefix sum over a generated sequence of uint64 values.
 //
-// Three launches with no inter-workgroup waiting: per-tile sums, one workgroup scanning the
-// tile sums, then each tile re-generates its values and writes prefixes.  The length can be
+// Launches with no inter-workgroup waiting: per-tile sums, then each tile re-generates its values
+// and writes prefixes.  A sequence of at most kScanFoldMax tiles has each tile add up the tile sums
+// before it itself (a few thousand cached values); a longer one scans the tile sums in one
+// workgroup in between (a third launch).  The length can be
 // a host value or a device-resident count (so a sequence whose size a previous kernel
 // produced needs no host round trip); the grid is sized for the host-side upper bound.
 // Used for the record-length scan at load (replacing cub::DeviceScan::ExclusiveSum,
@@ -17,6 +19,7 @@ namespace grass {
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 16;
 constexpr uint64_t kScanTile = (uint64_t)kScanBlock * kScanItems;
+constexpr uint64_t kScanFoldMax = 2048;  // tiles whose start each tile sums itself
 
 struct Count {
     uint64_t imm;
@@ -79,17 +82,32 @@ __device__ __forceinline__ uint64_t wave_inclusive_sum_u64(uint64_t x) {
 // kWave * kScanItems consecutive items in kScanItems rounds of kWave consecutive ones (coalesced
 // loads and stores): its lanes' sums and one exchange of the wave sums give each wave its start,
 // and each round is one DPP wave scan carried into the next (no workgroup scan per round).
-template <class Gen>
+// FOLD: partials[] are the tile sums themselves (at most kScanFoldMax of them): each tile adds up
+// those before it, and the tile holding item n - 1 also writes *total; otherwise partials[] are
+// the tiles' exclusive prefixes and *total was written with them.
+template <class Gen, bool FOLD>
 __global__ __launch_bounds__(kScanBlock) void scan_write_kernel(Gen gen, Count cnt,
                                                                 const uint64_t* partials,
-                                                                const uint64_t* total,
+                                                                uint64_t* total,
                                                                 uint64_t* out) {
     __shared__ uint64_t wsum[kScanBlock / kWave];
+    __shared__ uint64_t fold_lds[kScanBlock / kWave];
     const uint64_t n = cnt.get();
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
     if (base >= n) {
-        if (base == 0 && threadIdx.x == 0) out[0] = 0;  // empty sequence
+        if (base == 0 && threadIdx.x == 0) {  // empty sequence
+            out[0] = 0;
+            if (FOLD && total) *total = 0;
+        }
         return;
+    }
+    uint64_t start;
+    if (FOLD) {
+        uint64_t p = 0;
+        for (uint64_t k = threadIdx.x; k < blockIdx.x; k += kScanBlock) p += partials[k];
+        start = block_sum<uint64_t, kScanBlock>(p, fold_lds);
+    } else {
+        start = partials[blockIdx.x];
     }
     const int lane = lane_id(), wid = threadIdx.x / kWave;
     const uint64_t wb = base + (uint64_t)wid * kWave * kScanItems + lane;
@@ -104,7 +122,7 @@ __global__ __launch_bounds__(kScanBlock) void scan_write_kernel(Gen gen, Count c
     const uint64_t st = wave_inclusive_sum_u64(s);
     if (lane == kWave - 1) wsum[wid] = st;
     __syncthreads();
-    uint64_t carry = partials[blockIdx.x];
+    uint64_t carry = start;
     for (int w = 0; w < wid; ++w) carry += wsum[w];
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
@@ -113,7 +131,16 @@ __global__ __launch_bounds__(kScanBlock) void scan_write_kernel(Gen gen, Count c
         if (i < n) out[i] = carry + inc - x[j];
         carry += readlane_u64(inc, kWave - 1);
     }
-    if (threadIdx.x == 0 && base + kScanTile >= n) out[n] = *total;
+    if (threadIdx.x == 0 && base + kScanTile >= n) {
+        if (FOLD) {
+            uint64_t t = start;
+            for (int w = 0; w < kScanBlock / kWave; ++w) t += wsum[w];
+            out[n] = t;
+            if (total) *total = t;
+        } else {
+            out[n] = *total;
+        }
+    }
 }
 
 inline uint64_t scan_blocks(uint64_t n_max) { return n_max ? (n_max + kScanTile - 1) / kScanTile : 1; }
@@ -125,9 +152,14 @@ hipError_t exclusive_scan(Gen gen, Count cnt, uint64_t n_max, uint64_t* out, uin
     const uint64_t nb = scan_blocks(n_max);
     hipLaunchKernelGGL(scan_reduce_kernel<Gen>, dim3((unsigned)nb), dim3(kScanBlock), 0, stream,
                        gen, cnt, partials);
-    hipLaunchKernelGGL(scan_partials_kernel<uint64_t>, dim3(1), dim3(1024), 0, stream, partials, nb, total);
-    hipLaunchKernelGGL(scan_write_kernel<Gen>, dim3((unsigned)nb), dim3(kScanBlock), 0, stream,
-                       gen, cnt, (const uint64_t*)partials, (const uint64_t*)total, out);
+    if (nb <= kScanFoldMax) {
+        hipLaunchKernelGGL((scan_write_kernel<Gen, true>), dim3((unsigned)nb), dim3(kScanBlock), 0, stream,
+                           gen, cnt, (const uint64_t*)partials, total, out);
+    } else {
+        hipLaunchKernelGGL(scan_partials_kernel<uint64_t>, dim3(1), dim3(1024), 0, stream, partials, nb, total);
+        hipLaunchKernelGGL((scan_write_kernel<Gen, false>), dim3((unsigned)nb), dim3(kScanBlock), 0, stream,
+                           gen, cnt, (const uint64_t*)partials, total, out);
+    }
     return hipGetLastError();
 }
 
