@@ -147,11 +147,9 @@ struct PackTab {
         const unsigned long long w = (k << pb) | p;
         uint64_t h = home(k);
         for (uint64_t probes = 0; probes <= mask; ++probes) {
-            unsigned long long cur = __atomic_load_n(slot + h, __ATOMIC_RELAXED);
-            if (cur == ~0ull) {
-                cur = atomicCAS(slot + h, ~0ull, w);
-                if (cur == ~0ull) return;
-            }
+            // the CAS claims an empty slot, or returns what the slot holds
+            const unsigned long long cur = atomicCAS(slot + h, ~0ull, w);
+            if (cur == ~0ull) return;
             if ((cur >> pb) == k) {
                 if (cur > w) atomicMin(slot + h, w);
                 return;
