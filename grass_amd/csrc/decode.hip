@@ -342,7 +342,9 @@ __device__ __forceinline__ void run_ids(const uint32_t* __restrict__ w, const Hd
 }
 
 struct UDivBy {  // per-lane division by a list constant (float reciprocal, corrected)
-    __device__ __forceinline__ uint32_t operator()(uint32_t n, uint32_t d) const { return udiv_small(n, d); }
+    __device__ __forceinline__ uint32_t operator()(uint32_t n, uint32_t d) const {
+        return d == 1 ? n : udiv_small(n, d);  // short lists: one run per block
+    }
 };
 struct UDivShared {  // division by the one divisor it was made for
     UDiv u;
@@ -356,7 +358,9 @@ __global__ __launch_bounds__(kDecBlock) void decode_runs_kernel(DevGraph g, cons
                                                                 uint32_t* __restrict__ col, uint64_t nruns,
                                                                 uint32_t per_wave) {
     __shared__ uint32_t stage_lds[(kDecBlock / kWave) * kWave * kRunMaxP];
+    __shared__ uint32_t owner_lds[kDecBlock];
     uint32_t* L = stage_lds + (threadIdx.x / kWave) * (kWave * kRunMaxP);
+    uint32_t* own = owner_lds + (threadIdx.x & ~(kWave - 1));
     const uint64_t wave = ((uint64_t)blockIdx.x * kDecBlock + threadIdx.x) / kWave;
     const int lane = lane_id();
     const uint64_t r0 = wave * per_wave;
@@ -391,10 +395,12 @@ __global__ __launch_bounds__(kDecBlock) void decode_runs_kernel(DevGraph g, cons
             v = hv;
             a = run_at(h, uvs, (uint32_t)(rr - rc), sh, u1, u0);
         } else {
-            // Owner of run rr among the 64 vertices after `cur` (their run ends, headers and
-            // first slots in one coalesced load each), found by a 6-step search in registers;
-            // a run more than 64 vertices on (only across vertices without entries) searches
-            // roff.  The next pass's bounds come from the run ends.
+            // The 64 vertices from `cur`: their run ends (relative to rc), headers and first slots
+            // in one coalesced load each.  Vertex cur + j covers the pass's lanes from its first
+            // run on; each vertex that starts a lane of the pass (or holds its first run) writes
+            // j + 1 into that lane's LDS slot, and an inclusive max over the slots gives every
+            // lane its vertex (no search).  When those vertices end before the pass does (only
+            // across vertices without entries), each lane searches the run offsets instead.
             const uint64_t bj = (uint64_t)cur + 1 + lane <= g.n ? roff[cur + 1 + lane] : ~0ull;
             uint4 hj = make_uint4(0u, 0u, 0u, 0u);
             uint64_t sj = 0;
@@ -403,40 +409,46 @@ __global__ __launch_bounds__(kDecBlock) void decode_runs_kernel(DevGraph g, cons
                 sj = rowptr[cur + lane];
             }
             const uint32_t rj = bj - rc > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)(bj - rc);
-            const uint32_t er = (uint32_t)(rr - rc);
-            uint32_t pos = 0;
-#pragma unroll
-            for (uint32_t st = 32; st >= 1; st >>= 1) {
-                const uint32_t t = __shfl(rj, (int)(pos + st - 1), kWave);
-                if (t <= er) pos += st;
-            }
-            const uint32_t last = __shfl(rj, kWave - 1, kWave);
-            if (pos == kWave - 1 && last <= er) pos = kWave;
-            const uint32_t prev = __shfl(rj, (int)(pos > 0 ? pos - 1 : 0), kWave);
-            const int src_lane = (int)(pos < (uint32_t)kWave ? pos : 0u);
-            uint4 hv4 = make_uint4(__shfl(hj.x, src_lane, kWave), __shfl(hj.y, src_lane, kWave),
-                                   __shfl(hj.z, src_lane, kWave), __shfl(hj.w, src_lane, kWave));
-            uint64_t vs = ((uint64_t)__shfl((uint32_t)(sj >> 32), src_lane, kWave) << 32) |
-                          __shfl((uint32_t)sj, src_lane, kWave);
+            const uint32_t pj = dpp_move<0x138>(rj);  // start of vertex cur + lane (lane 0: 0)
+            const uint32_t e0 = (uint32_t)(base - rc);
+            const uint32_t lastl = (uint32_t)(r1 - base < (uint64_t)kWave ? r1 - base - 1 : kWave - 1);
+            const uint32_t elast = e0 + lastl;
+            int jo;
             uint64_t rstart_v;
-            if (pos < (uint32_t)kWave) {
-                v = cur + pos;
-                rstart_v = rc + (pos ? prev : 0u);
+            if ((uint32_t)__builtin_amdgcn_readlane((int)rj, kWave - 1) > elast) {
+                const bool cand = rj > pj && rj > e0 && pj <= elast;
+                own[lane] = 0;
+                wave_sync();
+                if (cand) own[pj > e0 ? pj - e0 : 0u] = lane + 1;
+                wave_sync();
+                jo = (int)wave_inclusive_max(own[lane]) - 1;  // lane 0 always has a vertex
+                wave_sync();
+                jo = act ? jo : __builtin_amdgcn_readlane(jo, (int)lastl);
+                v = cur + jo;
+                rstart_v = rc + __shfl(pj, jo, kWave);
             } else {
-                v = owner_of(roff, cur + kWave, g.n - 1, rr);
+                v = owner_of(roff, cur, g.n - 1, rr);
+                jo = (int)(v - cur < (uint32_t)kWave ? v - cur : 0u);
                 rstart_v = roff[v];
+            }
+            uint4 hv4 = make_uint4(__shfl(hj.x, jo, kWave), __shfl(hj.y, jo, kWave), __shfl(hj.z, jo, kWave),
+                                   __shfl(hj.w, jo, kWave));
+            uint64_t vs = ((uint64_t)__shfl((uint32_t)(sj >> 32), jo, kWave) << 32) | __shfl((uint32_t)sj, jo, kWave);
+            if (v - cur >= (uint32_t)kWave) {
                 hv4 = g.hdr[v];
                 vs = rowptr[v];
             }
-            const uint32_t p63 = (uint32_t)__builtin_amdgcn_readlane((int)pos, kWave - 1);
-            if (p63 < (uint32_t)kWave) {
-                if (p63) {
-                    cur += p63;
-                    rc = readlane_u64(bj, (int)p63 - 1);
-                    rn = readlane_u64(bj, (int)p63);
+            // the next pass starts at or after the last active lane's vertex
+            const uint32_t vl = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lastl);
+            if (vl - cur < (uint32_t)kWave) {
+                const int jl = (int)(vl - cur);
+                if (jl) {
+                    cur = vl;
+                    rc = readlane_u64(bj, jl - 1);
+                    rn = readlane_u64(bj, jl);
                 }
             } else {
-                cur = __builtin_amdgcn_readlane((int)v, kWave - 1);
+                cur = vl;
                 rc = roff[cur];
                 rn = roff[cur + 1];
             }
