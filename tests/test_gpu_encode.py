@@ -145,3 +145,36 @@ def test_encode_one_huge_list(env, fmt):
     csr = ingest.csr_from_sorted_edges(src, col)
     assert csr.n == n
     _check(torch, ingest, rowptr, col, fmt, csr.compress(fmt, threads=8))
+
+
+@pytest.mark.parametrize("cp", ["2", "4"])
+def test_gt_decode_across_empty_vertices(env, monkeypatch, cp):
+    """GT full decode by runs over lists separated by long stretches of vertices without entries
+    (more than a pass's 64 vertices), short lists whose runs span many vertices per pass, and a
+    few long ones: the CSR comes back, at checkpoint intervals 2 and 4."""
+    print(f"[gt decode empty stretches] P={cp}", flush=True)
+    torch, ingest = env
+    import grass_amd
+    monkeypatch.setenv("GRASS_GT_CHECKPOINT", cp)
+    rng = np.random.default_rng(11)
+    n = 6000
+    deg = rng.integers(1, 12, size=n).astype(np.int64)
+    deg[rng.choice(n, size=8, replace=False)] = rng.integers(300, 3000, size=8)
+    for lo, hi in ((100, 300), (1000, 1070), (2000, 2065), (4000, 4500)):
+        deg[lo:hi] = 0
+    deg[n - 1] = 1
+    rowptr = np.zeros(n + 1, np.uint64)
+    rowptr[1:] = np.cumsum(deg)
+    # no self loops: a GT block head stores |first - v| - 1 (graph_sort drops self loops)
+    col = np.concatenate([np.sort(np.setdiff1d(rng.choice(n, size=int(d) + 1, replace=False), [v])[:int(d)])
+                          .astype(np.uint32) for v, d in enumerate(deg) if d])
+    assert len(col) == int(rowptr[-1])
+    rp, cl = _dev(torch, rowptr, col)
+    g = grass_amd.Graph.from_csr_device(rp, cl, "gt")
+    assert g.info()["gt_checkpoint"] == int(cp)
+    for e in (512, 2048):
+        g.set_tuning(decode_edges_per_wave=e)
+        r2, c2 = g.decode_csr()
+        np.testing.assert_array_equal(r2, rowptr)
+        np.testing.assert_array_equal(c2, col)
+    g.close()
