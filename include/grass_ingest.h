@@ -43,8 +43,8 @@ void grass_csr_free(grass_csr *csr);
 /* The LE (format 0) or GT (format 1) compressor in "normal" mode
  * (LE/chunk_compress/compressor.hpp:288-403,130-285; GT/chunk_compress/compressor.hpp:
  * 245-349,120-243): writes the .graph and .offset images.  As in the reference, a GT block head
- * holds |first - v| - 1, so a row that holds v itself (a self loop, which grass_graph_sort drops)
- * encodes to bytes that do not decode back to that row. */
+ * holds |first - v| - 1, so a GT block of row v that starts with v itself (a self loop, which
+ * grass_graph_sort drops) encodes to bytes that do not decode back to that row. */
 int grass_compress(const grass_csr *csr, int format, int nthreads, grass_blob **graph,
                    grass_blob **offset);
 
