@@ -453,7 +453,7 @@ __device__ __forceinline__ void sample_packed(GRASS_SAMPLE_PARAMS, int fmax) {
 
 // Whether a hop takes packed lanes: always, never, or (kLanesAuto) when its draws would fill less
 // than 70 % of the lanes of groups of G -- seeds, say, which are often short lists under a large
-// fanout.  Measured on R-MAT s24 2-hop [25, 10]: hop 0 fills 33 % of 32-lane groups and runs
+// fanout.  Measured on R-MAT s24 2-hop [25, 10]: hop 0 fills 29 % of 32-lane groups and runs
 // twice as fast packed, hop 1 fills 98 % of 10-lane groups and runs 12 % faster in groups.  The
 // frontier size and the hop's edge count are on the device by the time the hop runs.
 __device__ __forceinline__ bool packed_hop(int lanes, uint64_t edges, uint64_t items, int G) {
