@@ -107,7 +107,10 @@ int grass_sync(grass_graph *g, void *stream);
 
 /* Full decode to CSR: rowptr_dev[n+1] (uint64), col_dev[m] (uint32); col_dev == NULL writes
  * the row pointers only.  Replaces running
- * BCGVertex::get_vertex (LE/src/bcg.cuh:131-261, GT/src/bcg.cuh:110-168) over every k. */
+ * BCGVertex::get_vertex (LE/src/bcg.cuh:131-261, GT/src/bcg.cuh:110-168) over every k.  The
+ * first call with col_dev builds per-graph tables it keeps until grass_close: each decode wave's
+ * first vertex (4 bytes per wave) and, for GT with a checkpoint interval of at most 4, the runs
+ * before each vertex (8 (n + 1) bytes, with one host synchronisation). */
 int grass_decode_csr(grass_graph *g, uint64_t *rowptr_dev, uint32_t *col_dev, void *stream);
 
 /* Uniform random walks, one launch: out_dev is row-major [walkers][length+1] uint32; a walk
