@@ -580,6 +580,36 @@ static void launch_sample(grass_graph* g, uint32_t G, uint32_t fmax, int lanes, 
 #undef GRASS_SAMPLE
 }
 
+static uint32_t bit_width(uint64_t x) {
+    uint32_t k = 0;
+    while (k < 64 && (x >> k) != 0) ++k;
+    return k;
+}
+
+// Slots of the packed dedup table for `entries` positions (load <= 2/3).
+static uint64_t pack_slots(uint64_t entries) {
+    uint64_t slots = 1024;
+    while (slots < entries + entries / 2 + 2) slots <<= 1;
+    return slots;
+}
+
+// The batch's dedup table is the 8-byte packed one when (request, vertex, position) fit one word;
+// GRASS_KHOP_SLOT=16 forces the 16-byte generation-tagged table.
+static bool kh_packed_table(const grass_graph* g, uint32_t R, uint64_t S, uint64_t Dmax) {
+    const uint32_t vb = std::max<uint32_t>(bit_width(g->n ? g->n - 1 : 0), 1u);
+    const uint32_t pb = std::max<uint32_t>(bit_width(std::max(S, Dmax)), 1u);
+    const char* force16 = getenv("GRASS_KHOP_SLOT");
+    return bit_width(R - 1) + vb + pb <= 64 && !(force16 && strcmp(force16, "16") == 0);
+}
+
+// Device scratch of a batch of R requests, S seeds, frontier bound Imax and dedup bound Dmax.
+static size_t kh_scratch_need(uint64_t R, uint64_t S, uint64_t Imax, uint64_t Dmax) {
+    const uint64_t nb = std::max(scan_blocks(Imax), scan_blocks(Dmax));
+    return Carve::need(R * sizeof(KhReq)) + Carve::need((R + 1) * 8) * 4 + Carve::need(S * 4 + 4) +
+           Carve::need(Imax * 4) * 6 + Carve::need((Imax + 1) * 8) + Carve::need(Dmax * 4) +
+           Carve::need((Dmax + 1) * 8) + Carve::need(nb * 8) + Carve::need(64 * 8);
+}
+
 static hipError_t ensure_table(Workspace& ws, uint64_t entries, HashTab& t) {
     uint64_t slots = 1024;
     while (slots < 2 * entries + 2) slots <<= 1;
@@ -616,6 +646,22 @@ int khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uint6
     return out->total > out->capacity ? GRASS_ECAPACITY : GRASS_OK;
 }
 
+int khop_reserve(const grass_graph* g, Workspace& ws, uint32_t R, uint64_t S, uint64_t I, uint64_t D) {
+    if (!R) return GRASS_OK;
+    const uint64_t Imax = std::max<uint64_t>(I, 1), Dmax = std::max<uint64_t>(D, 1);
+    if (ws.reserve_stage(R * sizeof(KhReq) + (R + 1) * 8 + 64) != hipSuccess ||
+        ws.scratch.reserve(kh_scratch_need(R, S, Imax, Dmax)) != hipSuccess)
+        return GRASS_ENOMEM;
+    if (kh_packed_table(g, R, S, Dmax)) {
+        if (ws.ht.reserve(pack_slots(std::max(S, Dmax)) * 8) != hipSuccess) return GRASS_ENOMEM;
+        ws.ht_gen = 0xFFFF;
+    } else {
+        HashTab t{};
+        if (ensure_table(ws, Dmax, t) != hipSuccess) return GRASS_ENOMEM;
+    }
+    return GRASS_OK;
+}
+
 int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uint64_t seed,
                  grass_khop_output* out, hipStream_t s, Workspace& ws) {
     // the edge total goes to ws.result_slot(), taken after reserve_stage may have moved it
@@ -650,12 +696,7 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
     const uint64_t S = sbeg[R];
     const uint64_t Imax = std::max<uint64_t>(Ibound, 1), Dmax = std::max<uint64_t>(Dbound, 1);
     const uint64_t nbI = scan_blocks(Imax), nbD = scan_blocks(Dmax);
-
-    size_t need = Carve::need(R * sizeof(KhReq)) + Carve::need((R + 1) * 8) * 4 +
-                  Carve::need(S * 4 + 4) + Carve::need(Imax * 4) * 6 + Carve::need((Imax + 1) * 8) +
-                  Carve::need(Dmax * 4) + Carve::need((Dmax + 1) * 8) +
-                  Carve::need(std::max(nbI, nbD) * 8) + Carve::need(64 * 8);
-    hipError_t e = ws.scratch.reserve(need);
+    hipError_t e = ws.scratch.reserve(kh_scratch_need(R, S, Imax, Dmax));
     if (e != hipSuccess) return GRASS_ENOMEM;
     Carve cv(ws.scratch.ptr);
     KhReq* d_req = cv.take<KhReq>(R);
@@ -679,20 +720,9 @@ int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t R, uin
 
     // Dedup table: 8-byte packed slots when (request, vertex, position) fit one word, else the
     // 16-byte generation-tagged table.  GRASS_KHOP_SLOT=16 forces the latter.
-    auto bits = [](uint64_t x) {
-        uint32_t k = 0;
-        while (k < 64 && (x >> k) != 0) ++k;
-        return k;
-    };
-    const uint32_t vb = std::max<uint32_t>(bits(g->n ? g->n - 1 : 0), 1u);
-    const uint32_t pb = std::max<uint32_t>(bits(std::max(S, Dmax)), 1u);  // positions < 2^pb - 1
-    const char* force16 = getenv("GRASS_KHOP_SLOT");
-    const bool packed = bits(R - 1) + vb + pb <= 64 && !(force16 && strcmp(force16, "16") == 0);
-    auto pack_slots = [](uint64_t entries) {  // load <= 2/3
-        uint64_t slots = 1024;
-        while (slots < entries + entries / 2 + 2) slots <<= 1;
-        return slots;
-    };
+    const uint32_t vb = std::max<uint32_t>(bit_width(g->n ? g->n - 1 : 0), 1u);
+    const uint32_t pb = std::max<uint32_t>(bit_width(std::max(S, Dmax)), 1u);  // positions < 2^pb - 1
+    const bool packed = kh_packed_table(g, R, S, Dmax);
     HashTab ht{};
     if (packed) {
         if (ws.ht.reserve(pack_slots(std::max(S, Dmax)) * 8) != hipSuccess) return GRASS_ENOMEM;

@@ -267,6 +267,8 @@ int random_walk_batch(grass_graph* g, const grass_rw_request* reqs, uint32_t cou
 // before the stream has passed the batch.
 int rw_batch_enqueue(grass_graph* g, const grass_rw_request* reqs, uint32_t count, uint64_t seed,
                      hipStream_t stream, Workspace& ws, unsigned long long* bytes_dev);
+// Sizes a workspace for any walk batch of at most R requests (as khop_reserve).
+int rw_reserve(Workspace& ws, uint32_t R);
 // grass_random_walk_batch_noalloc: enqueues only, with the library's fixed device array
 int random_walk_batch_noalloc(grass_graph* g, const grass_rw_request* reqs, uint32_t count,
                               uint64_t seed, hipStream_t stream);
@@ -281,6 +283,10 @@ int khop_batch(grass_graph* g, const grass_khop_request* reqs, uint32_t count, u
 
 int khop_enqueue(grass_graph* g, const grass_khop_request* reqs, uint32_t count, uint64_t seed,
                  grass_khop_output* out, hipStream_t stream, Workspace& ws);
+// Sizes a workspace ahead of time so that no later batch of at most R requests whose seed total,
+// frontier bound and dedup bound (khop_bound's items / dedup) are at most S, I and D grows it: a
+// growth frees and reallocates memory, which waits for the whole device.
+int khop_reserve(const grass_graph* g, Workspace& ws, uint32_t R, uint64_t S, uint64_t I, uint64_t D);
 
 // mdrw.hip: an MDRW batch kept for capture.  _prepare runs it once eagerly on `stream` (which
 // sizes its outputs and working memory); _enqueue enqueues the same batch again into those

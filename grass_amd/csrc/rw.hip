@@ -250,6 +250,13 @@ static void launch_rw_batch(grass_graph* g, hipStream_t s, const RwDesc* d, uint
     else launch_rw_batch_stage<FMT, COUNT, 32>(g, s, d, R, gran, claim, queue, seed, err, bytes);
 }
 
+int rw_reserve(Workspace& ws, uint32_t R) {
+    if (ws.reserve_stage(R * sizeof(RwDesc)) != hipSuccess ||
+        ws.scratch.reserve(Carve::need(R * sizeof(RwDesc)) + Carve::need(64)) != hipSuccess)
+        return GRASS_ENOMEM;
+    return GRASS_OK;
+}
+
 int rw_batch_enqueue(grass_graph* g, const grass_rw_request* reqs, uint32_t count, uint64_t seed,
                      hipStream_t s, Workspace& ws, unsigned long long* bytes_dev) {
     if (count == 0) return GRASS_OK;

@@ -554,6 +554,22 @@ struct grass_session {
     }
 };
 
+// Sizes a slot's working memory now for the largest group the edge bound admits (seeds, frontier
+// and dedup bounds are each at most the group's edge bound when every fanout is at least 1; at
+// most kReserveEdges of them), so that serving never grows it: a growth frees and reallocates
+// device or pinned memory, and the free waits for the whole device, which stalls every request
+// in flight for milliseconds.  Best effort: where memory is short the slot keeps what it got and
+// a group beyond it grows its buffers as it would have without this.
+static constexpr uint64_t kReserveEdges = 8ull << 20;
+static void reserve_slot(const grass_graph* g, const grass_session_config& cfg, Slot& sl) {
+    const uint64_t E = std::min<uint64_t>(cfg.max_group_edges, kReserveEdges);
+    const uint32_t B = cfg.max_batch;
+    if (rw_reserve(sl.ws_rw, B) != GRASS_OK || khop_reserve(g, sl.ws_kh, B, E, E, E) != GRASS_OK ||
+        sl.stage.reserve(E * 4) != hipSuccess ||
+        sl.offs.reserve((size_t)GRASS_KHOP_MAX_HOPS * (B + 1) * 8) != hipSuccess)
+        (void)hipGetLastError();  // a failed allocation must not read as a later kernel's error
+}
+
 extern "C" {
 
 int grass_session_create(grass_graph* g, const grass_session_config* cfg, grass_session** out) {
@@ -591,6 +607,8 @@ int grass_session_create(grass_graph* g, const grass_session_config* cfg, grass_
             rc = GRASS_ENOMEM;
         else if (hipMemset(sl.ws_rw.err, 0, 64) != hipSuccess || hipMemset(sl.ws_kh.err, 0, 64) != hipSuccess)
             rc = GRASS_EDEVICE;
+        else
+            reserve_slot(g, s->cfg, sl);
     }
     (void)hipSetDevice(prev);
     if (rc) {
@@ -771,6 +789,12 @@ int grass_session_replay(grass_session* s, const grass_request* reqs, const doub
                          uint32_t count, double* latency_us, double* queue_us, double* wall_us) {
     if (!s || (count && (!reqs || !arrival_us || !latency_us))) return GRASS_EINVAL;
     std::vector<uint64_t> tickets(count, 0);
+    {
+        // every ticket of the replay stays in the map until the submit loop ends: size it now, or
+        // its rehashes (under the session lock, about 10 ms at 85 k tickets) stall the worker
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->reqs.reserve(s->reqs.size() + count);
+    }
     const double base = s->now_us();
     for (uint32_t i = 0; i < count; ++i) {
         const double due = base + arrival_us[i];

@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "philox.h"
@@ -159,10 +160,23 @@ int grass_autotune(grass_graph* g, uint32_t what, grass_tuning* chosen, grass_tu
     }
 
     if (!rc && (what & GRASS_TUNE_KHOP)) {
-        // 128 2-hop [25,10] requests of 400 uniform seeds
-        const uint32_t R = 128, S = 400;
-        std::vector<uint32_t> seeds((size_t)R * S);
-        for (uint32_t i = 0; i < R * S; ++i) seeds[i] = (uint32_t)uniform(kTuneSeed, i / S, i % S, kTagStart, 1, n);
+        // 256 2-hop [25,10] requests whose seed counts are lognormal(6, 1.2) clipped to [1, 8192]
+        // (about 212 k seeds in all): which lane layout wins depends on how full the sampling
+        // groups are, so the batch has the seed-count spread of the requests it tunes for.  The
+        // count is exp(6 + 1.2 z), z normal by Box-Muller from two Philox uniforms; seeds uniform.
+        const uint32_t R = 256;
+        std::vector<uint64_t> first(R + 1, 0);
+        for (uint32_t i = 0; i < R; ++i) {
+            const U4 u = philox4x32_10(i, 0, kTagStart, 2, kTuneSeed);
+            const double u1 = ((double)u.x + 1.0) / 4294967296.0, u2 = (double)u.y / 4294967296.0;
+            const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+            const double c = std::round(std::exp(6.0 + 1.2 * z));
+            first[i + 1] = first[i] + (uint64_t)std::min(std::max(c, 1.0), 8192.0);
+        }
+        std::vector<uint32_t> seeds(first[R]);
+        for (uint32_t i = 0; i < R; ++i)
+            for (uint64_t j = first[i]; j < first[i + 1]; ++j)
+                seeds[j] = (uint32_t)uniform(kTuneSeed, i, (uint32_t)(j - first[i]), kTagStart, 1, n);
         DevMem d_seeds, d_out;
         std::vector<grass_khop_request> reqs(R);
         if (hipMalloc(&d_seeds.p, seeds.size() * 4) != hipSuccess ||
@@ -171,8 +185,8 @@ int grass_autotune(grass_graph* g, uint32_t what, grass_tuning* chosen, grass_tu
         for (uint32_t i = 0; i < R && !rc; ++i) {
             grass_khop_request& q = reqs[i];
             memset(&q, 0, sizeof(q));
-            q.seeds_dev = (const uint32_t*)d_seeds.p + (size_t)i * S;
-            q.num_seeds = S;
+            q.seeds_dev = (const uint32_t*)d_seeds.p + first[i];
+            q.num_seeds = (uint32_t)(first[i + 1] - first[i]);
             q.instance = i;
             q.hops = 2;
             q.fanout[0] = 25;

@@ -293,8 +293,9 @@ typedef struct {
 
 /* Times each candidate of the selected knobs (what = a mask of GRASS_TUNE_*, 0 = all) on
  * synthetic requests drawn from the graph with Philox -- 128 random-walk requests of 2048
- * walkers and lengths 20-80; 128 2-hop [25,10] requests of 400 seeds; one full decode (skipped
- * when its CSR does not fit in free HBM) -- keeps the fastest in the graph and returns it.  A
+ * walkers and lengths 20-80; 256 2-hop [25,10] requests of lognormal(6, 1.2) seed counts clipped
+ * to [1, 8192]; one full decode (skipped when its CSR does not fit in free HBM) -- keeps the
+ * fastest in the graph and returns it.  A
  * candidate other than the setting in force at the call must be 3 % faster than that setting to
  * replace it (a near tie keeps the setting).  Synchronises; uses the graph's own stream.  report
  * may be NULL. */
@@ -337,7 +338,10 @@ int grass_plan_mdrw_output(const grass_plan *p, grass_mdrw_output *out);
  * as soon as HBM for their inputs and outputs is free in the session's arena, grouped into
  * launch groups (one persistent random-walk launch + one k-hop pipeline) and put on one of
  * `max_inflight` streams, so groups overlap on the device.  A request's result depends only on
- * its contents, its instance and the session seed -- never on what it was batched with. */
+ * its contents, its instance and the session seed -- never on what it was batched with.  Beside
+ * the arena, each of the `max_inflight` slots holds working memory sized at creation for the
+ * largest group `max_group_edges` admits (up to 8M edges; about 0.65 GB of device and 36 MB of
+ * pinned host memory per slot at 6M), so groups never reallocate it while serving. */
 
 typedef enum { GRASS_REQ_RW = 0, GRASS_REQ_KHOP = 1 } grass_request_kind;
 
